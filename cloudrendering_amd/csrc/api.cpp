@@ -1,0 +1,1314 @@
+// Host side of the gfx950 volumetric path tracer: the process-global renderer behind the crvpt C ABI.
+//
+// The renderer keeps the reference's settings semantics (src/PathTracer/VolumetricPathTracingPass.cpp,
+// src/PyTorch/Module.cpp, src/PyTorch/VolumetricPathTracingModuleRenderer.cpp): every setting that changes the
+// image resets the progressive accumulation; render_frame adds frame_count samples per pixel to the running mean.
+// Device work runs on one HIP stream; scene data (grid, supervoxel tables, environment image) stays resident in
+// HBM until the scene or a setting that shapes it changes.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "../../include/crvpt.h"
+#include "nanovdb_build.hpp"
+#include "render_api.hpp"
+#include "vpt_types.h"
+
+namespace crvpt {
+namespace {
+
+thread_local std::string g_lastError;
+
+int fail(int code, const std::string& msg) {
+    g_lastError = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess) return fail(CRVPT_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+// ------------------------------------------------------------------------------------------------------------
+// Small double-precision matrix helpers (column-major 4x4 like glm).
+// ------------------------------------------------------------------------------------------------------------
+struct Mat4 {
+    double m[16];
+};
+
+Mat4 mul(const Mat4& a, const Mat4& b) {
+    Mat4 r;
+    for (int c = 0; c < 4; c++)
+        for (int row = 0; row < 4; row++) {
+            double s = 0;
+            for (int k = 0; k < 4; k++) s += a.m[k * 4 + row] * b.m[c * 4 + k];
+            r.m[c * 4 + row] = s;
+        }
+    return r;
+}
+
+Mat4 inverse(const Mat4& in) {
+    double a[4][8];
+    for (int r = 0; r < 4; r++)
+        for (int c = 0; c < 4; c++) {
+            a[r][c] = in.m[c * 4 + r];
+            a[r][c + 4] = r == c ? 1.0 : 0.0;
+        }
+    for (int col = 0; col < 4; col++) {
+        int piv = col;
+        for (int r = col + 1; r < 4; r++)
+            if (std::fabs(a[r][col]) > std::fabs(a[piv][col])) piv = r;
+        for (int c = 0; c < 8; c++) std::swap(a[col][c], a[piv][c]);
+        const double p = a[col][col];
+        for (int c = 0; c < 8; c++) a[col][c] /= p;
+        for (int r = 0; r < 4; r++) {
+            if (r == col) continue;
+            const double f = a[r][col];
+            for (int c = 0; c < 8; c++) a[r][c] -= f * a[col][c];
+        }
+    }
+    Mat4 o;
+    for (int r = 0; r < 4; r++)
+        for (int c = 0; c < 4; c++) o.m[c * 4 + r] = a[r][c + 4];
+    return o;
+}
+
+// Right-handed look-at view matrix.
+Mat4 lookAt(const double eye[3], const double center[3], const double up[3]) {
+    double f[3] = {center[0] - eye[0], center[1] - eye[1], center[2] - eye[2]};
+    double fl = std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+    for (double& v : f) v /= fl;
+    double s[3] = {f[1] * up[2] - f[2] * up[1], f[2] * up[0] - f[0] * up[2], f[0] * up[1] - f[1] * up[0]};
+    double sl = std::sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+    for (double& v : s) v /= sl;
+    double u[3] = {s[1] * f[2] - s[2] * f[1], s[2] * f[0] - s[0] * f[2], s[0] * f[1] - s[1] * f[0]};
+    Mat4 V{};
+    V.m[0] = s[0]; V.m[4] = s[1]; V.m[8] = s[2];
+    V.m[1] = u[0]; V.m[5] = u[1]; V.m[9] = u[2];
+    V.m[2] = -f[0]; V.m[6] = -f[1]; V.m[10] = -f[2];
+    V.m[12] = -(s[0] * eye[0] + s[1] * eye[1] + s[2] * eye[2]);
+    V.m[13] = -(u[0] * eye[0] + u[1] * eye[1] + u[2] * eye[2]);
+    V.m[14] = f[0] * eye[0] + f[1] * eye[1] + f[2] * eye[2];
+    V.m[15] = 1.0;
+    return V;
+}
+
+// Right-handed perspective with depth in [0, 1] and y flipped so that image row 0 is the top of the view.
+Mat4 perspective(double fovy, double aspect, double zn, double zf) {
+    const double th = std::tan(fovy / 2.0);
+    Mat4 P{};
+    P.m[0] = 1.0 / (aspect * th);
+    P.m[5] = -1.0 / th;
+    P.m[10] = zf / (zn - zf);
+    P.m[11] = -1.0;
+    P.m[14] = -(zf * zn) / (zf - zn);
+    return P;
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// Device buffer
+// ------------------------------------------------------------------------------------------------------------
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t n) {
+        if (n <= bytes && ptr) return hipSuccess;
+        // A kernel on any stream may still read the old allocation.
+        if (ptr) (void)hipDeviceSynchronize();
+        release();
+        hipError_t e = hipMalloc(&ptr, n);
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            bytes = 0;
+            return e;
+        }
+        bytes = n;
+        return hipSuccess;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    template <class T>
+    T* as() const {
+        return reinterpret_cast<T*>(ptr);
+    }
+};
+
+const char* const kModeNames[] = {"Delta Tracking", "Delta Tracking (Spectral)", "Ratio Tracking",
+                                  "Decomposition Tracking", "Residual Ratio Tracking", "Next Event Tracking",
+                                  "Next Event Tracking (Spectral)", "Isosurfaces",
+                                  "Ray Marching (Emission/Absorption)"};
+
+struct Renderer {
+    // ---- settings (VolumetricPathTracingPass.hpp defaults; Module.cpp:797-798 sets delta tracking) ----
+    int mode = VPT_DELTA_TRACKING;
+    int interpolation = INTERP_STOCHASTIC;
+    double extinctionBase[3] = {1.0, 1.0, 1.0};
+    double extinctionScale = 1024.0;
+    double albedo[3] = {0.9, 0.9, 0.9};
+    double phaseG = 0.5;
+    double sunColor[3] = {1.0, 0.961538462, 0.884615385};
+    double sunIntensity = 2.6;
+    double sunDirection[3] = {0.5826, 0.7660, 0.2717};
+    int envKind = ENV_DEFAULT;
+    bool useEnvImage = false;
+    std::vector<float> envImage;  // RGBA32F
+    int envW = 0, envH = 0;
+    bool envImageBlack = false;
+    bool envFactorRgbMode = false;
+    double envFactor = 1.0;
+    double envFactorRgb[3] = {1.0, 1.0, 1.0};
+    int useLinearRGB = 1;
+    uint32_t seedOffset = 0;
+    bool useLights = false;
+    std::vector<Light> lights;
+    // camera (VolumetricPathTracingModuleRenderer.cpp:124-131)
+    double camPos[3] = {0.0, 0.0, 0.8};
+    double memberPos[3] = {0.0, 0.0, 0.0};
+    double memberTarget[3] = {0.0, 0.0, 0.0};
+    Mat4 view{};
+    double fovy = 2.0 * std::atan(0.5);
+    double znear = 1.0 / 512.0, zfar = 80.0;
+    double maxGridExtent = 0.25;
+    bool useSparse = false;
+    int denoiser = 0;  // 0 none, 1 EAW, 2 SVGF
+    int eawIterations = 0;
+    float eawPhiColor = 5.0f, eawPhiPosition = 0.1f, eawPhiNormal = 0.1f;
+    int eawUseColorDistance = 0;
+    bool useMajorantGrid = true;
+    int svSizeSetting = 0;
+    int blocksSetting = 0;
+    int stepsPerRefill = 4;
+    bool collectStats = false;
+
+    // ---- volume ----
+    int dims[3] = {0, 0, 0};
+    double voxelSize[3] = {1.0, 1.0, 1.0};
+    bool hasDense = false;
+    bool hasSparse = false;
+    bool denseOnHostValid = false;
+    std::vector<float> denseHost;
+    float vmin = 0.0f, vmax = 1.0f;
+    std::vector<uint8_t> nvdbHost;
+    NanoInfo nano{};
+    double boxMinDense[3], boxMaxDense[3], boxMinSparse[3], boxMaxSparse[3], gridMinSparse[3], gridMaxSparse[3];
+
+    // ---- device state ----
+    hipStream_t stream = nullptr;
+    DevBuf dRaw, dNorm, dNvdb, dMajorant, dRrt, dDecomp, dEnv, dAcc, dPaths, dCounter, dStage, dRays;
+    DevBuf dSvMin, dSvMax, dSvAvg, dDen0, dDen1, dStats;
+    bool sceneDirty = true;     // grid contents changed
+    bool tablesDirty = true;    // supervoxel tables must be rebuilt
+    bool envDirty = true;
+    int svSize = 8;
+    int svGrid[3] = {1, 1, 1};
+    int refSvSize = 8;
+    int refGrid[3] = {1, 1, 1};
+    bool refTablesValid = false;
+    int refTablesInterp = -1;
+    double refTablesExtinction = -1.0;
+    int cuCount = 256;
+
+    // ---- accumulation ----
+    int accW = 0, accH = 0, accC = 0;
+    uint32_t frameCount = 0;
+    uint32_t globalFrameNumber = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evT0 = nullptr, evT1 = nullptr;
+    double lastPathMs = 0.0, lastTotalMs = 0.0;
+    uint64_t lastPaths = 0;
+    int lastLaunches = 0;
+
+    void resetAccumulation() {
+        frameCount = 0;
+        globalFrameNumber = 0;
+    }
+};
+
+Renderer* g_r = nullptr;
+int g_refCount = 0;
+
+int requireInit() {
+    if (!g_r) return fail(CRVPT_ERR_NOT_INITIALIZED, "crvpt: renderer not initialised (call initialize first)");
+    return CRVPT_OK;
+}
+
+void computeDenseBounds(Renderer& r) {
+    // CloudData::computeGridBounds (CloudData.cpp:63-78)
+    const double sx = r.dims[0] * r.voxelSize[0], sy = r.dims[1] * r.voxelSize[1], sz = r.dims[2] * r.voxelSize[2];
+    const double maxSize = std::max(sx, std::max(sy, sz));
+    const float ext = float(r.maxGridExtent);
+    const float bx = float(sx) * ext / float(maxSize), by = float(sy) * ext / float(maxSize),
+                bz = float(sz) * ext / float(maxSize);
+    r.boxMaxDense[0] = bx; r.boxMaxDense[1] = by; r.boxMaxDense[2] = bz;
+    r.boxMinDense[0] = -bx; r.boxMinDense[1] = -by; r.boxMinDense[2] = -bz;
+}
+
+void computeSparseBounds(Renderer& r, bool fromDense) {
+    // CloudData::computeSparseGridMetadata (CloudData.cpp:1289-1358): world bbox of the grid, normalised into
+    // the +-maxGridExtent box unless the grid was built from the dense field (then world coordinates are kept).
+    for (int a = 0; a < 3; a++) {
+        r.gridMinSparse[a] = float(r.nano.worldBBoxMin[a]);
+        r.gridMaxSparse[a] = float(r.nano.worldBBoxMax[a]);
+    }
+    if (fromDense) {
+        for (int a = 0; a < 3; a++) {
+            r.boxMinSparse[a] = r.gridMinSparse[a];
+            r.boxMaxSparse[a] = r.gridMaxSparse[a];
+        }
+        return;
+    }
+    float diff[3], maxDim = 0.0f;
+    for (int a = 0; a < 3; a++) {
+        diff[a] = float(r.gridMaxSparse[a]) - float(r.gridMinSparse[a]);
+        maxDim = std::max(maxDim, diff[a]);
+    }
+    const float ext = float(r.maxGridExtent);
+    const float A = 2.0f * ext / maxDim;
+    for (int a = 0; a < 3; a++) {
+        const float L = diff[a] / maxDim;
+        const float B = ext * L - float(r.gridMaxSparse[a]) * A;
+        r.boxMinSparse[a] = A * float(r.gridMinSparse[a]) + B;
+        r.boxMaxSparse[a] = A * float(r.gridMaxSparse[a]) + B;
+    }
+}
+
+int setDenseHost(Renderer& r, int nx, int ny, int nz, std::vector<float>&& values) {
+    HIP_TRY(hipDeviceSynchronize());  // in-flight renders on any stream may read the grid
+    r.dims[0] = nx; r.dims[1] = ny; r.dims[2] = nz;
+    r.voxelSize[0] = r.voxelSize[1] = r.voxelSize[2] = 1.0;
+    r.denseHost = std::move(values);
+    r.denseOnHostValid = true;
+    float mn = 3.4e38f, mx = -3.4e38f;
+    for (float v : r.denseHost) {
+        mn = std::min(mn, v);
+        mx = std::max(mx, v);
+    }
+    if (mn == mx) mn = 0.0f;  // DensityField::computeMinMax (DensityField.cpp:149-152)
+    r.vmin = mn;
+    r.vmax = mx;
+    r.hasDense = true;
+    r.hasSparse = false;
+    r.nvdbHost.clear();
+    computeDenseBounds(r);
+    HIP_TRY(r.dRaw.ensure(r.denseHost.size() * sizeof(float)));
+    HIP_TRY(hipMemcpy(r.dRaw.ptr, r.denseHost.data(), r.denseHost.size() * sizeof(float), hipMemcpyHostToDevice));
+    r.sceneDirty = r.tablesDirty = true;
+    r.refTablesValid = false;
+    r.resetAccumulation();
+    return CRVPT_OK;
+}
+
+int setSparseHost(Renderer& r, std::vector<uint8_t>&& buf, bool fromDense) {
+    NanoInfo info;
+    std::string err;
+    if (!parseNanoVdb(buf.data(), buf.size(), info, err)) return fail(CRVPT_ERR_INVALID_ARGUMENT, err);
+    HIP_TRY(hipDeviceSynchronize());
+    r.nvdbHost = std::move(buf);
+    r.nano = info;
+    r.hasSparse = true;
+    if (!fromDense) {
+        r.hasDense = false;
+        r.denseOnHostValid = false;
+        for (int a = 0; a < 3; a++) {
+            r.dims[a] = info.indexBBoxMax[a] - info.indexBBoxMin[a] + 1;
+            r.voxelSize[a] = info.voxelSize[a];
+        }
+    }
+    computeSparseBounds(r, fromDense);
+    HIP_TRY(r.dNvdb.ensure(r.nvdbHost.size()));
+    HIP_TRY(hipMemcpy(r.dNvdb.ptr, r.nvdbHost.data(), r.nvdbHost.size(), hipMemcpyHostToDevice));
+    r.sceneDirty = r.tablesDirty = true;
+    r.refTablesValid = false;
+    r.resetAccumulation();
+    return CRVPT_OK;
+}
+
+// NanoVDB fog-volume sphere of the default scene (Module.cpp:793-795: radius 0.25, centre 0, voxel 0.01),
+// produced as a dense field: density 1 deeper than three voxels inside, a linear ramp to 0 across the last three
+// voxels, 0 outside.
+std::vector<float> fogSphere(int& n) {
+    const double radius = 0.25, voxel = 0.01, halfWidth = 3.0;
+    n = int(std::ceil(2.0 * radius / voxel)) + 2;
+    std::vector<float> v(size_t(n) * n * n);
+    const double c = 0.5 * n;
+    for (int z = 0; z < n; z++)
+        for (int y = 0; y < n; y++)
+            for (int x = 0; x < n; x++) {
+                const double dx = (x + 0.5 - c) * voxel, dy = (y + 0.5 - c) * voxel, dz = (z + 0.5 - c) * voxel;
+                const double sdf = std::sqrt(dx * dx + dy * dy + dz * dz) - radius;
+                double d = -sdf / (halfWidth * voxel);
+                d = std::min(std::max(d, 0.0), 1.0);
+                v[(size_t(z) * n + y) * n + x] = float(d);
+            }
+    return v;
+}
+
+bool sparseActive(const Renderer& r) { return r.useSparse && r.hasSparse; }
+
+// Builds the NanoVDB buffer from the dense field when the sparse grid is requested and missing
+// (CloudData::getSparseDensityField, CloudData.cpp:1367-1438).
+int ensureSparse(Renderer& r) {
+    if (!r.useSparse || r.hasSparse) return CRVPT_OK;
+    if (!r.hasDense) return fail(CRVPT_ERR_INVALID_ARGUMENT, "crvpt: no density field loaded");
+    if (!r.denseOnHostValid) {
+        r.denseHost.resize(size_t(r.dims[0]) * r.dims[1] * r.dims[2]);
+        HIP_TRY(hipMemcpy(r.denseHost.data(), r.dRaw.ptr, r.denseHost.size() * sizeof(float), hipMemcpyDeviceToHost));
+        r.denseOnHostValid = true;
+    }
+    std::vector<float> norm(r.denseHost.size());
+    const float mn = r.vmin, mx = r.vmax;
+    for (size_t i = 0; i < norm.size(); i++) norm[i] = (r.denseHost[i] - mn) / (mx - mn);
+    const double dx = double(float(r.boxMaxDense[0]) - float(r.boxMinDense[0])) / double(r.dims[0]);
+    const double origin[3] = {float(r.boxMinDense[0]), float(r.boxMinDense[1]), float(r.boxMinDense[2])};
+    std::vector<uint8_t> buf = buildNanoVdbFromDense(norm.data(), r.dims, dx, origin, "tmp");
+    const bool keepDense = r.hasDense;
+    int rc = setSparseHost(r, std::move(buf), true);
+    r.hasDense = keepDense;
+    return rc;
+}
+
+int uploadEnv(Renderer& r) {
+    if (!r.envDirty) return CRVPT_OK;
+    if (r.useEnvImage && !r.envImage.empty()) {
+        HIP_TRY(hipDeviceSynchronize());
+        HIP_TRY(r.dEnv.ensure(r.envImage.size() * sizeof(float)));
+        HIP_TRY(hipMemcpy(r.dEnv.ptr, r.envImage.data(), r.envImage.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+    r.envDirty = false;
+    return CRVPT_OK;
+}
+
+float extinctionX(const Renderer& r) { return float(r.extinctionBase[0]) * float(r.extinctionScale); }
+
+int chooseSvSize(const Renderer& r) {
+    if (r.svSizeSetting > 0) return r.svSizeSetting;
+    return sparseActive(r) ? 16 : 8;
+}
+
+// Grid views of the current volume.
+void gridViews(Renderer& r, DenseGrid& dg, NanoGrid& ng, bool& sparse) {
+    sparse = sparseActive(r);
+    dg.values = r.dNorm.as<float>();
+    for (int a = 0; a < 3; a++) dg.dim[a] = r.dims[a];
+    dg.borderValue = (0.0f - r.vmin) / (r.vmax - r.vmin);
+    ng.buf = r.dNvdb.as<uint8_t>();
+    ng.rootOffset = uint32_t(r.nano.rootOffset);
+    ng.tileCount = r.nano.tileCount;
+    for (int i = 0; i < 9; i++) ng.invMat[i] = r.nano.invMatF[i];
+    for (int i = 0; i < 3; i++) {
+        ng.vec[i] = r.nano.vecF[i];
+        ng.indexMin[i] = r.nano.indexBBoxMin[i];
+    }
+    ng.background = r.nano.background;
+}
+
+// Uploads the grid and rebuilds the supervoxel tables the current mode needs.
+int prepareScene(Renderer& r) {
+    if (!r.hasDense && !r.hasSparse) return fail(CRVPT_ERR_INVALID_ARGUMENT, "crvpt: no volume loaded");
+    int rc = ensureSparse(r);
+    if (rc) return rc;
+    rc = uploadEnv(r);
+    if (rc) return rc;
+    const bool sparse = sparseActive(r);
+    if (r.sceneDirty) {
+        if (!sparse) {
+            const size_t n = size_t(r.dims[0]) * r.dims[1] * r.dims[2];
+            HIP_TRY(r.dNorm.ensure(n * sizeof(float)));
+            HIP_TRY(launchNormalize(r.dRaw.as<float>(), r.dNorm.as<float>(), n, r.vmin, r.vmax, r.stream));
+        }
+        r.sceneDirty = false;
+        r.tablesDirty = true;
+    }
+    DenseGrid dg;
+    NanoGrid ng;
+    bool sp;
+    gridViews(r, dg, ng, sp);
+    if (r.tablesDirty) {
+        r.svSize = chooseSvSize(r);
+        for (int a = 0; a < 3; a++) r.svGrid[a] = (r.dims[a] + r.svSize - 1) / r.svSize;
+        const size_t n = size_t(r.svGrid[0]) * r.svGrid[1] * r.svGrid[2];
+        HIP_TRY(r.dMajorant.ensure(n * sizeof(float)));
+        HIP_TRY(launchMajorant(sp ? nullptr : &dg, sp ? &ng : nullptr, r.dims, r.svSize, r.svGrid,
+                               r.dMajorant.as<float>(), r.stream));
+        r.tablesDirty = false;
+        r.refTablesValid = false;
+    }
+    const bool needRef = r.mode == VPT_RESIDUAL_RATIO_TRACKING || r.mode == VPT_DECOMPOSITION_TRACKING;
+    if (needRef && (!r.refTablesValid || r.refTablesInterp != r.interpolation ||
+                    r.refTablesExtinction != double(extinctionX(r)))) {
+        // SuperVoxelGridResidualRatioTracking / ...DecompositionTracking constructors (SuperVoxelGrid.cpp:40-91,
+        // 269-405): edge 8, halved until it divides grids smaller than the edge.
+        int sv = 8;
+        if (r.dims[0] < sv || r.dims[1] < sv || r.dims[2] < sv) {
+            while (!(r.dims[0] % sv == 0 && r.dims[1] % sv == 0 && r.dims[2] % sv == 0)) sv /= 2;
+        }
+        r.refSvSize = sv;
+        for (int a = 0; a < 3; a++) r.refGrid[a] = (r.dims[a] + sv - 1) / sv;
+        const size_t n = size_t(r.refGrid[0]) * r.refGrid[1] * r.refGrid[2];
+        HIP_TRY(r.dSvMin.ensure(n * sizeof(float)));
+        HIP_TRY(r.dSvMax.ensure(n * sizeof(float)));
+        HIP_TRY(r.dSvAvg.ensure(n * sizeof(float)));
+        HIP_TRY(launchRefSupervoxelStats(sp ? nullptr : &dg, sp ? &ng : nullptr, r.dims, sv, r.refGrid,
+                                         r.interpolation == INTERP_NEAREST ? 0 : 1, r.dSvMin.as<float>(),
+                                         r.dSvMax.as<float>(), r.dSvAvg.as<float>(), r.stream));
+        std::vector<float> mn(n), mx(n), avg(n);
+        HIP_TRY(hipStreamSynchronize(r.stream));
+        HIP_TRY(hipMemcpy(mn.data(), r.dSvMin.ptr, n * sizeof(float), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(mx.data(), r.dSvMax.ptr, n * sizeof(float), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(avg.data(), r.dSvAvg.ptr, n * sizeof(float), hipMemcpyDeviceToHost));
+        // Host part of SuperVoxelGrid.cpp:180-256 (std::pow on the host as the reference does).
+        std::vector<float> rrt(2 * n), dec(2 * n);
+        const float ext = extinctionX(r);
+        const float gamma = 2.0f;
+        const float D = std::sqrt(3.0f) * float(sv);
+        for (size_t i = 0; i < n; i++) {
+            const int sx = int(i % r.refGrid[0]), sy = int((i / r.refGrid[0]) % r.refGrid[1]),
+                      sz = int(i / (size_t(r.refGrid[0]) * r.refGrid[1]));
+            dec[2 * i] = mn[i];
+            dec[2 * i + 1] = mx[i];
+            float dmin = mn[i];
+            if ((r.dims[0] % sv != 0 && sx == r.refGrid[0] - 1) || (r.dims[1] % sv != 0 && sy == r.refGrid[1] - 1) ||
+                (r.dims[2] % sv != 0 && sz == r.refGrid[2] - 1))
+                dmin = 0.0f;
+            const float muMin = dmin * ext, muMax = mx[i] * ext, muAvg = avg[i] * ext;
+            const float muR = std::max(muMax - muMin, 0.1f);
+            const float muC = muMin + muR * std::pow(gamma, (1.0f / (D * muR)) - 1.0f);
+            rrt[2 * i] = std::min(std::max(muC, muMin), muAvg);
+            rrt[2 * i + 1] = muR;
+        }
+        HIP_TRY(r.dRrt.ensure(2 * n * sizeof(float)));
+        HIP_TRY(r.dDecomp.ensure(2 * n * sizeof(float)));
+        HIP_TRY(hipMemcpy(r.dRrt.ptr, rrt.data(), 2 * n * sizeof(float), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(r.dDecomp.ptr, dec.data(), 2 * n * sizeof(float), hipMemcpyHostToDevice));
+        r.refTablesValid = true;
+        r.refTablesInterp = r.interpolation;
+        r.refTablesExtinction = double(ext);
+    }
+    return CRVPT_OK;
+}
+
+Mat4 projection(const Renderer& r, int w, int h) { return perspective(r.fovy, double(w) / double(h), r.znear, r.zfar); }
+
+void fillSceneParams(Renderer& r, int w, int h, SceneParams& P, SupervoxelGrid& S) {
+    std::memset(&P, 0, sizeof(P));
+    const Mat4 vp = mul(projection(r, w, h), r.view);
+    const Mat4 ivp = inverse(vp);
+    const Mat4 iv = inverse(r.view);
+    for (int i = 0; i < 16; i++) {
+        P.invViewProj[i] = float(ivp.m[i]);
+        P.invView[i] = float(iv.m[i]);
+    }
+    const bool sparse = sparseActive(r);
+    for (int a = 0; a < 3; a++) {
+        P.boxMin[a] = float(sparse ? r.boxMinSparse[a] : r.boxMinDense[a]);
+        P.boxMax[a] = float(sparse ? r.boxMaxSparse[a] : r.boxMaxDense[a]);
+        P.gridMin[a] = sparse ? float(r.gridMinSparse[a]) : 0.0f;
+        P.gridMax[a] = sparse ? float(r.gridMaxSparse[a]) : 1.0f;
+        P.worldToTex[a] = float(r.dims[a]) / (P.boxMax[a] - P.boxMin[a]);
+    }
+    P.extinction = extinctionX(r);
+    P.albedo = float(r.albedo[0]);
+    P.phaseG = float(r.phaseG);
+    P.mode = r.mode;
+    P.interpolation = r.interpolation;
+    P.useLinearRGB = r.useLinearRGB;
+    P.useMajorantGrid = r.useMajorantGrid ? 1 : 0;
+    // sun = sunlightIntensity * sunlightColor; direction normalised like glm::normalize of the default
+    double dl = std::sqrt(r.sunDirection[0] * r.sunDirection[0] + r.sunDirection[1] * r.sunDirection[1] +
+                          r.sunDirection[2] * r.sunDirection[2]);
+    for (int a = 0; a < 3; a++) {
+        P.sunDirection[a] = float(r.sunDirection[a] / dl);
+        P.sunIntensity[a] = float(r.sunIntensity * r.sunColor[a]);
+        P.envFactor[a] = float(r.envFactorRgbMode ? r.envFactorRgb[a] : r.envFactor);
+    }
+    if (r.useEnvImage && !r.envImage.empty()) {
+        P.envKind = ENV_IMAGE;
+        P.envImage = r.dEnv.as<float>();
+        P.envW = r.envW;
+        P.envH = r.envH;
+        // VolumetricPathTracingPass.cpp:2114-2131
+        bool black = r.envImageBlack;
+        if (r.envFactorRgbMode) {
+            if (r.envFactorRgb[0] < 1e-5 && r.envFactorRgb[1] < 1e-5 && r.envFactorRgb[2] < 1e-5) black = true;
+        } else if (r.envFactor < 1e-5) {
+            black = true;
+        }
+        P.isEnvMapBlack = black ? 1u : 0u;
+    } else {
+        P.envKind = r.envKind;
+        P.isEnvMapBlack = r.envKind == ENV_BLACK ? 1u : 0u;
+    }
+    P.numLights = r.useLights ? int(std::min<size_t>(r.lights.size(), kMaxLights)) : 0;
+    for (int i = 0; i < P.numLights; i++) P.lights[i] = r.lights[i];
+    P.sparse = sparse ? 1 : 0;
+    P.seedOffset = r.seedOffset;
+    P.width = w;
+    P.height = h;
+    S.majorant = r.dMajorant.as<float>();
+    S.rrt = r.dRrt.as<float>();
+    S.decomp = r.dDecomp.as<float>();
+    S.size = r.svSize;
+    S.refSize = r.refSvSize;
+    for (int a = 0; a < 3; a++) {
+        S.grid[a] = r.svGrid[a];
+        S.refGrid[a] = r.refGrid[a];
+    }
+}
+
+int launchBlocks(Renderer& r) {
+    if (r.blocksSetting > 0) return r.blocksSetting;
+    return r.cuCount * occupancyBlocksPerCU(r.mode, sparseActive(r));
+}
+
+std::string trim(const std::string& s) {
+    size_t a = s.find_first_not_of(" \t"), b = s.find_last_not_of(" \t");
+    return a == std::string::npos ? std::string() : s.substr(a, b - a + 1);
+}
+
+bool parseVec3(const std::string& s, float out[3]) {
+    std::vector<float> v;
+    size_t i = 0;
+    while (i < s.size()) {
+        while (i < s.size() && (s[i] == ' ' || s[i] == ',' || s[i] == '\t')) i++;
+        if (i >= s.size()) break;
+        size_t j = i;
+        while (j < s.size() && s[j] != ' ' && s[j] != ',' && s[j] != '\t') j++;
+        try {
+            v.push_back(std::stof(s.substr(i, j - i)));
+        } catch (...) {
+            return false;
+        }
+        i = j;
+    }
+    if (v.size() != 3) return false;
+    for (int a = 0; a < 3; a++) out[a] = v[a];
+    return true;
+}
+
+Light defaultLight() {
+    // Light defaults (LightEditorWidget.hpp:69-85)
+    Light L{};
+    L.lightType = 0;
+    L.lightSpace = 1;
+    L.spotTotalWidth = 0.0981747704f;
+    L.spotFalloffStart = 0.0245436926f;
+    L.color[0] = 1.0f; L.color[1] = 0.961538462f; L.color[2] = 0.884615385f;
+    L.intensity = 0.5f;
+    L.useDistance = 1u;
+    L.spotDirection[2] = -1.0f;
+    return L;
+}
+
+}  // namespace
+}  // namespace crvpt
+
+using namespace crvpt;
+
+extern "C" {
+
+const char* crvpt_last_error(void) { return g_lastError.c_str(); }
+
+int crvpt_initialize(void) {
+    if (g_refCount == 0) {
+        int ndev = 0;
+        hipError_t e = hipGetDeviceCount(&ndev);
+        if (e != hipSuccess || ndev == 0) return fail(CRVPT_ERR_HIP, "crvpt: no HIP device available");
+        Renderer* r = new Renderer();
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, 0) == hipSuccess) r->cuCount = prop.multiProcessorCount;
+        HIP_TRY(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreate(&r->ev0));
+        HIP_TRY(hipEventCreate(&r->ev1));
+        HIP_TRY(hipEventCreate(&r->evT0));
+        HIP_TRY(hipEventCreate(&r->evT1));
+        const double eye[3] = {0.0, 0.0, 0.8}, ctr[3] = {0.0, 0.0, -0.2}, up[3] = {0.0, 1.0, 0.0};
+        r->view = lookAt(eye, ctr, up);
+        // LightEditorWidget constructor (LightEditorWidget.cpp:51-63): one white point light, view orientation.
+        Light l0 = defaultLight();
+        l0.color[0] = l0.color[1] = l0.color[2] = 1.0f;
+        l0.intensity = 1.0f;
+        l0.position[0] = 0.68f; l0.position[1] = 0.22f; l0.position[2] = 0.70f;
+        l0.lightSpace = 2;
+        r->lights.push_back(l0);
+        g_r = r;
+        int n = 0;
+        std::vector<float> sphere = fogSphere(n);
+        int rc = setDenseHost(*r, n, n, n, std::move(sphere));
+        if (rc) return rc;
+    }
+    g_refCount++;
+    return CRVPT_OK;
+}
+
+int crvpt_cleanup(void) {
+    if (g_refCount <= 0) return fail(CRVPT_ERR_NOT_INITIALIZED, "crvpt: cleanup without initialize");
+    g_refCount--;
+    if (g_refCount == 0 && g_r) {
+        Renderer* r = g_r;
+        (void)hipStreamSynchronize(r->stream);
+        for (DevBuf* b : {&r->dRaw, &r->dNorm, &r->dNvdb, &r->dMajorant, &r->dRrt, &r->dDecomp, &r->dEnv, &r->dAcc,
+                          &r->dPaths, &r->dCounter, &r->dStage, &r->dRays, &r->dSvMin, &r->dSvMax, &r->dSvAvg,
+                          &r->dDen0, &r->dDen1, &r->dStats})
+            b->release();
+        (void)hipEventDestroy(r->ev0);
+        (void)hipEventDestroy(r->ev1);
+        (void)hipEventDestroy(r->evT0);
+        (void)hipEventDestroy(r->evT1);
+        (void)hipStreamDestroy(r->stream);
+        delete r;
+        g_r = nullptr;
+    }
+    return CRVPT_OK;
+}
+
+int crvpt_render_frame(float* out, int channels, int height, int width, int frame_count, int out_on_device,
+                       void* stream) {
+    if (int rc = requireInit()) return rc;
+    // Module.cpp:287-300
+    if (channels != 3 && channels != 4)
+        return fail(CRVPT_ERR_INVALID_ARGUMENT,
+                    "Error in renderFrame: The number of image channels is not equal to 3 or 4.");
+    if (height <= 0 || width <= 0) return fail(CRVPT_ERR_INVALID_ARGUMENT, "Error in renderFrame: empty image.");
+    if (frame_count < 0) return fail(CRVPT_ERR_INVALID_ARGUMENT, "Error in renderFrame: negative frame count.");
+    if (!out) return fail(CRVPT_ERR_INVALID_ARGUMENT, "Error in renderFrame: null output.");
+    Renderer& r = *g_r;
+    hipStream_t s = out_on_device ? reinterpret_cast<hipStream_t>(stream) : r.stream;
+    if (int rc = prepareScene(r)) return rc;
+    const uint32_t npix = uint32_t(width) * uint32_t(height);
+    if (width != r.accW || height != r.accH || channels != r.accC) {
+        HIP_TRY(r.dAcc.ensure(size_t(npix) * sizeof(float4)));
+        r.accW = width;
+        r.accH = height;
+        r.accC = channels;
+        r.resetAccumulation();
+    }
+    // Scene preparation ran on the renderer stream; order it before work on the caller's stream.
+    if (s != r.stream) HIP_TRY(hipStreamSynchronize(r.stream));
+    SceneParams P;
+    SupervoxelGrid S;
+    fillSceneParams(r, width, height, P, S);
+    DenseGrid dg;
+    NanoGrid ng;
+    bool sp;
+    gridViews(r, dg, ng, sp);
+    HIP_TRY(r.dCounter.ensure(64));
+    unsigned long long* stats = nullptr;
+    if (r.collectStats) {
+        HIP_TRY(r.dStats.ensure(4 * sizeof(unsigned long long)));
+        HIP_TRY(hipMemsetAsync(r.dStats.ptr, 0, 4 * sizeof(unsigned long long), s));
+        stats = r.dStats.as<unsigned long long>();
+    }
+    const uint32_t maxPaths = 1u << 24;
+    const uint32_t framesPerLaunch = std::max(1u, std::min(uint32_t(std::max(frame_count, 1)), maxPaths / npix));
+    HIP_TRY(r.dPaths.ensure(size_t(npix) * framesPerLaunch * sizeof(float4)));
+    LaunchConfig cfg;
+    cfg.numBlocks = uint32_t(launchBlocks(r));
+    cfg.stepsPerRefill = r.stepsPerRefill;
+    const bool denoise = r.denoiser == 1 && r.eawIterations > 0;
+    float* directOut = (out_on_device && !denoise) ? out : nullptr;
+    if (!out_on_device) {
+        HIP_TRY(r.dStage.ensure(size_t(npix) * channels * sizeof(float)));
+        if (!denoise) directOut = r.dStage.as<float>();
+    }
+    int launches = 0;
+    HIP_TRY(hipEventRecord(r.evT0, s));
+    HIP_TRY(hipEventRecord(r.ev0, s));
+    uint32_t done = 0;
+    while (done < uint32_t(frame_count)) {
+        const uint32_t f = std::min(framesPerLaunch, uint32_t(frame_count) - done);
+        P.frameBase = r.globalFrameNumber + done;
+        P.numFrames = f;
+        P.numUnits = uint64_t(npix) * f;
+        HIP_TRY(launchRender(P, sp ? nullptr : &dg, sp ? &ng : nullptr, S, r.dPaths.as<float4>(),
+                             r.dCounter.as<uint32_t>(), cfg, stats, s));
+        const bool last = done + f == uint32_t(frame_count);
+        if (last) HIP_TRY(hipEventRecord(r.ev1, s));
+        HIP_TRY(launchAccumulate(r.dPaths.as<float4>(), npix, f, r.dAcc.as<float4>(), r.frameCount + done,
+                                 last ? directOut : nullptr, channels, s));
+        done += f;
+        launches++;
+    }
+    if (frame_count == 0) HIP_TRY(hipEventRecord(r.ev1, s));
+    if (frame_count == 0 || denoise) {
+        float* target = out_on_device ? out : r.dStage.as<float>();
+        if (frame_count == 0 && r.frameCount == 0) {
+            HIP_TRY(hipMemsetAsync(r.dAcc.ptr, 0, size_t(npix) * sizeof(float4), s));
+        }
+        if (denoise) {
+            HIP_TRY(r.dDen0.ensure(size_t(npix) * sizeof(float4)));
+            HIP_TRY(r.dDen1.ensure(size_t(npix) * sizeof(float4)));
+            EawParams ep{width, height, r.eawIterations, r.eawPhiColor, r.eawPhiPosition, r.eawPhiNormal,
+                         r.eawUseColorDistance};
+            float4* res = nullptr;
+            HIP_TRY(launchEaw(r.dAcc.as<float4>(), nullptr, nullptr, r.dDen0.as<float4>(), r.dDen1.as<float4>(), ep,
+                              &res, s));
+            HIP_TRY(launchWriteCHW(res, npix, target, channels, s));
+        } else {
+            HIP_TRY(launchWriteCHW(r.dAcc.as<float4>(), npix, target, channels, s));
+        }
+    }
+    HIP_TRY(hipEventRecord(r.evT1, s));
+    r.frameCount += uint32_t(frame_count);
+    r.globalFrameNumber += uint32_t(frame_count);
+    r.lastPaths = uint64_t(npix) * uint64_t(frame_count);
+    r.lastLaunches = launches;
+    if (!out_on_device) {
+        HIP_TRY(hipStreamSynchronize(s));
+        HIP_TRY(hipMemcpy(out, r.dStage.ptr, size_t(npix) * channels * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    return CRVPT_OK;
+}
+
+int crvpt_get_last_render_stats(double* path_kernel_ms, double* total_ms, uint64_t* num_paths, int* launches) {
+    if (int rc = requireInit()) return rc;
+    Renderer& r = *g_r;
+    float a = 0.0f, b = 0.0f;
+    HIP_TRY(hipEventSynchronize(r.evT1));
+    HIP_TRY(hipEventElapsedTime(&a, r.ev0, r.ev1));
+    HIP_TRY(hipEventElapsedTime(&b, r.evT0, r.evT1));
+    if (path_kernel_ms) *path_kernel_ms = a;
+    if (total_ms) *total_ms = b;
+    if (num_paths) *num_paths = r.lastPaths;
+    if (launches) *launches = r.lastLaunches;
+    return CRVPT_OK;
+}
+
+int crvpt_set_collect_stats(int collect) {
+    if (int rc = requireInit()) return rc;
+    g_r->collectStats = collect != 0;
+    return CRVPT_OK;
+}
+
+int crvpt_get_last_render_counters(uint64_t out[3]) {
+    if (int rc = requireInit()) return rc;
+    Renderer& r = *g_r;
+    if (!r.collectStats || !r.dStats.ptr) return fail(CRVPT_ERR_INVALID_ARGUMENT, "get_last_render_counters: "
+                                                      "enable crvpt_set_collect_stats before rendering");
+    unsigned long long v[3];
+    HIP_TRY(hipStreamSynchronize(r.stream));
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(v, r.dStats.ptr, sizeof(v), hipMemcpyDeviceToHost));
+    for (int i = 0; i < 3; i++) out[i] = v[i];
+    return CRVPT_OK;
+}
+
+int crvpt_load_cloud_file(const char* filename) {
+    if (int rc = requireInit()) return rc;
+    if (!filename) return fail(CRVPT_ERR_INVALID_ARGUMENT, "load_cloud_file: null file name");
+    std::string fn(filename);
+    std::ifstream f(fn, std::ios::binary);
+    if (!f) return fail(CRVPT_ERR_IO, "Error in CloudData::loadFromFile: File \"" + fn + "\" does not exist.");
+    std::vector<uint8_t> bytes((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    const std::string ext = fn.size() >= 4 ? fn.substr(fn.size() - 4) : "";
+    Renderer& r = *g_r;
+    if (ext == ".xyz") {
+        // CloudData::loadFromXyzFile (CloudData.cpp:236-316): u32 x3 sizes, f64 x3 voxel sizes, z-fastest floats.
+        if (bytes.size() < 36) return fail(CRVPT_ERR_IO, "load_cloud_file: truncated .xyz header");
+        uint32_t sz[3];
+        double vs[3];
+        std::memcpy(sz, bytes.data(), 12);
+        std::memcpy(vs, bytes.data() + 12, 24);
+        const size_t n = size_t(sz[0]) * sz[1] * sz[2];
+        if (bytes.size() < 36 + n * 4) return fail(CRVPT_ERR_IO, "load_cloud_file: truncated .xyz data");
+        const float* src = reinterpret_cast<const float*>(bytes.data() + 36);
+        std::vector<float> v(n);
+        for (uint32_t z = 0; z < sz[2]; z++)
+            for (uint32_t y = 0; y < sz[1]; y++)
+                for (uint32_t x = 0; x < sz[0]; x++)
+                    v[x + (y + size_t(z) * sz[1]) * sz[0]] = src[z + (y + size_t(x) * sz[1]) * sz[2]];
+        float mn = 0.0f, mx = -3.4e38f;  // sgl::reduceFloatArrayMinMax seeded with (0, lowest)
+        for (float x : v) {
+            mn = std::min(mn, x);
+            mx = std::max(mx, x);
+        }
+        if (mx - mn > 1e-6f)
+            for (float& x : v) x = (x - mn) / (mx - mn);
+        int rc = setDenseHost(r, int(sz[0]), int(sz[1]), int(sz[2]), std::move(v));
+        if (rc) return rc;
+        r.voxelSize[0] = float(vs[0]);
+        r.voxelSize[1] = float(vs[1]);
+        r.voxelSize[2] = float(vs[2]);
+        computeDenseBounds(r);
+        return CRVPT_OK;
+    }
+    if (ext == "nvdb") {
+        std::vector<uint8_t> grid;
+        std::string err;
+        if (!extractNanoVdbGridFromFile(bytes, grid, err)) return fail(CRVPT_ERR_IO, err);
+        r.useSparse = true;
+        return setSparseHost(r, std::move(grid), false);
+    }
+    return fail(CRVPT_ERR_UNSUPPORTED, "load_cloud_file: unsupported file format \"" + fn + "\"");
+}
+
+int crvpt_set_density_field(int nx, int ny, int nz, const float* values) {
+    if (int rc = requireInit()) return rc;
+    if (nx <= 0 || ny <= 0 || nz <= 0 || !values)
+        return fail(CRVPT_ERR_INVALID_ARGUMENT, "set_density_field: invalid grid size or null data");
+    std::vector<float> v(values, values + size_t(nx) * ny * nz);
+    return setDenseHost(*g_r, nx, ny, nz, std::move(v));
+}
+
+int crvpt_set_nanovdb_buffer(const uint8_t* data, uint64_t size) {
+    if (int rc = requireInit()) return rc;
+    if (!data || size == 0) return fail(CRVPT_ERR_INVALID_ARGUMENT, "set_nanovdb_buffer: empty buffer");
+    std::vector<uint8_t> b(data, data + size);
+    g_r->useSparse = true;
+    return setSparseHost(*g_r, std::move(b), false);
+}
+
+int crvpt_generate_fbm_cloud(int nx, int ny, int nz, uint32_t seed, int octaves, float lacunarity, float gain,
+                             float base_frequency, float edge0, float edge1, int height_falloff) {
+    if (int rc = requireInit()) return rc;
+    if (nx <= 0 || ny <= 0 || nz <= 0 || octaves <= 0 || !(edge1 > edge0))
+        return fail(CRVPT_ERR_INVALID_ARGUMENT, "generate_fbm_cloud: invalid parameters");
+    Renderer& r = *g_r;
+    const size_t n = size_t(nx) * ny * nz;
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(r.dRaw.ensure(n * sizeof(float)));
+    FbmParams F{seed, octaves, lacunarity, gain, base_frequency, edge0, edge1, height_falloff};
+    HIP_TRY(launchFbm(r.dRaw.as<float>(), nx, ny, nz, F, r.stream));
+    r.denseHost.resize(n);
+    HIP_TRY(hipStreamSynchronize(r.stream));
+    HIP_TRY(hipMemcpy(r.denseHost.data(), r.dRaw.ptr, n * sizeof(float), hipMemcpyDeviceToHost));
+    std::vector<float> v = std::move(r.denseHost);
+    return setDenseHost(r, nx, ny, nz, std::move(v));
+}
+
+int crvpt_get_density_field(float* out, uint64_t count, int normalized) {
+    if (int rc = requireInit()) return rc;
+    Renderer& r = *g_r;
+    if (!r.hasDense) return fail(CRVPT_ERR_INVALID_ARGUMENT, "get_density_field: no dense field");
+    const size_t n = size_t(r.dims[0]) * r.dims[1] * r.dims[2];
+    if (count < n || !out) return fail(CRVPT_ERR_INVALID_ARGUMENT, "get_density_field: output too small");
+    if (!r.denseOnHostValid) {
+        r.denseHost.resize(n);
+        HIP_TRY(hipMemcpy(r.denseHost.data(), r.dRaw.ptr, n * sizeof(float), hipMemcpyDeviceToHost));
+        r.denseOnHostValid = true;
+    }
+    for (size_t i = 0; i < n; i++) out[i] = normalized ? (r.denseHost[i] - r.vmin) / (r.vmax - r.vmin) : r.denseHost[i];
+    return CRVPT_OK;
+}
+
+int crvpt_get_nanovdb_buffer(uint8_t* out, uint64_t capacity, uint64_t* size) {
+    if (int rc = requireInit()) return rc;
+    Renderer& r = *g_r;
+    const bool prev = r.useSparse;
+    r.useSparse = true;
+    int rc = ensureSparse(r);
+    r.useSparse = prev;
+    if (rc) return rc;
+    if (size) *size = r.nvdbHost.size();
+    if (out) {
+        if (capacity < r.nvdbHost.size()) return fail(CRVPT_ERR_INVALID_ARGUMENT, "get_nanovdb_buffer: too small");
+        std::memcpy(out, r.nvdbHost.data(), r.nvdbHost.size());
+    }
+    return CRVPT_OK;
+}
+
+int crvpt_set_use_sparse_grid(int use_sparse) {
+    if (int rc = requireInit()) return rc;
+    g_r->useSparse = use_sparse != 0;
+    if (!g_r->useSparse && !g_r->hasDense)
+        return fail(CRVPT_ERR_UNSUPPORTED, "set_use_sparse_grid: the loaded volume has no dense representation");
+    g_r->sceneDirty = g_r->tablesDirty = true;
+    g_r->refTablesValid = false;
+    g_r->resetAccumulation();
+    return CRVPT_OK;
+}
+
+int crvpt_set_grid_interpolation_type(int type) {
+    if (int rc = requireInit()) return rc;
+    if (type < 0 || type > 2) return fail(CRVPT_ERR_INVALID_ARGUMENT, "set_grid_interpolation_type: invalid type");
+    g_r->interpolation = type;
+    g_r->refTablesValid = false;
+    g_r->resetAccumulation();
+    return CRVPT_OK;
+}
+
+int crvpt_set_vpt_mode(int64_t mode) {
+    if (int rc = requireInit()) return rc;
+    if (mode != VPT_DELTA_TRACKING && mode != VPT_RATIO_TRACKING && mode != VPT_DECOMPOSITION_TRACKING &&
+        mode != VPT_RESIDUAL_RATIO_TRACKING && mode != VPT_NEXT_EVENT_TRACKING)
+        return fail(CRVPT_ERR_UNSUPPORTED, "set_vpt_mode: mode " + std::to_string(mode) +
+                                               " is not on the gfx950 hot path (supported: 0, 2, 3, 4, 5)");
+    g_r->mode = int(mode);
+    g_r->resetAccumulation();
+    return CRVPT_OK;
+}
+
+int crvpt_set_vpt_mode_from_name(const char* mode_name) {
+    if (int rc = requireInit()) return rc;
+    if (!mode_name) return fail(CRVPT_ERR_INVALID_ARGUMENT, "set_vpt_mode_from_name: null name");
+    for (int m = 0; m < 9; m++)
+        if (std::string(mode_name) == kModeNames[m]) return crvpt_set_vpt_mode(m);
+    // Module.cpp:1046-1055 ignores unknown names; the C ABI reports them.
+    return fail(CRVPT_ERR_INVALID_ARGUMENT, std::string("set_vpt_mode_from_name: unknown mode \"") + mode_name + "\"");
+}
+
+int crvpt_set_scattering_albedo(const double albedo[3]) {
+    if (int rc = requireInit()) return rc;
+    for (int a = 0; a < 3; a++) g_r->albedo[a] = float(albedo[a]);
+    g_r->resetAccumulation();
+    return CRVPT_OK;
+}
+
+int crvpt_set_extinction_scale(double s) {
+    if (int rc = requireInit()) return rc;
+    g_r->extinctionScale = float(s);
+    g_r->resetAccumulation();
+    return CRVPT_OK;
+}
+
+int crvpt_set_extinction_base(const double b[3]) {
+    if (int rc = requireInit()) return rc;
+    for (int a = 0; a < 3; a++) g_r->extinctionBase[a] = float(b[a]);
+    g_r->resetAccumulation();
+    return CRVPT_OK;
+}
+
+int crvpt_set_phase_g(double g) {
+    if (int rc = requireInit()) return rc;
+    g_r->phaseG = float(g);
+    g_r->resetAccumulation();
+    return CRVPT_OK;
+}
+
+int crvpt_set_seed_offset(int64_t offset) {
+    if (int rc = requireInit()) return rc;
+    g_r->seedOffset = uint32_t(offset);
+    g_r->resetAccumulation();
+    return CRVPT_OK;
+}
+
+int crvpt_get_camera_position(double out[3]) {
+    if (int rc = requireInit()) return rc;
+    for (int a = 0; a < 3; a++) out[a] = g_r->camPos[a];
+    return CRVPT_OK;
+}
+
+int crvpt_get_camera_view_matrix(double out[16]) {
+    if (int rc = requireInit()) return rc;
+    for (int i = 0; i < 16; i++) out[i] = float(g_r->view.m[i]);
+    return CRVPT_OK;
+}
+
+int crvpt_get_camera_fovy(double* out) {
+    if (int rc = requireInit()) return rc;
+    *out = g_r->fovy;
+    return CRVPT_OK;
+}
+
+int crvpt_set_camera_position(const double p[3]) {
+    if (int rc = requireInit()) return rc;
+    Renderer& r = *g_r;
+    const double up[3] = {0.0, 1.0, 0.0};
+    for (int a = 0; a < 3; a++) r.memberPos[a] = r.camPos[a] = float(p[a]);
+    r.view = lookAt(r.memberPos, r.memberTarget, up);
+    r.resetAccumulation();
+    return CRVPT_OK;
+}
+
+int crvpt_set_camera_target(const double t[3]) {
+    if (int rc = requireInit()) return rc;
+    Renderer& r = *g_r;
+    const double up[3] = {0.0, 1.0, 0.0};
+    for (int a = 0; a < 3; a++) r.memberTarget[a] = float(t[a]);
+    r.view = lookAt(r.memberPos, r.memberTarget, up);
+    r.resetAccumulation();
+    return CRVPT_OK;
+}
+
+int crvpt_overwrite_camera_view_matrix(const double v[16]) {
+    if (int rc = requireInit()) return rc;
+    for (int i = 0; i < 16; i++) g_r->view.m[i] = float(v[i]);
+    g_r->resetAccumulation();
+    return CRVPT_OK;
+}
+
+int crvpt_set_camera_fovy(double fovy_degrees) {
+    if (int rc = requireInit()) return rc;
+    g_r->fovy = fovy_degrees * 3.14159265358979323846 / 180.0;
+    g_r->resetAccumulation();
+    return CRVPT_OK;
+}
+
+int crvpt_set_use_builtin_environment_map(const char* name) {
+    if (int rc = requireInit()) return rc;
+    const std::string n = name ? name : "";
+    Renderer& r = *g_r;
+    r.useEnvImage = false;
+    if (n == "Default") r.envKind = ENV_DEFAULT;
+    else if (n == "Black") r.envKind = ENV_BLACK;
+    else if (n == "Single Color") r.envKind = ENV_SINGLE_COLOR;
+    else return fail(CRVPT_ERR_INVALID_ARGUMENT, "set_use_builtin_environment_map: unknown map \"" + n + "\"");
+    r.resetAccumulation();
+    return CRVPT_OK;
+}
+
+int crvpt_load_environment_map_from_buffer(const float* px, int channels, int height, int width) {
+    if (int rc = requireInit()) return rc;
+    if (!px || (channels != 3 && channels != 4) || height <= 0 || width <= 0)
+        return fail(CRVPT_ERR_INVALID_ARGUMENT,
+                    "Error in loadEnvironmentMapFromTensor: Number of channels must be 3 or 4.");
+    Renderer& r = *g_r;
+    const size_t npix = size_t(width) * height;
+    r.envImage.assign(npix * 4, 1.0f);
+    bool black = true;
+    for (size_t i = 0; i < npix; i++) {
+        for (int c = 0; c < 3; c++) {
+            const float v = px[size_t(c) * npix + i];
+            r.envImage[i * 4 + c] = v;
+            if (v != 0.0f) black = false;
+        }
+        if (channels == 4) r.envImage[i * 4 + 3] = px[3 * npix + i];
+    }
+    r.envW = width;
+    r.envH = height;
+    r.envImageBlack = black;
+    r.useEnvImage = true;
+    r.envDirty = true;
+    r.resetAccumulation();
+    return CRVPT_OK;
+}
+
+int crvpt_set_environment_map_intensity(double f) {
+    if (int rc = requireInit()) return rc;
+    g_r->envFactorRgbMode = false;
+    g_r->envFactor = float(f);
+    g_r->resetAccumulation();
+    return CRVPT_OK;
+}
+
+int crvpt_set_environment_map_intensity_rgb(const double f[3]) {
+    if (int rc = requireInit()) return rc;
+    g_r->envFactorRgbMode = true;
+    for (int a = 0; a < 3; a++) g_r->envFactorRgb[a] = float(f[a]);
+    g_r->resetAccumulation();
+    return CRVPT_OK;
+}
+
+int crvpt_set_use_lights(int use_lights) {
+    if (int rc = requireInit()) return rc;
+    g_r->useLights = use_lights != 0;
+    g_r->resetAccumulation();
+    return CRVPT_OK;
+}
+
+int crvpt_clear_lights(void) {
+    if (int rc = requireInit()) return rc;
+    g_r->lights.clear();
+    g_r->resetAccumulation();
+    return CRVPT_OK;
+}
+
+int crvpt_add_light(void) {
+    if (int rc = requireInit()) return rc;
+    if (g_r->lights.size() >= size_t(kMaxLights))
+        return fail(CRVPT_ERR_UNSUPPORTED, "add_light: at most " + std::to_string(kMaxLights) + " lights");
+    g_r->lights.push_back(defaultLight());
+    g_r->resetAccumulation();
+    return CRVPT_OK;
+}
+
+int crvpt_remove_light(int64_t idx) {
+    if (int rc = requireInit()) return rc;
+    if (idx >= 0 && size_t(idx) < g_r->lights.size()) g_r->lights.erase(g_r->lights.begin() + idx);
+    g_r->resetAccumulation();
+    return CRVPT_OK;
+}
+
+int crvpt_get_num_lights(int64_t* out) {
+    if (int rc = requireInit()) return rc;
+    *out = int64_t(g_r->lights.size());
+    return CRVPT_OK;
+}
+
+int crvpt_set_light_property(int64_t idx, const char* key_c, const char* value_c) {
+    if (int rc = requireInit()) return rc;
+    Renderer& r = *g_r;
+    // LightEditorWidget::setLightProperty (LightEditorWidget.cpp:286-320)
+    if (idx < 0 || size_t(idx) >= r.lights.size())
+        return fail(CRVPT_ERR_INVALID_ARGUMENT, "Error in LightEditorWidget::setLightProperty: Invalid light index.");
+    Light& L = r.lights[size_t(idx)];
+    const std::string key = key_c ? key_c : "", value = trim(value_c ? value_c : "");
+    float v3[3];
+    try {
+        if (key == "type") {
+            const char* names[] = {"Point Light", "Spotlight", "Directional Light"};
+            for (int i = 0; i < 3; i++)
+                if (value == names[i]) L.lightType = uint32_t(i);
+        } else if (key == "color") {
+            if (!parseVec3(value, v3)) return fail(CRVPT_ERR_INVALID_ARGUMENT, "Error in stringToVec3: Invalid number of entries.");
+            for (int a = 0; a < 3; a++) L.color[a] = v3[a];
+        } else if (key == "intensity") {
+            L.intensity = std::stof(value);
+        } else if (key == "use_distance") {
+            // The reference writes this key into the intensity field (LightEditorWidget.cpp:302-303).
+            L.intensity = (value == "1" || value == "true") ? 1.0f : 0.0f;
+        } else if (key == "position" || key == "direction") {
+            if (!parseVec3(value, v3)) return fail(CRVPT_ERR_INVALID_ARGUMENT, "Error in stringToVec3: Invalid number of entries.");
+            for (int a = 0; a < 3; a++) L.position[a] = v3[a];
+        } else if (key == "space") {
+            const char* names[] = {"World", "View", "View Orientation"};
+            for (int i = 0; i < 3; i++)
+                if (value == names[i]) L.lightSpace = uint32_t(i);
+        } else if (key == "spot_total_width") {
+            L.spotTotalWidth = std::stof(value);
+        } else if (key == "spot_falloff_start") {
+            L.spotFalloffStart = std::stof(value);
+        } else if (key == "spot_direction") {
+            if (!parseVec3(value, v3)) return fail(CRVPT_ERR_INVALID_ARGUMENT, "Error in stringToVec3: Invalid number of entries.");
+            for (int a = 0; a < 3; a++) L.spotDirection[a] = v3[a];
+        }
+    } catch (...) {
+        return fail(CRVPT_ERR_INVALID_ARGUMENT, "set_light_property: cannot parse \"" + value + "\"");
+    }
+    r.resetAccumulation();
+    return CRVPT_OK;
+}
+
+int crvpt_set_max_grid_extent(double e) {
+    if (int rc = requireInit()) return rc;
+    g_r->maxGridExtent = float(e);
+    // Like CloudData::setMaxGridExtent, this takes effect for volumes loaded afterwards.
+    return CRVPT_OK;
+}
+
+int crvpt_get_render_bounding_box(double out[6]) {
+    if (int rc = requireInit()) return rc;
+    Renderer& r = *g_r;
+    const bool sp = sparseActive(r);
+    for (int a = 0; a < 3; a++) {
+        out[2 * a] = sp ? r.boxMinSparse[a] : r.boxMinDense[a];
+        out[2 * a + 1] = sp ? r.boxMaxSparse[a] : r.boxMaxDense[a];
+    }
+    return CRVPT_OK;
+}
+
+int crvpt_get_volume_voxel_size(int64_t out[3]) {
+    if (int rc = requireInit()) return rc;
+    out[0] = g_r->dims[2];
+    out[1] = g_r->dims[1];
+    out[2] = g_r->dims[0];
+    return CRVPT_OK;
+}
+
+int crvpt_set_denoiser(const char* name) {
+    if (int rc = requireInit()) return rc;
+    const std::string n = name ? name : "";
+    Renderer& r = *g_r;
+    if (n == "None") r.denoiser = 0;
+    else if (n == "Default" || n == "EAW" || n == "Edge-Avoiding À-Trous Wavelet") r.denoiser = 1;
+    else return fail(CRVPT_ERR_INVALID_ARGUMENT, "Error in setDenoiser: Invalid denoiser name \"" + n + "\".");
+    // VolumetricPathTracingModuleRenderer::setDenoiserType clears the settings (EAW defaults, EAWDenoiser.hpp:107-119)
+    r.eawIterations = 0;
+    r.eawPhiColor = 5.0f;
+    r.eawPhiPosition = 0.1f;
+    r.eawPhiNormal = 0.1f;
+    return CRVPT_OK;
+}
+
+int crvpt_set_denoiser_property(const char* key_c, const char* value_c) {
+    if (int rc = requireInit()) return rc;
+    Renderer& r = *g_r;
+    const std::string key = key_c ? key_c : "", value = value_c ? value_c : "";
+    try {
+        if (key == "maxNumIterations" || key == "numIterations") r.eawIterations = std::stoi(value);
+        else if (key == "phiColor") r.eawPhiColor = std::stof(value);
+        else if (key == "phiPosition") r.eawPhiPosition = std::stof(value);
+        else if (key == "phiNormal") r.eawPhiNormal = std::stof(value);
+        else if (key == "useColorDistance") r.eawUseColorDistance = std::stoi(value);
+        else return fail(CRVPT_ERR_INVALID_ARGUMENT, "set_denoiser_property: unknown key \"" + key + "\"");
+    } catch (...) {
+        return fail(CRVPT_ERR_INVALID_ARGUMENT, "set_denoiser_property: cannot parse \"" + value + "\"");
+    }
+    return CRVPT_OK;
+}
+
+int crvpt_transmittance_batch(int estimator, int n_rays, const float* origins, const float* directions,
+                              const float* t_max, int n_samples, uint32_t seed, float* mean, float* m2,
+                              int on_device, void* stream) {
+    if (int rc = requireInit()) return rc;
+    if (estimator < 0 || estimator > 3 || n_rays < 0 || n_samples <= 0 || !origins || !directions || !mean)
+        return fail(CRVPT_ERR_INVALID_ARGUMENT, "transmittance_batch: invalid arguments");
+    Renderer& r = *g_r;
+    const int prevMode = r.mode;
+    if (estimator == T_RESIDUAL_RATIO) r.mode = VPT_RESIDUAL_RATIO_TRACKING;
+    if (estimator == T_DECOMPOSITION) r.mode = VPT_DECOMPOSITION_TRACKING;
+    int rc = prepareScene(r);
+    r.mode = prevMode;
+    if (rc) return rc;
+    if (n_rays == 0) return CRVPT_OK;
+    hipStream_t s = on_device ? reinterpret_cast<hipStream_t>(stream) : r.stream;
+    if (s != r.stream) HIP_TRY(hipStreamSynchronize(r.stream));
+    SceneParams P;
+    SupervoxelGrid S;
+    fillSceneParams(r, 1, 1, P, S);
+    DenseGrid dg;
+    NanoGrid ng;
+    bool sp;
+    gridViews(r, dg, ng, sp);
+    const float *o = origins, *d = directions, *tm = t_max;
+    float *mo = mean, *m2o = m2;
+    if (!on_device) {
+        const size_t nb = size_t(n_rays) * (3 + 3 + 1 + 1 + 1) * sizeof(float);
+        HIP_TRY(r.dRays.ensure(nb));
+        float* base = r.dRays.as<float>();
+        float* dO = base;
+        float* dD = dO + 3 * size_t(n_rays);
+        float* dT = dD + 3 * size_t(n_rays);
+        float* dM = dT + n_rays;
+        float* dM2 = dM + n_rays;
+        HIP_TRY(hipMemcpy(dO, origins, 3 * size_t(n_rays) * sizeof(float), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(dD, directions, 3 * size_t(n_rays) * sizeof(float), hipMemcpyHostToDevice));
+        if (t_max) HIP_TRY(hipMemcpy(dT, t_max, size_t(n_rays) * sizeof(float), hipMemcpyHostToDevice));
+        o = dO;
+        d = dD;
+        tm = t_max ? dT : nullptr;
+        mo = dM;
+        m2o = m2 ? dM2 : nullptr;
+    }
+    HIP_TRY(hipEventRecord(r.ev0, s));
+    HIP_TRY(launchTransmittance(P, sp ? nullptr : &dg, sp ? &ng : nullptr, S, estimator, o, d, tm, n_rays, n_samples,
+                                seed, mo, m2o, s));
+    HIP_TRY(hipEventRecord(r.ev1, s));
+    HIP_TRY(hipEventRecord(r.evT1, s));
+    if (!on_device) {
+        HIP_TRY(hipStreamSynchronize(s));
+        HIP_TRY(hipMemcpy(mean, mo, size_t(n_rays) * sizeof(float), hipMemcpyDeviceToHost));
+        if (m2) HIP_TRY(hipMemcpy(m2, m2o, size_t(n_rays) * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    return CRVPT_OK;
+}
+
+int crvpt_set_use_majorant_grid(int use) {
+    if (int rc = requireInit()) return rc;
+    g_r->useMajorantGrid = use != 0;
+    return CRVPT_OK;
+}
+
+int crvpt_set_launch_config(int supervoxel_size, int blocks, int steps_per_refill) {
+    if (int rc = requireInit()) return rc;
+    if (supervoxel_size < 0 || blocks < 0 || steps_per_refill < 0)
+        return fail(CRVPT_ERR_INVALID_ARGUMENT, "set_launch_config: negative value");
+    Renderer& r = *g_r;
+    if (supervoxel_size != r.svSizeSetting) {
+        r.svSizeSetting = supervoxel_size;
+        r.tablesDirty = true;
+    }
+    r.blocksSetting = blocks;
+    r.stepsPerRefill = steps_per_refill > 0 ? steps_per_refill : 4;
+    return CRVPT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,93 @@
+// Device read path for NanoVDB 32.3 grids with FP32 values (GridType::Float), the sparse format the reference
+// uploads (VolumetricPathTracingPass.cpp:599-674) and reads through PNanoVDB (VptUtils.glsl:496-580).
+//
+// Byte layout (offsets relative to each node, FP32 value type; PNanoVDB.glsl:646-1111 lists the same table):
+//   grid  672 B: map at 296 (mInvMatF 9 floats at +36, mVecF 3 floats at +72)
+//   tree   64 B: root node offset (u64) at 24, relative to the tree
+//   root   64 B: background f32 at 28, tile count u32 at 24, tiles of 32 B follow:
+//                key u64 at 0, child i64 at 8 (relative to the root, 0 = tile), state u32 at 16, value f32 at 20
+//   upper 32^3 children of 128^3 voxels: child mask at 4128, 8-byte table at 8256 (child offset or tile value)
+//   lower 16^3 children of 8^3 voxels:   child mask at 544,  8-byte table at 1088
+//   leaf  8^3 voxels: f32 values at 96, x-major (x << 6 | y << 3 | z)
+// Root keys pack the 4096^3 tile coordinate as (k >> 12) | (j >> 12) << 21 | (i >> 12) << 42.
+#pragma once
+
+#include "device_common.hpp"
+
+namespace crvpt {
+
+struct NanoAccess {
+    NanoGrid g;
+
+    CRVPT_DEV uint32_t u32(uint32_t off) const { return *reinterpret_cast<const uint32_t*>(g.buf + off); }
+    CRVPT_DEV float f32(uint32_t off) const { return *reinterpret_cast<const float*>(g.buf + off); }
+    CRVPT_DEV uint64_t u64(uint32_t off) const {
+        const uint2 v = *reinterpret_cast<const uint2*>(g.buf + off);
+        return uint64_t(v.x) | (uint64_t(v.y) << 32);
+    }
+
+    // Root tile lookup: the tile count is 1 for every domain that fits one 4096^3 tile, so the scan is short.
+    CRVPT_DEV float value(int i, int j, int k) const {
+        const uint64_t key = uint64_t(uint32_t(k) >> 12) | (uint64_t(uint32_t(j) >> 12) << 21) |
+                             (uint64_t(uint32_t(i) >> 12) << 42);
+        uint32_t tile = g.rootOffset + 64u;
+        uint32_t t = 0;
+        for (; t < g.tileCount; t++, tile += 32u) {
+            if (u64(tile) == key) break;
+        }
+        if (t == g.tileCount) return g.background;
+        const uint64_t child = u64(tile + 8u);
+        if (child == 0ull) return f32(tile + 20u);
+        const uint32_t upper = g.rootOffset + uint32_t(child);
+        const uint32_t nu = ((uint32_t(i) & 4095u) >> 7) << 10 | ((uint32_t(j) & 4095u) >> 7) << 5 |
+                            ((uint32_t(k) & 4095u) >> 7);
+        if (((u32(upper + 4128u + 4u * (nu >> 5)) >> (nu & 31u)) & 1u) == 0u) return f32(upper + 8256u + 8u * nu);
+        const uint32_t lower = upper + uint32_t(u64(upper + 8256u + 8u * nu));
+        const uint32_t nl = ((uint32_t(i) & 127u) >> 3) << 8 | ((uint32_t(j) & 127u) >> 3) << 4 |
+                            ((uint32_t(k) & 127u) >> 3);
+        if (((u32(lower + 544u + 4u * (nl >> 5)) >> (nl & 31u)) & 1u) == 0u) return f32(lower + 1088u + 8u * nl);
+        const uint32_t leaf = lower + uint32_t(u64(lower + 1088u + 8u * nl));
+        const uint32_t n = (uint32_t(i) & 7u) << 6 | (uint32_t(j) & 7u) << 3 | (uint32_t(k) & 7u);
+        return f32(leaf + 96u + 4u * n);
+    }
+
+    // Voxel (i, j, k) of the dense-equivalent texel grid: index coordinate = index bbox min + (i, j, k).
+    CRVPT_DEV float voxelAt(int i, int j, int k) const {
+        return value(i + g.indexMin[0], j + g.indexMin[1], k + g.indexMin[2]);
+    }
+
+    // pnanovdb_grid_world_to_indexf in FP32 with the reference's operation order (PNanoVDB.glsl:2008-2018).
+    CRVPT_DEV V3 worldToIndex(V3 c) const {
+        const float sx = c.x - g.vec[0], sy = c.y - g.vec[1], sz = c.z - g.vec[2];
+        return v3(sx * g.invMat[0] + sy * g.invMat[1] + sz * g.invMat[2],
+                  sx * g.invMat[3] + sy * g.invMat[4] + sz * g.invMat[5],
+                  sx * g.invMat[6] + sy * g.invMat[7] + sz * g.invMat[8]);
+    }
+
+    // sampleCloud for NanoVDB (VptUtils.glsl:515-580, 674-684): raw values, no normalisation.
+    CRVPT_DEV float lookupWorld(const SceneParams& P, Rng& rng, V3 pos) const {
+        V3 coord = (pos - v3(P.boxMin)) / (v3(P.boxMax) - v3(P.boxMin));
+        coord = coord * (v3(P.gridMax) - v3(P.gridMin)) + v3(P.gridMin);
+        V3 idx = worldToIndex(coord);
+        if (P.interpolation == INTERP_TRILINEAR) {
+            idx = idx - v3(0.5f);
+            const float fx = floorf(idx.x), fy = floorf(idx.y), fz = floorf(idx.z);
+            const int i = int(fx), j = int(fy), k = int(fz);
+            const float a = idx.x - fx, b = idx.y - fy, c = idx.z - fz;
+            const float c00 = lerpf(value(i, j, k), value(i + 1, j, k), a);
+            const float c10 = lerpf(value(i, j + 1, k), value(i + 1, j + 1, k), a);
+            const float c01 = lerpf(value(i, j, k + 1), value(i + 1, j, k + 1), a);
+            const float c11 = lerpf(value(i, j + 1, k + 1), value(i + 1, j + 1, k + 1), a);
+            return lerpf(lerpf(c00, c10, b), lerpf(c01, c11, b), c);
+        }
+        if (P.interpolation == INTERP_STOCHASTIC) {
+            const float j0 = rng.uniform() - 0.5f;
+            const float j1 = rng.uniform() - 0.5f;
+            const float j2 = rng.uniform() - 0.5f;
+            idx = idx + v3(j0, j1, j2);
+        }
+        return value(int(floorf(idx.x)), int(floorf(idx.y)), int(floorf(idx.z)));
+    }
+};
+
+}  // namespace crvpt

@@ -1,0 +1,475 @@
+// NanoVDB 32.3 writer and reader for FP32 fog volumes (see nanovdb_build.hpp).
+//
+// Serialised layout, breadth first (every node 32-byte aligned, offsets from the node that holds them):
+//   GridData 672 B | TreeData 64 B | RootData 64 B + 32 B per tile | upper nodes 270400 B | lower nodes 33856 B |
+//   leaves 2144 B.
+// Root tiles are ordered by tile origin (x, then y, then z), children of an internal node by ascending mask bit,
+// which is the order nanovdb::GridBuilder emits them in.
+#include "nanovdb_build.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <tuple>
+
+namespace crvpt {
+namespace {
+
+constexpr uint64_t kMagic = 0x304244566f6e614eULL;  // "NanoVDB0"
+constexpr uint32_t kVersion = (32u << 21) | (3u << 10) | 3u;
+constexpr uint32_t kGridSize = 672, kTreeSize = 64, kRootSize = 64, kTileSize = 32;
+constexpr uint32_t kUpperSize = 270400, kLowerSize = 33856, kLeafSize = 2144;
+constexpr uint32_t kGridTypeFloat = 1, kGridClassFog = 2;
+
+template <class T>
+void put(std::vector<uint8_t>& b, size_t off, T v) {
+    std::memcpy(b.data() + off, &v, sizeof(T));
+}
+template <class T>
+T get(const uint8_t* b, size_t off) {
+    T v;
+    std::memcpy(&v, b + off, sizeof(T));
+    return v;
+}
+
+struct Coord {
+    int x, y, z;
+    bool operator<(const Coord& o) const { return std::tie(x, y, z) < std::tie(o.x, o.y, o.z); }
+};
+
+struct Leaf {
+    Coord origin;
+    float values[512];
+    uint64_t mask[8];
+    uint64_t offset = 0;
+};
+struct Lower {
+    Coord origin;
+    uint64_t childMask[64] = {0}, valueMask[64] = {0};
+    float tileValue[4096];
+    int child[4096];  // leaf index or -1
+    uint64_t offset = 0;
+};
+struct Upper {
+    Coord origin;
+    uint64_t childMask[512] = {0}, valueMask[512] = {0};
+    float tileValue[32768];
+    int child[32768];  // lower index or -1
+    uint64_t offset = 0;
+};
+
+struct Stats {
+    int32_t bmin[3] = {INT32_MAX, INT32_MAX, INT32_MAX};
+    int32_t bmax[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+    float mn = 3.402823466e38f, mx = -3.402823466e38f;
+    double sum = 0.0, sum2 = 0.0;
+    uint64_t count = 0;
+    void addVoxel(int x, int y, int z, float v) {
+        const int c[3] = {x, y, z};
+        for (int a = 0; a < 3; a++) {
+            bmin[a] = std::min(bmin[a], c[a]);
+            bmax[a] = std::max(bmax[a], c[a]);
+        }
+        mn = std::min(mn, v);
+        mx = std::max(mx, v);
+        sum += v;
+        sum2 += double(v) * v;
+        count++;
+    }
+    void addTile(const Coord& o, int dim, float v) {
+        const int lo[3] = {o.x, o.y, o.z};
+        for (int a = 0; a < 3; a++) {
+            bmin[a] = std::min(bmin[a], lo[a]);
+            bmax[a] = std::max(bmax[a], lo[a] + dim - 1);
+        }
+        const double n = double(dim) * dim * dim;
+        mn = std::min(mn, v);
+        mx = std::max(mx, v);
+        sum += v * n;
+        sum2 += double(v) * v * n;
+        count += uint64_t(n);
+    }
+    void merge(const Stats& o) {
+        if (!o.count) return;
+        for (int a = 0; a < 3; a++) {
+            bmin[a] = std::min(bmin[a], o.bmin[a]);
+            bmax[a] = std::max(bmax[a], o.bmax[a]);
+        }
+        mn = std::min(mn, o.mn);
+        mx = std::max(mx, o.mx);
+        sum += o.sum;
+        sum2 += o.sum2;
+        count += o.count;
+    }
+    float avg() const { return count ? float(sum / double(count)) : 0.0f; }
+    float stddev() const {
+        if (!count) return 0.0f;
+        const double m = sum / double(count);
+        return float(std::sqrt(std::max(0.0, sum2 / double(count) - m * m)));
+    }
+};
+
+void putCoord(std::vector<uint8_t>& b, size_t off, const int32_t c[3]) {
+    for (int a = 0; a < 3; a++) put<int32_t>(b, off + 4 * a, c[a]);
+}
+
+}  // namespace
+
+bool parseNanoVdb(const uint8_t* buf, size_t size, NanoInfo& info, std::string& err) {
+    if (size < kGridSize + kTreeSize + kRootSize) {
+        err = "NanoVDB buffer too small";
+        return false;
+    }
+    if (get<uint64_t>(buf, 0) != kMagic) {
+        err = "NanoVDB buffer: bad magic number";
+        return false;
+    }
+    if ((get<uint32_t>(buf, 16) >> 21) != 32u) {
+        err = "NanoVDB buffer: unsupported major version (expected 32)";
+        return false;
+    }
+    // CloudData::computeSparseGridMetadata rejects non-float grids (CloudData.cpp:1292-1296).
+    if (get<uint32_t>(buf, 636) != kGridTypeFloat) {
+        err = "Fatal error in CloudData::computeSparseGridMetadata: The grid handle does not store a grid with value "
+              "type float.";
+        return false;
+    }
+    info.gridSize = get<uint64_t>(buf, 32);
+    if (info.gridSize > size) {
+        err = "NanoVDB buffer: grid size exceeds the buffer";
+        return false;
+    }
+    for (int i = 0; i < 9; i++) info.invMatF[i] = get<float>(buf, 296 + 36 + 4 * i);
+    for (int i = 0; i < 3; i++) info.vecF[i] = get<float>(buf, 296 + 72 + 4 * i);
+    for (int i = 0; i < 3; i++) {
+        info.worldBBoxMin[i] = get<double>(buf, 560 + 8 * i);
+        info.worldBBoxMax[i] = get<double>(buf, 584 + 8 * i);
+        info.voxelSize[i] = get<double>(buf, 608 + 8 * i);
+    }
+    const uint64_t rootRel = get<uint64_t>(buf, kGridSize + 24);
+    info.rootOffset = kGridSize + rootRel;
+    for (int i = 0; i < 3; i++) info.nodeCount[i] = get<uint32_t>(buf, kGridSize + 32 + 4 * i);
+    info.activeVoxelCount = get<uint64_t>(buf, kGridSize + 56);
+    if (info.rootOffset + kRootSize > size) {
+        err = "NanoVDB buffer: root outside the buffer";
+        return false;
+    }
+    for (int i = 0; i < 3; i++) {
+        info.indexBBoxMin[i] = get<int32_t>(buf, info.rootOffset + 4 * i);
+        info.indexBBoxMax[i] = get<int32_t>(buf, info.rootOffset + 12 + 4 * i);
+    }
+    info.tileCount = get<uint32_t>(buf, info.rootOffset + 24);
+    info.background = get<float>(buf, info.rootOffset + 28);
+    if (info.gridSize >= (uint64_t(1) << 32)) {
+        err = "NanoVDB buffer: grids of 4 GiB or more need 64-bit offsets";
+        return false;
+    }
+    return true;
+}
+
+bool extractNanoVdbGridFromFile(const std::vector<uint8_t>& file, std::vector<uint8_t>& grid, std::string& err) {
+    // nanovdb::io segment: Header (16 B: magic, version, grid count u16, codec u16), then per grid a 176-byte
+    // MetaData record followed by the grid name, then the grid buffers.
+    if (file.size() < 16 || get<uint64_t>(file.data(), 0) != kMagic) {
+        err = "load_cloud_file: not a NanoVDB file";
+        return false;
+    }
+    const uint16_t gridCount = get<uint16_t>(file.data(), 12);
+    const uint16_t codec = get<uint16_t>(file.data(), 14);
+    if (gridCount == 0) {
+        err = "load_cloud_file: NanoVDB file without grids";
+        return false;
+    }
+    if (codec != 0) {
+        err = "load_cloud_file: compressed NanoVDB files are not supported";
+        return false;
+    }
+    size_t off = 16;
+    uint64_t firstSize = 0;
+    for (uint16_t g = 0; g < gridCount; g++) {
+        if (off + 176 > file.size()) {
+            err = "load_cloud_file: truncated NanoVDB metadata";
+            return false;
+        }
+        if (g == 0) firstSize = get<uint64_t>(file.data(), off + 8);
+        const uint32_t nameSize = get<uint32_t>(file.data(), off + 136);
+        off += 176 + nameSize;
+    }
+    if (off + firstSize > file.size()) {
+        err = "load_cloud_file: truncated NanoVDB grid";
+        return false;
+    }
+    grid.assign(file.begin() + off, file.begin() + off + firstSize);
+    return true;
+}
+
+std::vector<uint8_t> buildNanoVdbFromDense(const float* values, const int dims[3], double dx, const double p0[3],
+                                           const std::string& name) {
+    // 1. Leaves: every 8^3 block with at least one non-background voxel (GridBuilder.h:251-300).
+    std::vector<Leaf> leaves;
+    std::map<Coord, int> leafIndex;
+    const int lx = (dims[0] + 7) / 8, ly = (dims[1] + 7) / 8, lz = (dims[2] + 7) / 8;
+    std::vector<std::pair<Coord, float>> lowerTiles;  // pruned constant, fully active leaves
+    for (int bz = 0; bz < lz; bz++)
+        for (int by = 0; by < ly; by++)
+            for (int bx = 0; bx < lx; bx++) {
+                Leaf L;
+                L.origin = {bx * 8, by * 8, bz * 8};
+                std::fill(std::begin(L.values), std::end(L.values), 0.0f);
+                std::fill(std::begin(L.mask), std::end(L.mask), 0ull);
+                int active = 0;
+                for (int x = 0; x < 8; x++)
+                    for (int y = 0; y < 8; y++)
+                        for (int z = 0; z < 8; z++) {
+                            const int gx = bx * 8 + x, gy = by * 8 + y, gz = bz * 8 + z;
+                            if (gx >= dims[0] || gy >= dims[1] || gz >= dims[2]) continue;
+                            const float v = values[size_t(gx) + (size_t(gy) + size_t(gz) * dims[1]) * dims[0]];
+                            if (v == 0.0f) continue;
+                            const int n = x << 6 | y << 3 | z;
+                            L.values[n] = v;
+                            L.mask[n >> 6] |= 1ull << (n & 63);
+                            active++;
+                        }
+                if (!active) continue;
+                if (active == 512) {
+                    bool constant = true;
+                    for (int n = 1; n < 512 && constant; n++) constant = L.values[n] == L.values[0];
+                    if (constant) {
+                        lowerTiles.push_back({L.origin, L.values[0]});
+                        continue;
+                    }
+                }
+                leafIndex[L.origin] = int(leaves.size());
+                leaves.push_back(L);
+            }
+    // 2. Internal nodes.
+    std::map<Coord, int> lowerIndex, upperIndex;
+    std::vector<Lower> lowers;
+    std::vector<Upper> uppers;
+    auto lowerOf = [&](const Coord& c) -> Lower& {
+        const Coord o{c.x & ~127, c.y & ~127, c.z & ~127};
+        auto it = lowerIndex.find(o);
+        if (it != lowerIndex.end()) return lowers[size_t(it->second)];
+        lowerIndex[o] = int(lowers.size());
+        lowers.emplace_back();
+        Lower& n = lowers.back();
+        n.origin = o;
+        std::fill(std::begin(n.tileValue), std::end(n.tileValue), 0.0f);
+        std::fill(std::begin(n.child), std::end(n.child), -1);
+        return n;
+    };
+    for (auto& kv : leafIndex) {
+        Lower& n = lowerOf(kv.first);
+        const int i = ((kv.first.x & 127) >> 3) << 8 | ((kv.first.y & 127) >> 3) << 4 | ((kv.first.z & 127) >> 3);
+        n.child[i] = kv.second;
+        n.childMask[i >> 6] |= 1ull << (i & 63);
+    }
+    for (auto& t : lowerTiles) {
+        Lower& n = lowerOf(t.first);
+        const int i = ((t.first.x & 127) >> 3) << 8 | ((t.first.y & 127) >> 3) << 4 | ((t.first.z & 127) >> 3);
+        n.tileValue[i] = t.second;
+        n.valueMask[i >> 6] |= 1ull << (i & 63);
+    }
+    auto upperOf = [&](const Coord& c) -> Upper& {
+        const Coord o{c.x & ~4095, c.y & ~4095, c.z & ~4095};
+        auto it = upperIndex.find(o);
+        if (it != upperIndex.end()) return uppers[size_t(it->second)];
+        upperIndex[o] = int(uppers.size());
+        uppers.emplace_back();
+        Upper& n = uppers.back();
+        n.origin = o;
+        std::fill(std::begin(n.tileValue), std::end(n.tileValue), 0.0f);
+        std::fill(std::begin(n.child), std::end(n.child), -1);
+        return n;
+    };
+    for (auto& kv : lowerIndex) {
+        Upper& n = upperOf(kv.first);
+        const int i = ((kv.first.x & 4095) >> 7) << 10 | ((kv.first.y & 4095) >> 7) << 5 | ((kv.first.z & 4095) >> 7);
+        n.child[i] = kv.second;
+        n.childMask[i >> 6] |= 1ull << (i & 63);
+    }
+    // 3. Breadth-first offsets in root-table / mask order.
+    std::vector<int> upperOrder, lowerOrder, leafOrder;
+    for (auto& kv : upperIndex) {
+        Upper& u = uppers[size_t(kv.second)];
+        upperOrder.push_back(kv.second);
+        for (int i = 0; i < 32768; i++)
+            if (u.child[i] >= 0) {
+                lowerOrder.push_back(u.child[i]);
+                Lower& l = lowers[size_t(u.child[i])];
+                for (int j = 0; j < 4096; j++)
+                    if (l.child[j] >= 0) leafOrder.push_back(l.child[j]);
+            }
+    }
+    const uint64_t tiles = upperIndex.size();
+    const uint64_t rootOff = kGridSize + kTreeSize;
+    const uint64_t upperBase = rootOff + kRootSize + kTileSize * tiles;
+    const uint64_t lowerBase = upperBase + uint64_t(kUpperSize) * upperOrder.size();
+    const uint64_t leafBase = lowerBase + uint64_t(kLowerSize) * lowerOrder.size();
+    const uint64_t total = leafBase + uint64_t(kLeafSize) * leafOrder.size();
+    for (size_t i = 0; i < upperOrder.size(); i++) uppers[size_t(upperOrder[i])].offset = upperBase + kUpperSize * i;
+    for (size_t i = 0; i < lowerOrder.size(); i++) lowers[size_t(lowerOrder[i])].offset = lowerBase + kLowerSize * i;
+    for (size_t i = 0; i < leafOrder.size(); i++) leaves[size_t(leafOrder[i])].offset = leafBase + kLeafSize * i;
+
+    std::vector<uint8_t> b(total, 0);
+    // 4. Leaves.
+    std::vector<Stats> leafStats(leaves.size());
+    uint64_t voxelCount = 0;
+    for (size_t li = 0; li < leaves.size(); li++) {
+        const Leaf& L = leaves[li];
+        Stats& s = leafStats[li];
+        for (int n = 0; n < 512; n++)
+            if ((L.mask[n >> 6] >> (n & 63)) & 1ull)
+                s.addVoxel(L.origin.x + (n >> 6), L.origin.y + ((n >> 3) & 7), L.origin.z + (n & 7), L.values[n]);
+        voxelCount += s.count;
+        const size_t o = size_t(L.offset);
+        putCoord(b, o, s.bmin);
+        const uint8_t dif[3] = {uint8_t(s.bmax[0] - s.bmin[0]), uint8_t(s.bmax[1] - s.bmin[1]),
+                                uint8_t(s.bmax[2] - s.bmin[2])};
+        std::memcpy(b.data() + o + 12, dif, 3);
+        b[o + 15] = 0;  // flags
+        std::memcpy(b.data() + o + 16, L.mask, 64);
+        put<float>(b, o + 80, s.mn);
+        put<float>(b, o + 84, s.mx);
+        put<float>(b, o + 88, s.avg());
+        put<float>(b, o + 92, s.stddev());
+        std::memcpy(b.data() + o + 96, L.values, 2048);
+    }
+    // 5. Lower nodes.
+    std::vector<Stats> lowerStats(lowers.size());
+    uint32_t lowerTileCount = 0;
+    for (size_t ni = 0; ni < lowers.size(); ni++) {
+        const Lower& n = lowers[ni];
+        Stats& s = lowerStats[ni];
+        const size_t o = size_t(n.offset);
+        for (int i = 0; i < 4096; i++) {
+            const Coord c{n.origin.x + ((i >> 8) << 3), n.origin.y + (((i >> 4) & 15) << 3), n.origin.z + ((i & 15) << 3)};
+            if (n.child[i] >= 0) {
+                s.merge(leafStats[size_t(n.child[i])]);
+                put<int64_t>(b, o + 1088 + 8 * size_t(i), int64_t(leaves[size_t(n.child[i])].offset - n.offset));
+            } else {
+                put<float>(b, o + 1088 + 8 * size_t(i), n.tileValue[i]);
+                if ((n.valueMask[i >> 6] >> (i & 63)) & 1ull) {
+                    s.addTile(c, 8, n.tileValue[i]);
+                    lowerTileCount++;
+                    voxelCount += 512;
+                }
+            }
+        }
+        putCoord(b, o, s.bmin);
+        putCoord(b, o + 12, s.bmax);
+        put<uint64_t>(b, o + 24, 0);
+        std::memcpy(b.data() + o + 32, n.valueMask, 512);
+        std::memcpy(b.data() + o + 544, n.childMask, 512);
+        put<float>(b, o + 1056, s.mn);
+        put<float>(b, o + 1060, s.mx);
+        put<float>(b, o + 1064, s.avg());
+        put<float>(b, o + 1068, s.stddev());
+    }
+    // 6. Upper nodes.
+    std::vector<Stats> upperStats(uppers.size());
+    for (size_t ni = 0; ni < uppers.size(); ni++) {
+        const Upper& n = uppers[ni];
+        Stats& s = upperStats[ni];
+        const size_t o = size_t(n.offset);
+        for (int i = 0; i < 32768; i++) {
+            if (n.child[i] >= 0) {
+                s.merge(lowerStats[size_t(n.child[i])]);
+                put<int64_t>(b, o + 8256 + 8 * size_t(i), int64_t(lowers[size_t(n.child[i])].offset - n.offset));
+            } else {
+                put<float>(b, o + 8256 + 8 * size_t(i), n.tileValue[i]);
+            }
+        }
+        putCoord(b, o, s.bmin);
+        putCoord(b, o + 12, s.bmax);
+        put<uint64_t>(b, o + 24, 0);
+        std::memcpy(b.data() + o + 32, n.valueMask, 4096);
+        std::memcpy(b.data() + o + 4128, n.childMask, 4096);
+        put<float>(b, o + 8224, s.mn);
+        put<float>(b, o + 8228, s.mx);
+        put<float>(b, o + 8232, s.avg());
+        put<float>(b, o + 8236, s.stddev());
+    }
+    // 7. Root and tiles.
+    Stats rootStats;
+    {
+        size_t t = 0;
+        for (auto& kv : upperIndex) {
+            const Upper& u = uppers[size_t(kv.second)];
+            rootStats.merge(upperStats[size_t(kv.second)]);
+            const size_t o = size_t(rootOff + kRootSize + kTileSize * t);
+            const uint64_t key = uint64_t(uint32_t(u.origin.z) >> 12) | (uint64_t(uint32_t(u.origin.y) >> 12) << 21) |
+                                 (uint64_t(uint32_t(u.origin.x) >> 12) << 42);
+            put<uint64_t>(b, o, key);
+            put<int64_t>(b, o + 8, int64_t(u.offset - rootOff));
+            put<uint32_t>(b, o + 16, 0);
+            put<float>(b, o + 20, 0.0f);
+            t++;
+        }
+    }
+    if (!rootStats.count) {
+        for (int a = 0; a < 3; a++) {
+            rootStats.bmin[a] = 0;
+            rootStats.bmax[a] = 0;
+        }
+        rootStats.mn = rootStats.mx = 0.0f;
+    }
+    putCoord(b, rootOff, rootStats.bmin);
+    putCoord(b, rootOff + 12, rootStats.bmax);
+    put<uint32_t>(b, rootOff + 24, uint32_t(tiles));
+    put<float>(b, rootOff + 28, 0.0f);
+    put<float>(b, rootOff + 32, rootStats.mn);
+    put<float>(b, rootOff + 36, rootStats.mx);
+    put<float>(b, rootOff + 40, rootStats.avg());
+    put<float>(b, rootOff + 44, rootStats.stddev());
+    // 8. Tree.
+    const size_t tree = kGridSize;
+    put<uint64_t>(b, tree + 0, leafOrder.empty() ? 0 : leafBase - tree);
+    put<uint64_t>(b, tree + 8, lowerOrder.empty() ? 0 : lowerBase - tree);
+    put<uint64_t>(b, tree + 16, upperOrder.empty() ? 0 : upperBase - tree);
+    put<uint64_t>(b, tree + 24, rootOff - tree);
+    put<uint32_t>(b, tree + 32, uint32_t(leafOrder.size()));
+    put<uint32_t>(b, tree + 36, uint32_t(lowerOrder.size()));
+    put<uint32_t>(b, tree + 40, uint32_t(upperOrder.size()));
+    put<uint32_t>(b, tree + 44, lowerTileCount);
+    put<uint32_t>(b, tree + 48, 0);
+    put<uint32_t>(b, tree + 52, 0);
+    put<uint64_t>(b, tree + 56, voxelCount);
+    // 9. Grid.
+    put<uint64_t>(b, 0, kMagic);
+    put<uint64_t>(b, 8, 0);
+    put<uint32_t>(b, 16, kVersion);
+    put<uint32_t>(b, 20, (1u << 1) | (1u << 2) | (1u << 5));  // HasBBox | HasMinMax | IsBreadthFirst
+    put<uint32_t>(b, 24, 0);
+    put<uint32_t>(b, 28, 1);
+    put<uint64_t>(b, 32, total);
+    std::strncpy(reinterpret_cast<char*>(b.data() + 40), name.c_str(), 255);
+    const size_t map = 296;
+    const double invdx = 1.0 / dx;
+    for (int i = 0; i < 3; i++) {
+        for (int j = 0; j < 3; j++) {
+            put<float>(b, map + 4 * (3 * i + j), i == j ? float(dx) : 0.0f);
+            put<float>(b, map + 36 + 4 * (3 * i + j), i == j ? float(invdx) : 0.0f);
+            put<double>(b, map + 88 + 8 * (3 * i + j), i == j ? dx : 0.0);
+            put<double>(b, map + 160 + 8 * (3 * i + j), i == j ? invdx : 0.0);
+        }
+        put<float>(b, map + 72 + 4 * i, float(p0[i]));
+        put<double>(b, map + 232 + 8 * i, p0[i]);
+    }
+    put<float>(b, map + 84, 1.0f);
+    put<double>(b, map + 256, 1.0);
+    // World bounding box of the active index box [min, max + 1) through the map.
+    for (int a = 0; a < 3; a++) {
+        put<double>(b, 560 + 8 * a, double(rootStats.bmin[a]) * dx + p0[a]);
+        put<double>(b, 584 + 8 * a, double(rootStats.bmax[a] + 1) * dx + p0[a]);
+        put<double>(b, 608 + 8 * a, dx);
+    }
+    put<uint32_t>(b, 632, kGridClassFog);
+    put<uint32_t>(b, 636, kGridTypeFloat);
+    put<int64_t>(b, 640, 0);
+    put<uint32_t>(b, 648, 0);
+    return b;
+}
+
+}  // namespace crvpt

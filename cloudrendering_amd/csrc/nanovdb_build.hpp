@@ -1,0 +1,40 @@
+// Host-side NanoVDB 32.3 support: header parsing of FP32 grids, extraction of the first grid of a .nvdb file,
+// and the dense -> sparse conversion the reference performs with nanovdb::GridBuilder in
+// CloudData::getSparseDensityField (CloudData.cpp:1367-1438).
+#pragma once
+
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+namespace crvpt {
+
+struct NanoInfo {
+    uint64_t gridSize = 0;
+    uint64_t rootOffset = 0;  // absolute byte offset of the root node
+    uint32_t tileCount = 0;
+    float background = 0.0f;
+    float invMatF[9] = {0};
+    float vecF[3] = {0};
+    double voxelSize[3] = {1, 1, 1};
+    double worldBBoxMin[3] = {0, 0, 0};
+    double worldBBoxMax[3] = {0, 0, 0};
+    int32_t indexBBoxMin[3] = {0, 0, 0};
+    int32_t indexBBoxMax[3] = {0, 0, 0};
+    uint32_t nodeCount[3] = {0, 0, 0};  // leaf, lower, upper
+    uint64_t activeVoxelCount = 0;
+};
+
+// Validates a single-grid FP32 NanoVDB buffer and reads the fields the renderer needs.
+bool parseNanoVdb(const uint8_t* buf, size_t size, NanoInfo& info, std::string& err);
+
+// Returns the first grid of an uncompressed .nvdb file (io::writeGrid layout: header, metadata, grid bytes).
+bool extractNanoVdbGridFromFile(const std::vector<uint8_t>& file, std::vector<uint8_t>& grid, std::string& err);
+
+// Builds a FogVolume FP32 NanoVDB grid from a dense field (x fastest) with background 0: voxels equal to the
+// background stay inactive, constant fully-active leaves become lower-node tiles, voxel size dx and origin p0.
+std::vector<uint8_t> buildNanoVdbFromDense(const float* values, const int dims[3], double dx, const double p0[3],
+                                           const std::string& name);
+
+}  // namespace crvpt

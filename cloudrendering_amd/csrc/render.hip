@@ -1,0 +1,978 @@
+// gfx950 kernels of the volumetric path tracer.
+//
+// Work decomposition: a "unit" is one camera path (pixel p, sample s); unit = s * (W*H) + p, so a wave's 64
+// consecutive units are 64 neighbouring pixels of one sample (coherent primary rays). A persistent grid of waves
+// drains a global unit counter: whenever lanes of a wave have no live path, one lane claims as many units as the
+// wave has free lanes (ballot + popcount + one atomic) and the free lanes take them in lane order (mbcnt), so
+// lanes refill at any step instead of idling until the longest path of the wave ends. Each path writes its
+// radiance to its own slot; a separate kernel sums the slots of a pixel in sample order, so images are
+// bit-reproducible whatever the lane/wave schedule was.
+//
+// Free-flight sampling walks a supervoxel majorant grid (piecewise-constant majorant along the ray): an Exp(1)
+// optical budget is spent supervoxel by supervoxel, empty supervoxels cost one DDA step and no density fetch, and
+// a tentative collision is accepted with probability sigma*rho / mu_local. With the grid disabled the walker
+// degenerates to the reference's global majorant (extinction.x), i.e. the reference trackers' exact free-path
+// distribution.
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+#include "device_common.hpp"
+#include "nanovdb_access.hpp"
+#include "render_api.hpp"
+
+namespace crvpt {
+
+constexpr float kInf = 3.0e38f;
+
+// ------------------------------------------------------------------------------------------------------------
+// Supervoxel walker
+// ------------------------------------------------------------------------------------------------------------
+struct Walker {
+    uint32_t svReads;   // supervoxel majorant reads since the path started (work counter)
+    V3 o, w;            // ray origin / direction (world)
+    float t, tEnd;      // current distance along the ray, end of the segment
+    int ix, iy, iz;     // current supervoxel
+    float nx, ny, nz;   // distance of the next supervoxel plane per axis
+    float dx, dy, dz;   // plane spacing along the ray per axis
+    float mu;           // majorant of the current supervoxel, extinction units
+    float tau;          // remaining optical length of the current Exp(1) sample; < 0 when none is drawn
+};
+
+CRVPT_DEV float svMajorant(const SceneParams& P, const SupervoxelGrid& S, int ix, int iy, int iz) {
+    return P.extinction * S.majorant[(size_t(iz) * S.grid[1] + iy) * S.grid[0] + ix];
+}
+
+// Starts a segment from x along w, clipped to the box and to [0, limit]. False when the segment is empty.
+CRVPT_DEV bool walkerStart(Walker& W, const SceneParams& P, const SupervoxelGrid& S, V3 x, V3 w, float limit) {
+    float tMin, tMax;
+    if (!rayBoxIntersect(v3(P.boxMin), v3(P.boxMax), x, w, tMin, tMax)) return false;
+    tMax = fminf(tMax, limit);
+    if (!(tMax > tMin)) return false;
+    W.o = x;
+    W.w = w;
+    W.t = tMin;
+    W.tEnd = tMax;
+    W.tau = -1.0f;
+    if (!P.useMajorantGrid) {
+        W.mu = P.extinction;
+        W.nx = W.ny = W.nz = kInf;
+        return true;
+    }
+    // Cell of the point slightly inside the segment (robust against an entry point on a cell plane).
+    const float tc = tMin + 1e-4f * (tMax - tMin);
+    const V3 p = x + w * tc;
+    int idx[3];
+    float nxt[3], dlt[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const float wa = comp(w, a), pa = comp(p, a);
+        // supervoxel edge in world units: S.size voxels of the density grid
+        const float cw = float(S.size) / P.worldToTex[a];
+        int i = int(floorf((pa - P.boxMin[a]) / cw));
+        i = min(max(i, 0), S.grid[a] - 1);
+        idx[a] = i;
+        if (wa > 0.0f) {
+            nxt[a] = tc + (P.boxMin[a] + float(i + 1) * cw - pa) / wa;
+            dlt[a] = cw / wa;
+        } else if (wa < 0.0f) {
+            nxt[a] = tc + (P.boxMin[a] + float(i) * cw - pa) / wa;
+            dlt[a] = -cw / wa;
+        } else {
+            nxt[a] = kInf;
+            dlt[a] = kInf;
+        }
+    }
+    W.ix = idx[0]; W.iy = idx[1]; W.iz = idx[2];
+    W.nx = nxt[0]; W.ny = nxt[1]; W.nz = nxt[2];
+    W.dx = dlt[0]; W.dy = dlt[1]; W.dz = dlt[2];
+    W.mu = svMajorant(P, S, W.ix, W.iy, W.iz);
+    W.svReads++;
+    return true;
+}
+
+enum : int { WALK_MOVED = 0, WALK_COLLIDE = 1, WALK_END = 2 };
+
+// One walker event: a tentative collision (W.t at the collision), a supervoxel crossing, or the segment end.
+CRVPT_DEV int walkStep(Walker& W, Rng& rng, const SceneParams& P, const SupervoxelGrid& S) {
+    if (W.tau < 0.0f) W.tau = rng.exponential();
+    const float tExit = fminf(fminf(W.nx, W.ny), fminf(W.nz, W.tEnd));
+    const float opt = W.mu * (tExit - W.t);
+    if (W.tau < opt) {
+        W.t += W.tau / W.mu;
+        W.tau = -1.0f;
+        return WALK_COLLIDE;
+    }
+    W.tau -= opt;
+    W.t = tExit;
+    if (tExit >= W.tEnd) return WALK_END;
+    if (W.nx <= W.ny && W.nx <= W.nz) {
+        W.ix += W.w.x > 0.0f ? 1 : -1;
+        W.nx += W.dx;
+    } else if (W.ny <= W.nz) {
+        W.iy += W.w.y > 0.0f ? 1 : -1;
+        W.ny += W.dy;
+    } else {
+        W.iz += W.w.z > 0.0f ? 1 : -1;
+        W.nz += W.dz;
+    }
+    if ((unsigned)W.ix >= (unsigned)S.grid[0] || (unsigned)W.iy >= (unsigned)S.grid[1] ||
+        (unsigned)W.iz >= (unsigned)S.grid[2])
+        return WALK_END;
+    W.mu = svMajorant(P, S, W.ix, W.iy, W.iz);
+    W.svReads++;
+    return WALK_MOVED;
+}
+
+template <class Grid>
+CRVPT_DEV float density(const Grid& g, const SceneParams& P, Rng& rng, V3 p) {
+    return g.lookupWorld(P, rng, p);
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// Path pool for the delta (DeltaTracking.glsl:167-345), analog ratio (RatioTracking.glsl:25-96) and next-event
+// (NextEventTracking.glsl:223-394 with NeeTransmittance.glsl ratio-tracking shadow rays) estimators.
+// ------------------------------------------------------------------------------------------------------------
+struct PathSlot {
+    uint32_t taps;    // density lookups of this lane since the launch started (work counter)
+    uint32_t unit;
+    bool alive;
+    bool exhausted;
+    bool hasValue;
+    bool shadow;
+    float thr;        // analog ratio weight, or shadow-ray transmittance while shadow == true
+    V3 color;         // next-event accumulated radiance
+    V3 shadowScale;   // radiance factor of the pending shadow ray
+    V3 mainX, mainW;  // where the main path resumes after the shadow ray
+};
+
+CRVPT_DEV void writePath(float4* out, uint32_t unit, V3 c, bool hasValue) {
+    out[unit] = make_float4(c.x, c.y, c.z, hasValue ? 1.0f : 0.0f);
+}
+
+// Camera ray of a unit: seeds the path's Philox stream with (pixel, global frame, seed offset) and jitters the
+// pixel position with two draws (Clouds.glsl:80-90).
+CRVPT_DEV void cameraRayOfUnit(const SceneParams& P, uint32_t unit, Rng& rng, V3& x, V3& w) {
+    const uint32_t npix = uint32_t(P.width) * uint32_t(P.height);
+    const uint32_t s = unit / npix;
+    const uint32_t p = unit - s * npix;
+    const uint32_t px = p % uint32_t(P.width), py = p / uint32_t(P.width);
+    rng.init(p, P.frameBase + s, P.seedOffset);
+    const float jx = rng.uniform(), jy = rng.uniform();
+    const float sx = 2.0f * (float(px) + jx) / float(P.width) - 1.0f;
+    const float sy = 2.0f * (float(py) + jy) / float(P.height) - 1.0f;
+    createCameraRay(P, sx, sy, x, w);
+}
+
+template <int MODE, class Grid>
+CRVPT_DEV bool startPath(PathSlot& st, Walker& W, Rng& rng, const SceneParams& P, const SupervoxelGrid& S,
+                         float4* out) {
+    V3 x, w;
+    cameraRayOfUnit(P, st.unit, rng, x, w);
+    st.hasValue = false;
+    st.shadow = false;
+    st.thr = 1.0f;
+    st.color = v3(0.0f);
+    if (!walkerStart(W, P, S, x, w, kInf)) {
+        writePath(out, st.unit, skyRadiance(P, w), false);
+        return false;
+    }
+    return true;
+}
+
+// One event of a live path. Returns false when the path finished (its radiance is written).
+template <int MODE, class Grid>
+CRVPT_DEV bool stepPath(PathSlot& st, Walker& W, Rng& rng, const SceneParams& P, const SupervoxelGrid& S,
+                        const Grid& grid, float4* out) {
+    const int ev = walkStep(W, rng, P, S);
+    if (MODE == VPT_NEXT_EVENT_TRACKING && st.shadow) {
+        if (ev == WALK_END) {
+            st.color += st.shadowScale * st.thr;
+            st.shadow = false;
+            if (!walkerStart(W, P, S, st.mainX, st.mainW, kInf)) {
+                writePath(out, st.unit, st.color, true);
+                return false;
+            }
+        } else if (ev == WALK_COLLIDE) {
+            const V3 p = W.o + W.w * W.t;
+            const float rho = density(grid, P, rng, p);
+            st.taps++;
+            st.thr *= 1.0f - P.extinction * rho / W.mu;
+        }
+        return true;
+    }
+    if (ev == WALK_END) {
+        V3 c;
+        if (MODE == VPT_NEXT_EVENT_TRACKING) {
+            c = st.hasValue ? st.color : st.color + skyRadiance(P, W.w);
+        } else {
+            c = skyRadiance(P, W.w) * st.thr;
+        }
+        writePath(out, st.unit, c, st.hasValue);
+        return false;
+    }
+    if (ev != WALK_COLLIDE) return true;
+
+    const V3 p = W.o + W.w * W.t;
+    const float rho = density(grid, P, rng, p);
+    st.taps++;
+    const float xi = rng.uniform();
+    const float pReal = P.extinction * rho / W.mu;
+    const float pAbs = (1.0f - P.albedo) * pReal;
+    if (MODE != VPT_RATIO_TRACKING && xi < pAbs) {  // absorption: delta returns 0, NEE returns its sum
+        writePath(out, st.unit, MODE == VPT_NEXT_EVENT_TRACKING ? st.color : v3(0.0f), true);
+        return false;
+    }
+    if (!(xi < pReal)) return true;  // null collision
+
+    // Real scattering collision.
+    const V3 nw = samplePhase(rng, P.phaseG, W.w);
+    if (MODE == VPT_RATIO_TRACKING) st.thr *= 1.0f - pAbs / pReal;
+    st.hasValue = true;
+    if (MODE == VPT_NEXT_EVENT_TRACKING) {
+        const float pdfEnv = 1.0f / (4.0f * kPi);
+        bool toLight = false;
+        float probFactor = 1.0f, dist = 0.0f;
+        int li = 0;
+        V3 dir;
+        if (P.numLights > 0) {
+            toLight = P.isEnvMapBlack ? true : (rng.uniform() > 0.5f);
+            probFactor = P.isEnvMapBlack ? 1.0f : 2.0f;
+            if (toLight) dir = lightDirection(P, rng, p, li, probFactor, dist);
+            else dir = uniformSphereDirection(rng, v3(1.0f, 0.0f, 0.0f));
+        } else {
+            dir = uniformSphereDirection(rng, v3(1.0f, 0.0f, 0.0f));
+        }
+        const float ph = evalPhase(P.phaseG, W.w, dir);
+        const V3 L = toLight ? lightRadiance(P, p, li) * (probFactor * ph)
+                             : skyRadiance(P, dir) * (probFactor * ph / pdfEnv);
+        st.mainX = p;
+        st.mainW = nw;
+        if ((L.x > 0.0f || L.y > 0.0f || L.z > 0.0f) && walkerStart(W, P, S, p, dir, toLight ? dist : kInf)) {
+            st.shadow = true;
+            st.shadowScale = L;
+            st.thr = 1.0f;
+            return true;
+        }
+        st.color += L;  // unoccluded (segment outside the box) or zero contribution
+    }
+    if (!walkerStart(W, P, S, p, nw, kInf)) {
+        V3 c = MODE == VPT_NEXT_EVENT_TRACKING ? st.color : skyRadiance(P, nw) * st.thr;
+        writePath(out, st.unit, c, true);
+        return false;
+    }
+    return true;
+}
+
+// Wave-reduced 64-bit work counters: [0] density taps, [1] supervoxel majorant reads, [2] paths started.
+CRVPT_DEV void flushCounters(unsigned long long* stats, uint32_t taps, uint32_t svs, uint32_t paths) {
+    unsigned long long a = taps, b = svs, c = paths;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_down(a, off);
+        b += __shfl_down(b, off);
+        c += __shfl_down(c, off);
+    }
+    if (__lane_id() == 0) {
+        atomicAdd(&stats[0], a);
+        atomicAdd(&stats[1], b);
+        atomicAdd(&stats[2], c);
+    }
+}
+
+template <int MODE, class Grid>
+__global__ __launch_bounds__(256) void k_pathPool(SceneParams P, Grid grid, SupervoxelGrid S,
+                                                  float4* __restrict__ out, uint32_t* __restrict__ counter,
+                                                  int stepsPerRefill, unsigned long long* __restrict__ stats) {
+    const int lane = __lane_id();
+    PathSlot st;
+    st.alive = false;
+    st.exhausted = false;
+    st.unit = 0;
+    st.taps = 0;
+    uint32_t paths = 0;
+    Rng rng;
+    Walker W;
+    W.svReads = 0;
+    while (true) {
+        const bool needs = !st.alive && !st.exhausted;
+        const uint64_t needMask = __ballot(needs);
+        if (needMask) {
+            const int leader = __ffsll((unsigned long long)needMask) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(counter, uint32_t(__popcll(needMask)));
+            base = __shfl(base, leader);
+            if (needs) {
+                st.unit = base + uint32_t(__popcll(needMask & ((1ull << lane) - 1ull)));
+                if (uint64_t(st.unit) >= P.numUnits) {
+                    st.exhausted = true;
+                } else {
+                    paths++;
+                    st.alive = startPath<MODE, Grid>(st, W, rng, P, S, out);
+                }
+            }
+        }
+        if (__ballot(st.alive) == 0) {
+            if (__ballot(!st.exhausted) == 0) break;
+            continue;
+        }
+        for (int k = 0; k < stepsPerRefill; k++) {
+            if (st.alive) st.alive = stepPath<MODE, Grid>(st, W, rng, P, S, grid, out);
+        }
+    }
+    if (stats) flushCounters(stats, st.taps, W.svReads, paths);
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// Decomposition tracking (DecompositionTracking.glsl:35-123) and residual ratio tracking
+// (ResidualRatioTracking.glsl:33-224) on the reference's supervoxel tables. One lane runs a whole path; lanes
+// claim paths from the same per-wave ballot queue.
+// ------------------------------------------------------------------------------------------------------------
+CRVPT_DEV float signf(float v) { return v > 0.0f ? 1.0f : (v < 0.0f ? -1.0f : 0.0f); }
+CRVPT_DEV float fractf(float v) { return v - floorf(v); }
+
+CRVPT_DEV bool refInside(const SupervoxelGrid& S, int x, int y, int z) {
+    return x >= 0 && y >= 0 && z >= 0 && x < S.refGrid[0] && y < S.refGrid[1] && z < S.refGrid[2];
+}
+CRVPT_DEV size_t refIndex(const SupervoxelGrid& S, int x, int y, int z) {
+    return (size_t(z) * S.refGrid[1] + y) * S.refGrid[0] + x;
+}
+CRVPT_DEV V3 gridDims(const SceneParams& P) {
+    return v3(P.worldToTex[0] * (P.boxMax[0] - P.boxMin[0]), P.worldToTex[1] * (P.boxMax[1] - P.boxMin[1]),
+              P.worldToTex[2] * (P.boxMax[2] - P.boxMin[2]));
+}
+
+template <class Grid>
+CRVPT_DEV V3 decompositionPath(const SceneParams& P, const SupervoxelGrid& S, const Grid& grid, Rng& rng, V3 x,
+                               V3 w) {
+    const V3 eps = v3(1e-6f);
+    float tMinVal, tMaxVal;
+    if (rayBoxIntersect(v3(P.boxMin) + eps, v3(P.boxMax) - eps, x, w, tMinVal, tMaxVal)) {
+        const float majorant = P.extinction;
+        const float absorptionAlbedo = 1.0f - P.albedo;
+        const V3 dims = gridDims(P);
+        const V3 boxDelta = v3(P.boxMax) - v3(P.boxMin);
+        const V3 svSize = v3(float(S.refSize)) * boxDelta / dims;
+        x += w * tMinVal;
+        const V3 sp = (x - v3(P.boxMin)) / boxDelta * dims / v3(float(S.refSize));
+        int ix = int(floorf(sp.x)), iy = int(floorf(sp.y)), iz = int(floorf(sp.z));
+        while (refInside(S, ix, iy, iz)) {
+            const V3 minP = v3(P.boxMin) + v3(float(ix), float(iy), float(iz)) * svSize;
+            const V3 maxP = minP + svSize;
+            float a = 0.0f, b = 0.0f;
+            rayBoxIntersect(minP, maxP, x, w, a, b);
+            float dMax = b - a;
+            x += w * a;
+            const size_t si = refIndex(S, ix, iy, iz);
+            const float mmMin = S.decomp[2 * si], mmMax = S.decomp[2 * si + 1];
+            if (mmMax < 1e-5f) {
+                x += w * dMax;
+            } else {
+                const float muC = fmaxf(0.0000000001f, majorant * mmMin);
+                const float muR = fmaxf(0.0000000001f, majorant * mmMax - muC);
+                float tc = rng.exponential() / muC;
+                float tr = 0.0f;
+                while (true) {
+                    tr += rng.exponential() / muR;
+                    if (tc >= dMax && tr >= dMax) {
+                        x = x + w * dMax;
+                        break;
+                    }
+                    const V3 xs = x + w * fminf(tc, tr);
+                    bool collide;
+                    if (tc <= tr) {
+                        collide = true;
+                    } else {
+                        const float rho = density(grid, P, rng, xs);
+                        collide = rng.uniform() * muR < P.extinction * rho - muC;
+                    }
+                    if (collide) {
+                        x = xs;
+                        if (rng.uniform() < absorptionAlbedo) return v3(0.0f);
+                        w = samplePhase(rng, P.phaseG, w);
+                        tr = 0.0f;
+                        tc = rng.exponential() / muC;
+                        rayBoxIntersect(minP, maxP, x, w, a, b);
+                        dMax = b - a;
+                    }
+                }
+            }
+            const V3 cc = (minP + maxP) * 0.5f;
+            V3 mov = x + w * 0.00001f - cc;
+            const V3 sm = v3(signf(mov.x), signf(mov.y), signf(mov.z));
+            mov = mov * sm;
+            ix += ((mov.x >= mov.y && mov.x >= mov.z) ? 1 : 0) * int(sm.x);
+            iy += ((mov.y >= mov.x && mov.y >= mov.z) ? 1 : 0) * int(sm.y);
+            iz += ((mov.z >= mov.x && mov.z >= mov.y) ? 1 : 0) * int(sm.z);
+        }
+    }
+    return skyRadiance(P, w);
+}
+
+struct RefDda {
+    int step[3];
+    float tMax[3], tDelta[3];
+    int idx[3];
+};
+
+CRVPT_DEV bool refDdaInit(const SceneParams& P, const SupervoxelGrid& S, V3 x, V3 w, float dTotal, RefDda& q) {
+    const V3 dims = gridDims(P);
+    const V3 boxDelta = v3(P.boxMax) - v3(P.boxMin);
+    const V3 sp = (x - v3(P.boxMin)) / boxDelta * dims / v3(float(S.refSize));
+    const V3 ep = (x + w * dTotal - v3(P.boxMin)) / boxDelta * dims / v3(float(S.refSize));
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const float s0 = comp(sp, a), e0 = comp(ep, a);
+        const int st = int(signf(e0 - s0));
+        const float td = st != 0 ? fminf(float(st) / (e0 - s0), 1e7f) : 1e7f;
+        q.step[a] = st;
+        q.tDelta[a] = td;
+        q.tMax[a] = st > 0 ? td * (1.0f - fractf(s0)) : td * fractf(s0);
+        q.idx[a] = int(floorf(s0));
+    }
+    return !(q.step[0] == 0 && q.step[1] == 0 && q.step[2] == 0);
+}
+
+CRVPT_DEV void refDdaAdvance(RefDda& q) {
+    int a;
+    if (q.tMax[0] < q.tMax[1]) a = q.tMax[0] < q.tMax[2] ? 0 : 2;
+    else a = q.tMax[1] < q.tMax[2] ? 1 : 2;
+    q.idx[a] += q.step[a];
+    q.tMax[a] += q.tDelta[a];
+}
+
+CRVPT_DEV void refSvBounds(const SceneParams& P, const SupervoxelGrid& S, const int idx[3], V3& mn, V3& mx) {
+    const V3 dims = gridDims(P);
+    const V3 boxDelta = v3(P.boxMax) - v3(P.boxMin);
+    const V3 lo = v3(float(idx[0] * S.refSize), float(idx[1] * S.refSize), float(idx[2] * S.refSize));
+    const V3 hi = lo + v3(float(S.refSize));
+    mn = lo / dims * boxDelta + v3(P.boxMin);
+    mx = hi / dims * boxDelta + v3(P.boxMin);
+}
+
+template <class Grid>
+CRVPT_DEV float rrtEstimator(const SceneParams& P, const Grid& grid, Rng& rng, V3 x, V3 w, float dStart, float dEnd,
+                             float T, float& wSum, float& resT, float& resDist, float muC, float muR) {
+    const float Tc = __expf(-muC * (dEnd - dStart));
+    float Tr = 1.0f;
+    float dist = dStart;
+    while (true) {
+        const float t = rng.exponential() / muR;
+        x += w * t;
+        dist += t;
+        if (dist >= dEnd) break;
+        const float rho = density(grid, P, rng, x);
+        Tr *= 1.0f - (P.extinction * rho - muC) / muR;
+        const float Tlocal = T * Tr * __expf(-muC * (dist - dStart));
+        wSum += Tlocal;
+        if (rng.uniform() < Tlocal / wSum) {
+            resT = Tlocal;
+            resDist = dist;
+        }
+    }
+    return Tc * Tr;
+}
+
+template <class Grid>
+CRVPT_DEV V3 residualRatioPath(const SceneParams& P, const SupervoxelGrid& S, const Grid& grid, Rng& rng, V3 x,
+                               V3 w) {
+    float wSum = 0.0f, resT = 0.0f, resDist = 0.0f;
+    float T = 1.0f;
+    V3 acc = v3(0.0f);
+    const V3 eps = v3(1e-6f);
+    V3 oldX = x;
+    int iteration = 0;
+    while (true) {
+        float tMinVal, tMaxVal;
+        if (!rayBoxIntersect(v3(P.boxMin) + eps, v3(P.boxMax) - eps, x, w, tMinVal, tMaxVal)) break;
+        x += w * tMinVal;
+        oldX = x;
+        RefDda q;
+        if (!refDdaInit(P, S, x, w, tMaxVal - tMinVal, q)) break;
+        while (refInside(S, q.idx[0], q.idx[1], q.idx[2])) {
+            const size_t si = refIndex(S, q.idx[0], q.idx[1], q.idx[2]);
+            const float muC = S.rrt[2 * si], muR = S.rrt[2 * si + 1];
+            V3 mn, mx;
+            refSvBounds(P, S, q.idx, mn, mx);
+            float a = 0.0f, b = 0.0f;
+            rayBoxIntersect(mn, mx, oldX, w, a, b);
+            x = oldX + w * a;
+            T *= rrtEstimator(P, grid, rng, x, w, a, b, T, wSum, resT, resDist, muC, muR);
+            refDdaAdvance(q);
+        }
+        const float xi = rng.uniform();
+        if (xi > wSum || iteration >= 10) break;
+        acc += skyRadiance(P, w) * T;
+        iteration++;
+        T = resT;
+        x = oldX + w * resDist;
+        w = samplePhase(rng, P.phaseG, w);
+        wSum = 0.0f;
+        resT = 0.0f;
+        resDist = 0.0f;
+    }
+    acc += skyRadiance(P, w) * T;
+    return acc;
+}
+
+template <int MODE, class Grid>
+__global__ __launch_bounds__(256) void k_wholePath(SceneParams P, Grid grid, SupervoxelGrid S,
+                                                   float4* __restrict__ out, uint32_t* __restrict__ counter) {
+    const int lane = __lane_id();
+    while (true) {
+        const uint64_t all = __ballot(true);
+        const int leader = __ffsll((unsigned long long)all) - 1;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(counter, uint32_t(__popcll(all)));
+        base = __shfl(base, leader);
+        const uint32_t unit = base + uint32_t(__popcll(all & ((1ull << lane) - 1ull)));
+        if (__ballot(uint64_t(unit) < P.numUnits) == 0) break;
+        if (uint64_t(unit) < P.numUnits) {
+            Rng rng;
+            V3 x, w;
+            cameraRayOfUnit(P, unit, rng, x, w);
+            V3 c = MODE == VPT_DECOMPOSITION_TRACKING ? decompositionPath(P, S, grid, rng, x, w)
+                                                       : residualRatioPath(P, S, grid, rng, x, w);
+            out[unit] = make_float4(c.x, c.y, c.z, 1.0f);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// Accumulation: running mean over all frames since the last reset (Clouds.glsl:172-179), summed per pixel in
+// sample order, then written as [C, H, W] float32 (alpha = 1) when requested.
+// ------------------------------------------------------------------------------------------------------------
+__global__ void k_accumulate(const float4* __restrict__ paths, uint32_t npix, uint32_t frames,
+                             float4* __restrict__ acc, uint32_t prevFrames, float* __restrict__ outCHW,
+                             int channels) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npix) return;
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    for (uint32_t f = 0; f < frames; f++) {
+        const float4 v = paths[size_t(f) * npix + p];
+        sx += v.x;
+        sy += v.y;
+        sz += v.z;
+    }
+    float4 a = prevFrames == 0 ? make_float4(0.0f, 0.0f, 0.0f, 1.0f) : acc[p];
+    const float inv = 1.0f / float(prevFrames + frames);
+    a.x = (a.x * float(prevFrames) + sx) * inv;
+    a.y = (a.y * float(prevFrames) + sy) * inv;
+    a.z = (a.z * float(prevFrames) + sz) * inv;
+    a.w = 1.0f;
+    acc[p] = a;
+    if (outCHW) {
+        outCHW[p] = a.x;
+        outCHW[npix + p] = a.y;
+        outCHW[2 * size_t(npix) + p] = a.z;
+        if (channels == 4) outCHW[3 * size_t(npix) + p] = 1.0f;
+    }
+}
+
+__global__ void k_writeCHW(const float4* __restrict__ img, uint32_t npix, float* __restrict__ outCHW, int channels) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npix) return;
+    const float4 a = img[p];
+    outCHW[p] = a.x;
+    outCHW[npix + p] = a.y;
+    outCHW[2 * size_t(npix) + p] = a.z;
+    if (channels == 4) outCHW[3 * size_t(npix) + p] = a.w;
+}
+
+__global__ void k_zeroU32(uint32_t* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// Batched transmittance queries.
+// ------------------------------------------------------------------------------------------------------------
+template <class Grid>
+CRVPT_DEV float transmittanceWalker(const SceneParams& P, const SupervoxelGrid& S, const Grid& grid, Rng& rng, V3 o,
+                                    V3 d, float limit, bool binary) {
+    Walker W;
+    if (!walkerStart(W, P, S, o, d, limit)) return 1.0f;
+    float T = 1.0f;
+    while (true) {
+        const int ev = walkStep(W, rng, P, S);
+        if (ev == WALK_END) break;
+        if (ev != WALK_COLLIDE) continue;
+        const float rho = density(grid, P, rng, W.o + W.w * W.t);
+        const float pReal = P.extinction * rho / W.mu;
+        if (binary) {
+            if (rng.uniform() < pReal) return 0.0f;
+        } else {
+            T *= 1.0f - pReal;
+        }
+    }
+    return T;
+}
+
+template <class Grid>
+CRVPT_DEV float transmittanceRrt(const SceneParams& P, const SupervoxelGrid& S, const Grid& grid, Rng& rng, V3 x,
+                                 V3 w, float limit) {
+    const V3 eps = v3(1e-6f);
+    float tMinVal, tMaxVal;
+    if (!rayBoxIntersect(v3(P.boxMin) + eps, v3(P.boxMax) - eps, x, w, tMinVal, tMaxVal)) return 1.0f;
+    x += w * tMinVal;
+    const V3 oldX = x;
+    const float lim = limit - tMinVal;
+    RefDda q;
+    if (!refDdaInit(P, S, x, w, tMaxVal - tMinVal, q)) return 1.0f;
+    float T = 1.0f, wSum = 0.0f, resT = 0.0f, resDist = 0.0f;
+    while (refInside(S, q.idx[0], q.idx[1], q.idx[2])) {
+        const size_t si = refIndex(S, q.idx[0], q.idx[1], q.idx[2]);
+        const float muC = S.rrt[2 * si], muR = S.rrt[2 * si + 1];
+        V3 mn, mx;
+        refSvBounds(P, S, q.idx, mn, mx);
+        float a = 0.0f, b = 0.0f;
+        rayBoxIntersect(mn, mx, oldX, w, a, b);
+        b = fminf(b, lim);
+        if (b > a) T *= rrtEstimator(P, grid, rng, oldX + w * a, w, a, b, T, wSum, resT, resDist, muC, muR);
+        refDdaAdvance(q);
+    }
+    return T;
+}
+
+template <class Grid>
+CRVPT_DEV float transmittanceDecomposition(const SceneParams& P, const SupervoxelGrid& S, const Grid& grid, Rng& rng,
+                                           V3 x, V3 w, float limit) {
+    const V3 eps = v3(1e-6f);
+    float tMinVal, tMaxVal;
+    if (!rayBoxIntersect(v3(P.boxMin) + eps, v3(P.boxMax) - eps, x, w, tMinVal, tMaxVal)) return 1.0f;
+    const V3 dims = gridDims(P);
+    const V3 boxDelta = v3(P.boxMax) - v3(P.boxMin);
+    const V3 svSize = v3(float(S.refSize)) * boxDelta / dims;
+    x += w * tMinVal;
+    float travelled = tMinVal;
+    const V3 sp = (x - v3(P.boxMin)) / boxDelta * dims / v3(float(S.refSize));
+    int ix = int(floorf(sp.x)), iy = int(floorf(sp.y)), iz = int(floorf(sp.z));
+    while (refInside(S, ix, iy, iz)) {
+        const V3 minP = v3(P.boxMin) + v3(float(ix), float(iy), float(iz)) * svSize;
+        const V3 maxP = minP + svSize;
+        float a = 0.0f, b = 0.0f;
+        rayBoxIntersect(minP, maxP, x, w, a, b);
+        const float dMax = b - a;
+        x += w * a;
+        travelled += a;
+        const float remaining = limit - travelled;
+        if (remaining <= 0.0f) break;
+        const float dUse = fminf(dMax, remaining);
+        const size_t si = refIndex(S, ix, iy, iz);
+        const float mmMin = S.decomp[2 * si], mmMax = S.decomp[2 * si + 1];
+        if (mmMax >= 1e-5f) {
+            const float muC = fmaxf(0.0000000001f, P.extinction * mmMin);
+            const float muR = fmaxf(0.0000000001f, P.extinction * mmMax - muC);
+            const float tc = rng.exponential() / muC;
+            float tr = 0.0f;
+            while (true) {
+                tr += rng.exponential() / muR;
+                if (tc >= dUse && tr >= dUse) break;
+                if (tc <= tr) return 0.0f;
+                const float rho = density(grid, P, rng, x + w * tr);
+                if (rng.uniform() * muR < P.extinction * rho - muC) return 0.0f;
+            }
+        }
+        if (dUse < dMax) break;
+        x = x + w * dMax;
+        travelled += dMax;
+        const V3 cc = (minP + maxP) * 0.5f;
+        V3 mov = x + w * 0.00001f - cc;
+        const V3 sm = v3(signf(mov.x), signf(mov.y), signf(mov.z));
+        mov = mov * sm;
+        ix += ((mov.x >= mov.y && mov.x >= mov.z) ? 1 : 0) * int(sm.x);
+        iy += ((mov.y >= mov.x && mov.y >= mov.z) ? 1 : 0) * int(sm.y);
+        iz += ((mov.z >= mov.x && mov.z >= mov.y) ? 1 : 0) * int(sm.z);
+    }
+    return 1.0f;
+}
+
+template <class Grid>
+__global__ __launch_bounds__(256) void k_transmittance(SceneParams P, Grid grid, SupervoxelGrid S, int estimator,
+                                                       const float* __restrict__ o, const float* __restrict__ d,
+                                                       const float* __restrict__ tmax, int nRays, int nSamples,
+                                                       uint32_t seed, float* __restrict__ mean,
+                                                       float* __restrict__ m2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nRays) return;
+    const V3 x = v3(o[3 * i], o[3 * i + 1], o[3 * i + 2]);
+    const V3 w = v3(d[3 * i], d[3 * i + 1], d[3 * i + 2]);
+    const float lim = tmax ? tmax[i] : kInf;
+    double s1 = 0.0, s2 = 0.0;
+    for (int s = 0; s < nSamples; s++) {
+        Rng rng;
+        rng.init(uint32_t(i), uint32_t(s), seed ^ 0x5bd1e995u);
+        float T;
+        switch (estimator) {
+            case T_RATIO: T = transmittanceWalker(P, S, grid, rng, x, w, lim, false); break;
+            case T_RESIDUAL_RATIO: T = transmittanceRrt(P, S, grid, rng, x, w, lim); break;
+            case T_DECOMPOSITION: T = transmittanceDecomposition(P, S, grid, rng, x, w, lim); break;
+            default: T = transmittanceWalker(P, S, grid, rng, x, w, lim, true); break;
+        }
+        s1 += T;
+        s2 += double(T) * T;
+    }
+    mean[i] = float(s1 / nSamples);
+    if (m2) m2[i] = float(s2 / nSamples);
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// Supervoxel tables
+// ------------------------------------------------------------------------------------------------------------
+// Majorant: max normalised density over the supervoxel plus a one-voxel halo (the voxels a jittered or filtered
+// lookup of a point inside the supervoxel can read), including the border value outside the grid.
+template <class Grid>
+__global__ void k_majorant(Grid grid, int svSize, int gx, int gy, int gz, float* __restrict__ maj) {
+    const int si = blockIdx.x * blockDim.x + threadIdx.x;
+    if (si >= gx * gy * gz) return;
+    const int sx = si % gx, sy = (si / gx) % gy, sz = si / (gx * gy);
+    float m = 0.0f;
+    for (int k = sz * svSize - 1; k <= (sz + 1) * svSize; k++)
+        for (int j = sy * svSize - 1; j <= (sy + 1) * svSize; j++)
+            for (int i = sx * svSize - 1; i <= (sx + 1) * svSize; i++) m = fmaxf(m, grid.voxelAt(i, j, k));
+    maj[si] = m;
+}
+
+// Reference tables (SuperVoxelGrid.cpp:101-211, 269-405): min, max and average of the normalised density over
+// S^3 (NEAREST) or the (S+2)^3 halo (other modes), zero outside the grid, summed in the reference's z, y, x order.
+template <class Grid>
+__global__ void k_refSupervoxelStats(Grid grid, int dimX, int dimY, int dimZ, int svSize, int gx, int gy, int gz,
+                                     int halo, float* __restrict__ mn, float* __restrict__ mx,
+                                     float* __restrict__ avg) {
+    const int si = blockIdx.x * blockDim.x + threadIdx.x;
+    if (si >= gx * gy * gz) return;
+    const int sx = si % gx, sy = (si / gx) % gy, sz = si / (gx * gy);
+    float dmin = 3.402823466e38f, dmax = -3.402823466e38f, sum = 0.0f;
+    int count = 0;
+    const int lo = halo ? -1 : 0, hi = halo ? svSize : svSize - 1;
+    for (int oz = lo; oz <= hi; oz++)
+        for (int oy = lo; oy <= hi; oy++)
+            for (int ox = lo; ox <= hi; ox++) {
+                const int x = sx * svSize + ox, y = sy * svSize + oy, z = sz * svSize + oz;
+                float v = 0.0f;
+                if (x >= 0 && y >= 0 && z >= 0 && x < dimX && y < dimY && z < dimZ) v = grid.voxelAt(x, y, z);
+                dmin = fminf(dmin, v);
+                dmax = fmaxf(dmax, v);
+                sum += v;
+                count++;
+            }
+    mn[si] = dmin;
+    mx[si] = dmax;
+    avg[si] = sum / float(count);
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// Procedural clouds: seeded fBm value noise (the build's generator; parameters documented with the entry point).
+// Contraction is off so every FP32 operation rounds exactly as in the CPU restatement of the generator.
+// ------------------------------------------------------------------------------------------------------------
+CRVPT_DEV uint32_t lowbias32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+CRVPT_DEV float latticeValue(int ix, int iy, int iz, uint32_t salt) {
+    const uint32_t h = lowbias32((uint32_t(ix) * 73856093u) ^ (uint32_t(iy) * 19349663u) ^
+                                 (uint32_t(iz) * 83492791u) ^ salt);
+    return float(h >> 8) * (1.0f / 16777216.0f);
+}
+
+CRVPT_DEV float fadeq(float t) {
+#pragma clang fp contract(off)
+    return t * t * t * (t * (t * 6.0f - 15.0f) + 10.0f);
+}
+
+CRVPT_DEV float valueNoise(float px, float py, float pz, uint32_t salt) {
+#pragma clang fp contract(off)
+    const float fx = floorf(px), fy = floorf(py), fz = floorf(pz);
+    const int ix = int(fx), iy = int(fy), iz = int(fz);
+    const float u = fadeq(px - fx), v = fadeq(py - fy), w = fadeq(pz - fz);
+    float c[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) c[k] = latticeValue(ix + (k & 1), iy + ((k >> 1) & 1), iz + ((k >> 2) & 1), salt);
+    const float x00 = c[0] + (c[1] - c[0]) * u;
+    const float x10 = c[2] + (c[3] - c[2]) * u;
+    const float x01 = c[4] + (c[5] - c[4]) * u;
+    const float x11 = c[6] + (c[7] - c[6]) * u;
+    const float y0 = x00 + (x10 - x00) * v;
+    const float y1 = x01 + (x11 - x01) * v;
+    return y0 + (y1 - y0) * w;
+}
+
+__global__ void k_fbm(float* __restrict__ out, int nx, int ny, int nz, FbmParams F) {
+#pragma clang fp contract(off)
+    const size_t total = size_t(nx) * ny * nz;
+    for (size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < total;
+         idx += size_t(gridDim.x) * blockDim.x) {
+        const int x = int(idx % nx);
+        const int y = int((idx / nx) % ny);
+        const int z = int(idx / (size_t(nx) * ny));
+        const float px = (float(x) + 0.5f) / float(nx);
+        const float py = (float(y) + 0.5f) / float(ny);
+        const float pz = (float(z) + 0.5f) / float(nz);
+        float sum = 0.0f, norm = 0.0f, amp = 1.0f, freq = F.baseFrequency;
+        for (int o = 0; o < F.octaves; o++) {
+            const uint32_t salt = F.seed * 2654435761u + uint32_t(o) * 0x9E3779B9u;
+            sum = sum + amp * valueNoise(px * freq, py * freq, pz * freq, salt);
+            norm = norm + amp;
+            amp = amp * F.gain;
+            freq = freq * F.lacunarity;
+        }
+        float v = sum / norm;
+        if (F.heightFalloff) {
+            // anvil profile: flat top, rounded base, in the normalised height py
+            const float h = py;
+            const float base = fminf(fmaxf((h - 0.05f) / 0.25f, 0.0f), 1.0f);
+            const float top = fminf(fmaxf((0.95f - h) / 0.1f, 0.0f), 1.0f);
+            const float spread = 1.0f + 0.5f * fminf(fmaxf((h - 0.6f) / 0.3f, 0.0f), 1.0f);
+            v = v * base * top * spread;
+        }
+        float t = (v - F.edge0) / (F.edge1 - F.edge0);
+        t = fminf(fmaxf(t, 0.0f), 1.0f);
+        out[idx] = t * t * (3.0f - 2.0f * t);
+    }
+}
+
+// Dense grid normalisation: (raw - vmin) / (vmax - vmin) per voxel, as sampleGridImage applies it per lookup.
+__global__ void k_normalize(const float* __restrict__ raw, float* __restrict__ out, size_t n, float vmin,
+                            float vmax) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+        out[i] = (raw[i] - vmin) / (vmax - vmin);
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// Host-side launchers
+// ------------------------------------------------------------------------------------------------------------
+template <class Grid>
+static hipError_t launchRenderT(const SceneParams& P, const Grid& grid, const SupervoxelGrid& S, float4* paths,
+                                uint32_t* counter, const LaunchConfig& cfg, unsigned long long* stats,
+                                hipStream_t stream) {
+    hipLaunchKernelGGL(k_zeroU32, dim3(1), dim3(64), 0, stream, counter, 1);
+    const dim3 block(256);
+    const dim3 gridDim(cfg.numBlocks);
+    switch (P.mode) {
+        case VPT_RATIO_TRACKING:
+            hipLaunchKernelGGL((k_pathPool<VPT_RATIO_TRACKING, Grid>), gridDim, block, 0, stream, P, grid, S, paths,
+                               counter, cfg.stepsPerRefill, stats);
+            break;
+        case VPT_NEXT_EVENT_TRACKING:
+            hipLaunchKernelGGL((k_pathPool<VPT_NEXT_EVENT_TRACKING, Grid>), gridDim, block, 0, stream, P, grid, S,
+                               paths, counter, cfg.stepsPerRefill, stats);
+            break;
+        case VPT_DECOMPOSITION_TRACKING:
+            hipLaunchKernelGGL((k_wholePath<VPT_DECOMPOSITION_TRACKING, Grid>), gridDim, block, 0, stream, P, grid,
+                               S, paths, counter);
+            break;
+        case VPT_RESIDUAL_RATIO_TRACKING:
+            hipLaunchKernelGGL((k_wholePath<VPT_RESIDUAL_RATIO_TRACKING, Grid>), gridDim, block, 0, stream, P,
+                               grid, S, paths, counter);
+            break;
+        default:
+            hipLaunchKernelGGL((k_pathPool<VPT_DELTA_TRACKING, Grid>), gridDim, block, 0, stream, P, grid, S, paths,
+                               counter, cfg.stepsPerRefill, stats);
+            break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launchRender(const SceneParams& P, const DenseGrid* dense, const NanoGrid* nano, const SupervoxelGrid& S,
+                        float4* paths, uint32_t* counter, const LaunchConfig& cfg, unsigned long long* stats,
+                        hipStream_t stream) {
+    if (nano) return launchRenderT(P, NanoAccess{*nano}, S, paths, counter, cfg, stats, stream);
+    return launchRenderT(P, DenseAccess{*dense}, S, paths, counter, cfg, stats, stream);
+}
+
+hipError_t launchAccumulate(const float4* paths, uint32_t npix, uint32_t frames, float4* acc, uint32_t prevFrames,
+                            float* outCHW, int channels, hipStream_t stream) {
+    hipLaunchKernelGGL(k_accumulate, dim3((npix + 255) / 256), dim3(256), 0, stream, paths, npix, frames, acc,
+                       prevFrames, outCHW, channels);
+    return hipGetLastError();
+}
+
+hipError_t launchWriteCHW(const float4* img, uint32_t npix, float* outCHW, int channels, hipStream_t stream) {
+    hipLaunchKernelGGL(k_writeCHW, dim3((npix + 255) / 256), dim3(256), 0, stream, img, npix, outCHW, channels);
+    return hipGetLastError();
+}
+
+hipError_t launchTransmittance(const SceneParams& P, const DenseGrid* dense, const NanoGrid* nano,
+                               const SupervoxelGrid& S, int estimator, const float* o, const float* d,
+                               const float* tmax, int nRays, int nSamples, uint32_t seed, float* mean, float* m2,
+                               hipStream_t stream) {
+    const dim3 grid((nRays + 255) / 256), block(256);
+    if (nano) {
+        hipLaunchKernelGGL(k_transmittance<NanoAccess>, grid, block, 0, stream, P, NanoAccess{*nano}, S, estimator,
+                           o, d, tmax, nRays, nSamples, seed, mean, m2);
+    } else {
+        hipLaunchKernelGGL(k_transmittance<DenseAccess>, grid, block, 0, stream, P, DenseAccess{*dense}, S,
+                           estimator, o, d, tmax, nRays, nSamples, seed, mean, m2);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launchMajorant(const DenseGrid* dense, const NanoGrid* nano, const int dims[3], int svSize,
+                          const int grid3[3], float* maj, hipStream_t stream) {
+    (void)dims;
+    const int n = grid3[0] * grid3[1] * grid3[2];
+    if (nano) {
+        hipLaunchKernelGGL(k_majorant<NanoAccess>, dim3((n + 127) / 128), dim3(128), 0, stream, NanoAccess{*nano},
+                           svSize, grid3[0], grid3[1], grid3[2], maj);
+    } else {
+        hipLaunchKernelGGL(k_majorant<DenseAccess>, dim3((n + 127) / 128), dim3(128), 0, stream,
+                           DenseAccess{*dense}, svSize, grid3[0], grid3[1], grid3[2], maj);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launchRefSupervoxelStats(const DenseGrid* dense, const NanoGrid* nano, const int dims[3], int svSize,
+                                    const int grid3[3], int halo, float* mn, float* mx, float* avg,
+                                    hipStream_t stream) {
+    const int n = grid3[0] * grid3[1] * grid3[2];
+    if (nano) {
+        hipLaunchKernelGGL(k_refSupervoxelStats<NanoAccess>, dim3((n + 63) / 64), dim3(64), 0, stream,
+                           NanoAccess{*nano}, dims[0], dims[1], dims[2], svSize, grid3[0], grid3[1], grid3[2], halo,
+                           mn, mx, avg);
+    } else {
+        hipLaunchKernelGGL(k_refSupervoxelStats<DenseAccess>, dim3((n + 63) / 64), dim3(64), 0, stream,
+                           DenseAccess{*dense}, dims[0], dims[1], dims[2], svSize, grid3[0], grid3[1], grid3[2],
+                           halo, mn, mx, avg);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launchFbm(float* out, int nx, int ny, int nz, const FbmParams& F, hipStream_t stream) {
+    const size_t total = size_t(nx) * ny * nz;
+    const size_t blocks = std::min<size_t>((total + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_fbm, dim3(unsigned(blocks)), dim3(256), 0, stream, out, nx, ny, nz, F);
+    return hipGetLastError();
+}
+
+hipError_t launchNormalize(const float* raw, float* out, size_t n, float vmin, float vmax, hipStream_t stream) {
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_normalize, dim3(unsigned(blocks)), dim3(256), 0, stream, raw, out, n, vmin, vmax);
+    return hipGetLastError();
+}
+
+int occupancyBlocksPerCU(int mode, bool sparse) {
+    int blocks = 0;
+    hipError_t e;
+    if (sparse) {
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &blocks,
+            mode == VPT_NEXT_EVENT_TRACKING ? (const void*)k_pathPool<VPT_NEXT_EVENT_TRACKING, NanoAccess>
+                                            : (const void*)k_pathPool<VPT_DELTA_TRACKING, NanoAccess>,
+            256, 0);
+    } else {
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &blocks,
+            mode == VPT_NEXT_EVENT_TRACKING ? (const void*)k_pathPool<VPT_NEXT_EVENT_TRACKING, DenseAccess>
+                                            : (const void*)k_pathPool<VPT_DELTA_TRACKING, DenseAccess>,
+            256, 0);
+    }
+    if (e != hipSuccess || blocks <= 0) return 4;
+    return blocks;
+}
+
+}  // namespace crvpt

@@ -1,0 +1,60 @@
+// Host-callable launchers of the gfx950 kernels (defined in render.hip, used by the context in api.cpp).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "vpt_types.h"
+
+namespace crvpt {
+
+struct LaunchConfig {
+    uint32_t numBlocks;      // persistent workgroups of 256 threads
+    int32_t stepsPerRefill;  // walker events per lane between two refill ballots (the compaction interval)
+};
+
+// Seeded fBm value-noise cloud: sum over octaves o < octaves of gain^o * noise(p * baseFrequency * lacunarity^o),
+// normalised by the sum of weights, optionally shaped by an anvil height profile, then smoothstep(edge0, edge1, .).
+struct FbmParams {
+    uint32_t seed;
+    int32_t octaves;
+    float lacunarity;
+    float gain;
+    float baseFrequency;
+    float edge0;
+    float edge1;
+    int32_t heightFalloff;
+};
+
+// stats (nullable): 64-bit device counters [density taps, supervoxel majorant reads, paths], accumulated.
+hipError_t launchRender(const SceneParams& P, const DenseGrid* dense, const NanoGrid* nano, const SupervoxelGrid& S,
+                        float4* paths, uint32_t* counter, const LaunchConfig& cfg, unsigned long long* stats,
+                        hipStream_t stream);
+hipError_t launchAccumulate(const float4* paths, uint32_t npix, uint32_t frames, float4* acc, uint32_t prevFrames,
+                            float* outCHW, int channels, hipStream_t stream);
+hipError_t launchWriteCHW(const float4* img, uint32_t npix, float* outCHW, int channels, hipStream_t stream);
+hipError_t launchTransmittance(const SceneParams& P, const DenseGrid* dense, const NanoGrid* nano,
+                               const SupervoxelGrid& S, int estimator, const float* o, const float* d,
+                               const float* tmax, int nRays, int nSamples, uint32_t seed, float* mean, float* m2,
+                               hipStream_t stream);
+hipError_t launchMajorant(const DenseGrid* dense, const NanoGrid* nano, const int dims[3], int svSize,
+                          const int grid3[3], float* maj, hipStream_t stream);
+hipError_t launchRefSupervoxelStats(const DenseGrid* dense, const NanoGrid* nano, const int dims[3], int svSize,
+                                    const int grid3[3], int halo, float* mn, float* mx, float* avg,
+                                    hipStream_t stream);
+hipError_t launchFbm(float* out, int nx, int ny, int nz, const FbmParams& F, hipStream_t stream);
+hipError_t launchNormalize(const float* raw, float* out, size_t n, float vmin, float vmax, hipStream_t stream);
+int occupancyBlocksPerCU(int mode, bool sparse);
+
+// Denoisers (denoise.hip)
+struct EawParams {
+    int32_t width, height;
+    int32_t iterations;
+    float phiColor, phiPosition, phiNormal;
+    int32_t useColorDistance;  // 0: the reference compute path's constant colour weight exp(-1/phiColor)
+};
+hipError_t launchEaw(const float4* color, const float4* position, const float4* normal, float4* ping, float4* pong,
+                     const EawParams& p, float4** result, hipStream_t stream);
+
+}  // namespace crvpt

@@ -1,0 +1,118 @@
+// Plain-old-data types shared by the host library and the gfx950 kernels of the volumetric path tracer.
+// Every struct is fixed-size and passed by value to a launch; device pointers inside point at HBM allocations
+// owned by the host context.
+#pragma once
+
+#include <stdint.h>
+
+namespace crvpt {
+
+// VptMode of the reference (src/PathTracer/RenderSettings.hpp:32-36); values follow the reference's enum order.
+enum VptMode : int32_t {
+    VPT_DELTA_TRACKING = 0,
+    VPT_SPECTRAL_DELTA_TRACKING = 1,
+    VPT_RATIO_TRACKING = 2,
+    VPT_DECOMPOSITION_TRACKING = 3,
+    VPT_RESIDUAL_RATIO_TRACKING = 4,
+    VPT_NEXT_EVENT_TRACKING = 5,
+    VPT_NEXT_EVENT_TRACKING_SPECTRAL = 6,
+    VPT_ISOSURFACE_RENDERING = 7,
+    VPT_RAY_MARCHING_EMISSION_ABSORPTION = 8,
+};
+
+// GridInterpolationType (RenderSettings.hpp:52-56).
+enum GridInterpolation : int32_t { INTERP_NEAREST = 0, INTERP_STOCHASTIC = 1, INTERP_TRILINEAR = 2 };
+
+// Environment: builtin DEFAULT / BLACK / SINGLE_COLOR (RenderSettings.hpp:118-121) or an equirectangular image.
+enum EnvKind : int32_t { ENV_DEFAULT = 0, ENV_BLACK = 1, ENV_SINGLE_COLOR = 2, ENV_IMAGE = 3 };
+
+// Transmittance estimators for batched ray queries.
+enum TransmittanceEstimator : int32_t {
+    T_DELTA = 0,           // binary: any real collision ends the ray
+    T_RATIO = 1,           // T *= 1 - sigma/majorant per tentative collision
+    T_RESIDUAL_RATIO = 2,  // exp(-mu_c d) times the residual ratio product, per supervoxel
+    T_DECOMPOSITION = 3,   // binary, control and residual competing exponentials per supervoxel
+};
+
+constexpr int kMaxLights = 8;
+
+// std140 light record of the reference (VptHeader.glsl:233-248), 64 bytes.
+struct Light {
+    uint32_t lightType;   // 0 point, 1 spot, 2 directional
+    uint32_t lightSpace;  // 0 world, 1 view, 2 view orientation
+    float spotTotalWidth;
+    float spotFalloffStart;
+    float color[3];
+    float intensity;
+    float position[3];    // direction (its length is the shadow-ray distance) for directional lights
+    uint32_t useDistance;
+    float spotDirection[3];
+    float padding;
+};
+static_assert(sizeof(Light) == 64, "Light must match the 64-byte std140 record");
+
+// Dense grid: normalised FP32 values, x fastest, resident in HBM.
+struct DenseGrid {
+    const float* values;
+    int32_t dim[3];
+    float borderValue;  // normalised value of the black border texel: (0 - vmin) / (vmax - vmin)
+};
+
+// Sparse grid: a NanoVDB 32.3 buffer with FP32 leaves, resident in HBM, read with 32-bit byte offsets.
+struct NanoGrid {
+    const uint8_t* buf;
+    uint32_t rootOffset;  // byte offset of the root node
+    uint32_t tileCount;
+    float invMat[9];      // world -> index map, FP32 copy (Map::mInvMatF, row-major)
+    float vec[3];         // Map::mVecF
+    float background;
+    int32_t indexMin[3];  // index bounding box minimum: texel (0,0,0) of the dense-equivalent grid
+};
+
+// Supervoxel tables. The majorant table drives the build's local-majorant free-flight sampling; the RRT and
+// decomposition tables are the reference's (SuperVoxelGrid.cpp) and feed those estimators.
+struct SupervoxelGrid {
+    const float* majorant;  // per supervoxel: max normalised density reachable by a lookup (halo for jitter/filter)
+    const float* rrt;       // (mu_c, mu_r_bar) pairs, reference RRT table
+    const float* decomp;    // (min, max) pairs, reference decomposition table
+    int32_t size;           // supervoxel edge in voxels (majorant grid)
+    int32_t grid[3];        // majorant grid dimensions
+    int32_t refSize;        // supervoxel edge of the reference tables
+    int32_t refGrid[3];
+};
+
+// Per-launch scene constants (the build's counterpart of the reference's UniformData, VptHeader.glsl:45-112).
+struct SceneParams {
+    float invViewProj[16];  // column-major
+    float invView[16];      // column-major
+    float boxMin[3];
+    float boxMax[3];
+    float worldToTex[3];    // grid dim / (boxMax - boxMin): texel coordinates of a world position
+    float gridMin[3];       // sparse grids: NanoVDB world bounding box the box maps onto (sampleCloud)
+    float gridMax[3];
+    float extinction;       // extinction.x = majorant of the reference trackers
+    float albedo;           // scatteringAlbedo.x
+    float phaseG;
+    int32_t mode;
+    int32_t interpolation;
+    int32_t envKind;
+    int32_t useLinearRGB;
+    uint32_t isEnvMapBlack;
+    int32_t useMajorantGrid;
+    float sunDirection[3];
+    float sunIntensity[3];  // sunlightIntensity * sunlightColor
+    float envFactor[3];
+    const float* envImage;  // RGBA32F rows of envW texels
+    int32_t envW, envH;
+    int32_t numLights;
+    int32_t sparse;         // 1: NanoVDB grid
+    Light lights[kMaxLights];
+    uint32_t seedOffset;
+    int32_t width, height;
+    uint32_t frameBase;     // global frame number of sample 0 of this launch
+    uint32_t numFrames;     // samples per pixel in this launch
+    uint32_t pad1;
+    uint64_t numUnits;      // width * height * numFrames
+};
+
+}  // namespace crvpt

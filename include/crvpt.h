@@ -1,0 +1,149 @@
+/*
+ * crvpt — C ABI of the MI355X (gfx950) volumetric path tracer.
+ *
+ * The entry points mirror the reference's PyTorch module `vpt` (src/PyTorch/Module.cpp, stub vpt.pyi) one to one:
+ * same names with a crvpt_ prefix, same argument meaning and order, plain pointers and sizes instead of torch
+ * tensors and std::vector. Every function returns 0 on success and a non-zero CRVPT_ERR_* code on failure;
+ * crvpt_last_error() then returns the message the reference would have raised through sgl::Logfile::throwError.
+ * One global, reference-counted renderer per process, as in the reference (Module.cpp:283-284, 701, 732-813);
+ * calls are not thread-safe (the reference has no locks either).
+ */
+#ifndef CRVPT_H
+#define CRVPT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    CRVPT_OK = 0,
+    CRVPT_ERR_INVALID_ARGUMENT = 1,
+    CRVPT_ERR_NOT_INITIALIZED = 2,
+    CRVPT_ERR_HIP = 3,
+    CRVPT_ERR_IO = 4,
+    CRVPT_ERR_UNSUPPORTED = 5,
+};
+
+/* Last error message of the calling process ("" when none). */
+const char* crvpt_last_error(void);
+
+/* Module.cpp:732-802 initialize(): ref-counted; the first call creates the renderer with the reference defaults
+ * (fog-sphere scene of radius 0.25 and voxel size 0.01, DELTA_TRACKING, linear RGB). */
+int crvpt_initialize(void);
+/* Module.cpp:804-813 cleanup(): drops one reference; the last one frees all device memory. */
+int crvpt_cleanup(void);
+
+/* Module.cpp:286-317 render_frame(input_tensor[C,H,W] f32, frame_count): renders frame_count samples per pixel
+ * into the running mean kept since the last reset and writes the image as C planes of H*W floats (C = 3 or 4,
+ * alpha = 1). out_on_device != 0: `out` is a device pointer written on `stream` (a hipStream_t, 0 = null stream)
+ * without host round trips, as the CUDA branch (Module.cpp:307-309); otherwise `out` is host memory and the
+ * call returns after the copy, as renderFrameCpu. */
+int crvpt_render_frame(float* out, int channels, int height, int width, int frame_count, int out_on_device,
+                       void* stream);
+
+/* Module.cpp:815-824 load_cloud_file / load_volume_file (.xyz dense float grids, CloudData.cpp:236-316;
+ * .nvdb NanoVDB float grids, CloudData.cpp:1360-1365). */
+int crvpt_load_cloud_file(const char* filename);
+/* CloudData::setDensityField (CloudData.cpp:147-167): dense grid, x fastest, size nx*ny*nz (copied). */
+int crvpt_set_density_field(int nx, int ny, int nz, const float* values);
+/* CloudData::setNanoVdbGridHandle (CloudData.cpp:169-172): a NanoVDB 32.3 FP32 grid buffer (copied). */
+int crvpt_set_nanovdb_buffer(const uint8_t* data, uint64_t size);
+/* No reference counterpart: seeded fBm procedural cloud generated on the GPU as the dense density field
+ * (octaves, lacunarity, gain, base frequency over the unit cube, smoothstep(edge0, edge1) remap, optional anvil
+ * height profile). */
+int crvpt_generate_fbm_cloud(int nx, int ny, int nz, uint32_t seed, int octaves, float lacunarity, float gain,
+                             float base_frequency, float edge0, float edge1, int height_falloff);
+/* Copies the current dense field (normalised or raw) or the current NanoVDB buffer to host memory. */
+int crvpt_get_density_field(float* out, uint64_t count, int normalized);
+int crvpt_get_nanovdb_buffer(uint8_t* out, uint64_t capacity, uint64_t* size);
+
+/* VolumetricPathTracingPass::setUseSparseGrid / setSparseGridInterpolationType (VolumetricPathTracingPass.cpp:
+ * 863-875); interpolation: 0 nearest, 1 stochastic, 2 trilinear (RenderSettings.hpp:52-56). */
+int crvpt_set_use_sparse_grid(int use_sparse);
+int crvpt_set_grid_interpolation_type(int type);
+
+/* Module.cpp set_vpt_mode / set_vpt_mode_from_name (VptMode order and names of RenderSettings.hpp:32-45). */
+int crvpt_set_vpt_mode(int64_t mode);
+int crvpt_set_vpt_mode_from_name(const char* mode_name);
+
+/* Module.cpp medium setters: set_scattering_albedo(vec3), set_extinction_scale, set_extinction_base(vec3),
+ * set_phase_g, set_seed_offset. */
+int crvpt_set_scattering_albedo(const double albedo[3]);
+int crvpt_set_extinction_scale(double extinction_scale);
+int crvpt_set_extinction_base(const double extinction_base[3]);
+int crvpt_set_phase_g(double phase_g);
+int crvpt_set_seed_offset(int64_t offset);
+
+/* Module.cpp camera API: get_camera_position, get_camera_view_matrix (16, column-major like glm),
+ * get_camera_fovy (radians), set_camera_position, set_camera_target, overwrite_camera_view_matrix,
+ * set_camera_fovy (degrees, Module.cpp:1386). */
+int crvpt_get_camera_position(double out[3]);
+int crvpt_get_camera_view_matrix(double out[16]);
+int crvpt_get_camera_fovy(double* out);
+int crvpt_set_camera_position(const double position[3]);
+int crvpt_set_camera_target(const double target[3]);
+int crvpt_overwrite_camera_view_matrix(const double view[16]);
+int crvpt_set_camera_fovy(double fovy_degrees);
+
+/* Module.cpp environment API: set_use_builtin_environment_map("Default" | "Black" | "Single Color"),
+ * load_environment_map_from_tensor (float32 [C=3|4, H, W] rows top to bottom, linear RGB),
+ * set_environment_map_intensity, set_environment_map_intensity_rgb. */
+int crvpt_set_use_builtin_environment_map(const char* name);
+int crvpt_load_environment_map_from_buffer(const float* pixels_chw, int channels, int height, int width);
+int crvpt_set_environment_map_intensity(double factor);
+int crvpt_set_environment_map_intensity_rgb(const double factor[3]);
+
+/* Module.cpp light API (LightEditorWidget.cpp:280-320): set_use_lights, clear_lights, add_light, remove_light,
+ * set_light_property(idx, key, value) with keys type, color, intensity, use_distance, position, direction, space,
+ * spot_total_width, spot_falloff_start, spot_direction. */
+int crvpt_set_use_lights(int use_lights);
+int crvpt_clear_lights(void);
+int crvpt_add_light(void);
+int crvpt_remove_light(int64_t light_idx);
+int crvpt_set_light_property(int64_t light_idx, const char* key, const char* value);
+int crvpt_get_num_lights(int64_t* out);
+
+/* Module.cpp volume placement: set_max_grid_extent, get_render_bounding_box (xmin, xmax, ymin, ymax, zmin, zmax),
+ * get_volume_voxel_size (z, y, x). */
+int crvpt_set_max_grid_extent(double max_grid_extent);
+int crvpt_get_render_bounding_box(double out[6]);
+int crvpt_get_volume_voxel_size(int64_t out[3]);
+
+/* Module.cpp denoiser API: set_denoiser(name) with "None", "Default" (EAW), "EAW", "SVGF"; set_denoiser_property
+ * (EAW: maxNumIterations, phiColor, phiPosition, phiNormal, useColorDistance; SVGF: numIterations). */
+int crvpt_set_denoiser(const char* name);
+int crvpt_set_denoiser_property(const char* key, const char* value);
+
+/* No reference counterpart: batched transmittance along n_rays segments o + t d, t in [0, t_max] (t_max may be
+ * NULL: to the box exit), n_samples estimates per ray with estimator 0 delta, 1 ratio, 2 residual ratio,
+ * 3 decomposition. All arrays are device pointers when on_device != 0, host otherwise; mean and m2 receive the
+ * per-ray mean and second moment (m2 may be NULL). */
+int crvpt_transmittance_batch(int estimator, int n_rays, const float* origins, const float* directions,
+                              const float* t_max, int n_samples, uint32_t seed, float* mean, float* m2,
+                              int on_device, void* stream);
+
+/* No reference counterpart: performance controls of the gfx950 path pool. use_majorant_grid: local majorants
+ * from the supervoxel grid (1, default) or the reference's global majorant (0); supervoxel_size: edge in voxels of
+ * the majorant grid (default 8 dense, 16 sparse when 0); blocks: persistent workgroups (0 = occupancy based);
+ * steps_per_refill: walker events between refill ballots. */
+int crvpt_set_use_majorant_grid(int use_majorant_grid);
+int crvpt_set_launch_config(int supervoxel_size, int blocks, int steps_per_refill);
+
+/* No reference counterpart: statistics of the last crvpt_render_frame: kernel milliseconds of the path kernels
+ * (device events on the render stream), number of paths, number of path-kernel launches. */
+int crvpt_get_last_render_stats(double* path_kernel_ms, double* total_ms, uint64_t* num_paths, int* launches);
+
+/* No reference counterpart: work counters of the path kernels for the algorithmic-byte roofline. When enabled,
+ * each crvpt_render_frame accumulates [density taps, supervoxel majorant reads, paths] (wave-reduced atomics at
+ * kernel exit); crvpt_get_last_render_counters reads the totals of the last call. */
+int crvpt_set_collect_stats(int collect);
+int crvpt_get_last_render_counters(uint64_t out[3]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CRVPT_H */

@@ -1,0 +1,265 @@
+"""GPU parity tests of the gfx950 path tracer against the CPU oracle (run on an MI355X).
+
+The GPU uses Philox streams and the oracle the reference's hybrid Taus, so per-path bits differ by design; parity is
+statistical: image means per channel within 3 sigma (sigma from the per-pixel sample variances of both sides),
+transmittance means within 5 sigma per ray of the FP64 optical depth, plus exact checks where the answer is fixed
+(white furnace, determinism, the fBm generator bit for bit).
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+RES, SPP = 32, 64
+
+
+def _cpu(t):
+    return t.detach().cpu().numpy()
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def _render(vpt, frames, res=RES, channels=3, device="cuda"):
+    torch = _torch()
+    return vpt.render_frame(torch.zeros((channels, res, res), dtype=torch.float32, device=device), frames)
+
+
+def _render_stats(vpt, frames, batches=4, res=RES):
+    """Image mean per channel and its standard error from `batches` independent renders (seed offsets)."""
+    means = []
+    for b in range(batches):
+        vpt.set_seed_offset(1000003 * (b + 1))
+        img = _cpu(_render(vpt, frames, res))
+        means.append(img.reshape(3, -1).mean(1))
+    vpt.set_seed_offset(0)
+    means = np.array(means, np.float64)
+    return means.mean(0), means.std(0, ddof=1) / math.sqrt(batches), means
+
+
+def _oracle_stats(scene, frames):
+    img, sq = scene.render(frames)
+    npix = img.shape[0] * img.shape[1]
+    var = np.maximum(sq / frames - img.astype(np.float64) ** 2, 0.0)
+    return img.reshape(-1, 3).mean(0), np.sqrt(var.reshape(-1, 3).sum(0) / frames) / npix
+
+
+def _set_volume(vpt, density):
+    vpt.set_density_field(density)
+
+
+def _common(vpt, mode, interp="stochastic", ext=64.0, albedo=0.9, g=0.5):
+    vpt.set_vpt_mode(mode)
+    vpt.set_grid_interpolation_type(interp)
+    vpt.set_extinction_scale(ext)
+    vpt.set_extinction_base([1.0, 1.0, 1.0])
+    vpt.set_scattering_albedo([albedo] * 3)
+    vpt.set_phase_g(g)
+    vpt.set_use_builtin_environment_map("Default")
+    vpt.set_use_lights(False)
+    vpt.set_seed_offset(0)
+
+
+def _check_equal(m_gpu, s_gpu, m_ref, s_ref, what):
+    tol = 3.0 * np.sqrt(s_gpu ** 2 + s_ref ** 2) + 1e-5
+    assert np.all(np.abs(m_gpu - m_ref) <= tol), f"{what}: gpu {m_gpu} ref {m_ref} tol {tol}"
+
+
+def test_fbm_generator_bit_exact(vpt, oracle):
+    print("case: fbm generator 40x36x28 seed 3", flush=True)
+    vpt.generate_fbm_cloud(40, 36, 28, seed=3)
+    got = vpt.get_density_field((28, 36, 40))
+    ref = oracle.fbm_cloud(40, 36, 28, seed=3)
+    np.testing.assert_array_equal(got, ref)
+
+
+def test_white_furnace(vpt, oracle):
+    print("case: white furnace delta 16^3", flush=True)
+    _set_volume(vpt, oracle.fbm_cloud(16, 16, 16, seed=1))
+    _common(vpt, 0, albedo=1.0)
+    vpt.set_use_builtin_environment_map("Single Color")
+    img = _cpu(_render(vpt, 8))
+    sky = np.array([2.6, 2.6 * 0.961538462, 2.6 * 0.884615385], np.float32)
+    np.testing.assert_allclose(img.reshape(3, -1).T, np.broadcast_to(sky, (RES * RES, 3)), rtol=2e-6)
+
+
+def test_render_deterministic(vpt, oracle):
+    print("case: determinism next-event 16^3", flush=True)
+    _set_volume(vpt, oracle.fbm_cloud(16, 16, 16, seed=4))
+    _common(vpt, 5)
+    a = _cpu(_render(vpt, 16))
+    vpt.set_seed_offset(0)  # any setting resets the accumulation
+    b = _cpu(_render(vpt, 16))
+    np.testing.assert_array_equal(a, b)
+
+
+def test_accumulation_is_running_mean(vpt, oracle):
+    print("case: accumulation 8 + 8 frames vs 16", flush=True)
+    _set_volume(vpt, oracle.fbm_cloud(16, 16, 16, seed=4))
+    _common(vpt, 0)
+    _render(vpt, 8)
+    two = _cpu(_render(vpt, 8))
+    vpt.set_seed_offset(0)
+    one = _cpu(_render(vpt, 16))
+    np.testing.assert_allclose(two, one, rtol=1e-5, atol=1e-6)
+
+
+def test_cpu_tensor_matches_device_tensor(vpt, oracle):
+    print("case: cpu vs hip tensor output", flush=True)
+    _set_volume(vpt, oracle.fbm_cloud(16, 16, 16, seed=4))
+    _common(vpt, 0)
+    a = _cpu(_render(vpt, 4, channels=4))
+    vpt.set_seed_offset(0)
+    b = _render(vpt, 4, channels=4, device="cpu").numpy()
+    np.testing.assert_array_equal(a, b)
+    assert np.all(a[3] == 1.0)
+
+
+MODES = [("delta", 0), ("ratio", 2), ("next_event", 5), ("decomposition", 3), ("residual_ratio", 4)]
+
+
+@pytest.mark.parametrize("name,mode", MODES)
+@pytest.mark.parametrize("interp", ["nearest", "stochastic"])
+def test_render_vs_oracle(vpt, oracle, name, mode, interp):
+    print(f"case: render vs oracle mode={name} interp={interp}", flush=True)
+    d = oracle.fbm_cloud(24, 20, 16, seed=5)
+    _set_volume(vpt, d)
+    _common(vpt, mode, interp=interp, ext=96.0, g=0.5)
+    m_gpu, s_gpu, _ = _render_stats(vpt, SPP)
+    sc = oracle.Scene(d, mode=name, interpolation=interp, extinction=96.0, g=0.5, width=RES, height=RES)
+    m_ref, s_ref = _oracle_stats(sc, SPP)
+    _check_equal(m_gpu, s_gpu, m_ref, s_ref, f"{name}/{interp}")
+
+
+@pytest.mark.parametrize("g", [0.0, 0.85])
+def test_next_event_env_image_and_sun_vs_oracle(vpt, oracle, g):
+    print(f"case: next-event env image + directional sun g={g}", flush=True)
+    torch = _torch()
+    d = oracle.fbm_cloud(24, 24, 24, seed=6)
+    _set_volume(vpt, d)
+    _common(vpt, 5, ext=128.0, g=g)
+    vpt.load_environment_map_from_tensor(torch.ones((3, 8, 16), dtype=torch.float32))
+    vpt.set_use_lights(True)
+    vpt.clear_lights()
+    vpt.add_light()
+    vpt.set_light_property(0, "type", "Directional Light")
+    vpt.set_light_property(0, "space", "World")
+    vpt.set_light_property(0, "position", "58.26 76.60 27.17")
+    vpt.set_light_property(0, "color", "1 0.961538462 0.884615385")
+    vpt.set_light_property(0, "intensity", "2.6")
+    m_gpu, s_gpu, _ = _render_stats(vpt, SPP)
+    L = oracle.make_light(light_type=2, light_space=0, position=(58.26, 76.60, 27.17), intensity=2.6,
+                          color=(1.0, 0.961538462, 0.884615385))
+    sc = oracle.Scene(d, mode="next_event", extinction=128.0, g=g, env_image=np.ones((3, 8, 16), np.float32),
+                      lights=[L], width=RES, height=RES)
+    m_ref, s_ref = _oracle_stats(sc, SPP)
+    _check_equal(m_gpu, s_gpu, m_ref, s_ref, f"nee env+sun g={g}")
+    vpt.set_use_lights(False)
+    vpt.set_use_builtin_environment_map("Default")
+
+
+def test_majorant_grid_matches_global_majorant(vpt, oracle):
+    print("case: majorant grid on/off next-event", flush=True)
+    d = oracle.fbm_cloud(32, 32, 32, seed=7)
+    _set_volume(vpt, d)
+    _common(vpt, 5, ext=256.0)
+    vpt.set_use_majorant_grid(True)
+    m1, s1, _ = _render_stats(vpt, SPP)
+    vpt.set_use_majorant_grid(False)
+    m0, s0, _ = _render_stats(vpt, SPP)
+    vpt.set_use_majorant_grid(True)
+    _check_equal(m1, s1, m0, s0, "majorant grid vs global majorant")
+
+
+def _rays(n, seed, half):
+    rng = np.random.default_rng(seed)
+    o = rng.normal(size=(n, 3))
+    o = o / np.linalg.norm(o, axis=1, keepdims=True) * (1.5 * math.sqrt(3) * half)
+    tgt = rng.uniform(-half, half, (n, 3))
+    w = tgt - o
+    w /= np.linalg.norm(w, axis=1, keepdims=True)
+    return o.astype(np.float32), w.astype(np.float32)
+
+
+@pytest.mark.parametrize("estimator", ["delta", "ratio", "residual_ratio", "decomposition"])
+@pytest.mark.parametrize("interp", ["nearest", "stochastic"])
+def test_transmittance_vs_fp64(vpt, oracle, estimator, interp):
+    print(f"case: transmittance {estimator} {interp}", flush=True)
+    d = oracle.fbm_cloud(32, 32, 32, seed=8)
+    _set_volume(vpt, d)
+    _common(vpt, 0, interp=interp, ext=48.0)
+    o, w = _rays(512, 9, 0.25)
+    n = 2048
+    mean, m2 = vpt.transmittance_batch(estimator, o, w, n_samples=n, seed=3)
+    sc = oracle.Scene(d, interpolation=interp, extinction=48.0)
+    exact = np.exp(-sc.optical_depth(o, w))
+    var = np.maximum(m2.astype(np.float64) - mean.astype(np.float64) ** 2, 1.0 / n)
+    z = (mean - exact) / np.sqrt(var / n)
+    assert np.max(np.abs(z)) < 5.5, (np.argmax(np.abs(z)), np.max(np.abs(z)))
+    assert abs(z.mean()) * math.sqrt(z.size) < 4.0
+
+
+def test_transmittance_device_tensors(vpt, oracle):
+    print("case: transmittance on device tensors", flush=True)
+    torch = _torch()
+    d = oracle.fbm_cloud(16, 16, 16, seed=8)
+    _set_volume(vpt, d)
+    _common(vpt, 0, ext=32.0)
+    o, w = _rays(256, 10, 0.25)
+    mh, _ = vpt.transmittance_batch("ratio", o, w, n_samples=64, seed=5)
+    md, _ = vpt.transmittance_batch("ratio", torch.from_numpy(o).cuda(), torch.from_numpy(w).cuda(), n_samples=64,
+                                    seed=5)
+    np.testing.assert_array_equal(mh, _cpu(md))
+
+
+# The reference's relational equal-mean tests (test/TestVolumetricPathTracing.cpp:156-281) on the GPU.
+def _block(n, boundary=False):
+    d = np.ones((n, n, n), np.float32)
+    if boundary and n >= 3:
+        d[:] = 0
+        d[1:-1, 1:-1, 1:-1] = 1
+    return d
+
+
+@pytest.mark.parametrize("case", ["delta_vs_ratio", "seed_offset", "delta_vs_decomposition_nearest",
+                                  "delta_vs_next_event"])
+def test_reference_relational(vpt, case):
+    print(f"case: relational {case}", flush=True)
+    if case == "delta_vs_ratio":
+        vol, a, b, interp = _block(1), (0, 0), (2, 0), "stochastic"
+    elif case == "seed_offset":
+        vol, a, b, interp = _block(1), (0, 0), (0, 268435456), "stochastic"
+    elif case == "delta_vs_decomposition_nearest":
+        vol, a, b, interp = _block(8, True), (0, 0), (3, 0), "nearest"
+    else:
+        vol, a, b, interp = _block(8, True), (0, 0), (5, 0), "stochastic"
+    _set_volume(vpt, vol)
+    res = []
+    for mode, off in (a, b):
+        _common(vpt, mode, interp=interp, ext=1024.0)
+        means = []
+        for k in range(4):
+            vpt.set_seed_offset(off + 7919 * k)
+            means.append(_cpu(_render(vpt, 64, res=64)).reshape(3, -1).mean(1))
+        means = np.array(means)
+        res.append((means.mean(0), means.std(0, ddof=1) / 2.0))
+    (ma, sa), (mb, sb) = res
+    tol = np.maximum(3.0 * np.sqrt(sa ** 2 + sb ** 2), 1e-3)
+    assert np.all(np.abs(ma - mb) <= tol), (ma, mb, tol)
+
+
+def test_invalid_arguments_raise(vpt):
+    torch = _torch()
+    with pytest.raises(RuntimeError):
+        vpt.render_frame(torch.zeros((2, 8, 8)), 1)
+    with pytest.raises(RuntimeError):
+        vpt.render_frame(torch.zeros((3, 8, 8), dtype=torch.float64), 1)
+    with pytest.raises(RuntimeError):
+        vpt.set_light_property(99, "intensity", "1")
+    with pytest.raises(RuntimeError):
+        vpt.set_vpt_mode_from_name("No Such Mode")
