@@ -259,8 +259,17 @@ def set_use_sparse_grid(use_sparse: bool) -> None:
     call("crvpt_set_use_sparse_grid", int(bool(use_sparse)))
 
 
-def set_grid_interpolation_type(interpolation: int) -> None:
-    """VolumetricPathTracingPass::setSparseGridInterpolationType: 0 nearest, 1 stochastic, 2 trilinear."""
+GRID_INTERPOLATION_TYPE_NAMES = ["Nearest", "Stochastic", "Trilinear"]  # RenderSettings.hpp:57-59
+
+
+def set_grid_interpolation_type(interpolation) -> None:
+    """VolumetricPathTracingPass::setSparseGridInterpolationType: 0 nearest, 1 stochastic, 2 trilinear (or the
+    name, case-insensitive)."""
+    if isinstance(interpolation, str):
+        names = [n.lower() for n in GRID_INTERPOLATION_TYPE_NAMES]
+        if interpolation.lower() not in names:
+            raise CrvptError(f"set_grid_interpolation_type: unknown type \"{interpolation}\"")
+        interpolation = names.index(interpolation.lower())
     call("crvpt_set_grid_interpolation_type", int(interpolation))
 
 
