@@ -9,7 +9,7 @@ import ctypes
 import os
 
 _PKG_DIR = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_PKG_DIR, "libcrvpt.so")
+LIB_PATH = os.environ.get("CRVPT_LIBRARY", os.path.join(_PKG_DIR, "libcrvpt.so"))
 
 _c_int = ctypes.c_int
 _c_i64 = ctypes.c_int64

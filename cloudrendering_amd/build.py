@@ -15,13 +15,14 @@ from concurrent.futures import ThreadPoolExecutor
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 REPO_DIR = os.path.dirname(PKG_DIR)
 CSRC = os.path.join(PKG_DIR, "csrc")
-BUILD_DIR = os.path.join(PKG_DIR, "_build")
-LIB_PATH = os.path.join(PKG_DIR, "libcrvpt.so")
+BUILD_DIR = os.environ.get("CRVPT_BUILD_DIR", os.path.join(PKG_DIR, "_build"))
+LIB_PATH = os.environ.get("CRVPT_LIBRARY", os.path.join(PKG_DIR, "libcrvpt.so"))
 
 HIP_SOURCES = ["render.hip", "denoise.hip"]
 CXX_SOURCES = ["api.cpp", "nanovdb_build.cpp"]
 
 ARCH = os.environ.get("CRVPT_OFFLOAD_ARCH", "gfx950")
+EXTRA_HIP_FLAGS = os.environ.get("CRVPT_EXTRA_HIP_FLAGS", "").split()
 
 
 def hipcc() -> str:
@@ -36,7 +37,7 @@ def _compile(src: str) -> str:
     common = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-I", CSRC, "-I", os.path.join(REPO_DIR, "include")]
     if src.endswith(".hip"):
         cmd = [hipcc(), "-x", "hip", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
-               "-packed-fp32-ops", "-munsafe-fp-atomics", *common, "-c", os.path.join(CSRC, src), "-o", obj]
+               "-packed-fp32-ops", "-munsafe-fp-atomics", *EXTRA_HIP_FLAGS, *common, "-c", os.path.join(CSRC, src), "-o", obj]
     else:
         cmd = [hipcc(), *common, "-c", os.path.join(CSRC, src), "-o", obj]
     res = subprocess.run(cmd, capture_output=True, text=True)

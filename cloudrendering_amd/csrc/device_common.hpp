@@ -147,7 +147,7 @@ CRVPT_DEV V3 uniformSphereDirection(Rng& rng, V3 d) {
 
 CRVPT_DEV float hgPdf(float g, float cosTheta) {
     float denom = (1.0f + g * g) - 2.0f * g * cosTheta;
-    return 0.25f / kPi * (1.0f - g * g) / (denom * sqrtf(denom));
+    return (0.25f / kPi) * (1.0f - g * g) * __builtin_amdgcn_rcpf(denom * sqrtf(denom));
 }
 
 // importanceSamplePhase: HG inversion (isotropic below |g| < 0.001).
@@ -281,30 +281,38 @@ struct DenseAccess {
         return g.values[(size_t(k) * g.dim[1] + j) * g.dim[0] + i];
     }
     CRVPT_DEV float voxelAt(int i, int j, int k) const { return voxel(i, j, k); }
-    CRVPT_DEV float lookupWorld(const SceneParams& P, Rng& rng, V3 pos) const {
-        return lookup(P, rng, v3((pos.x - P.boxMin[0]) * P.worldToTex[0], (pos.y - P.boxMin[1]) * P.worldToTex[1],
-                                 (pos.z - P.boxMin[2]) * P.worldToTex[2]));
-    }
-    // u: texel coordinates (grid dim * normalised texture coordinate).
-    CRVPT_DEV float lookup(const SceneParams& P, Rng& rng, V3 u) const {
+    // u: texel coordinates (grid dim * normalised texture coordinate); (jx, jy, jz) in (-0.5, 0.5) is the
+    // stochastic sub-voxel jitter, ignored by the other filters.
+    CRVPT_DEV float lookupTex(const SceneParams& P, V3 u, float jx, float jy, float jz) const {
         if (P.interpolation == INTERP_TRILINEAR) {
             u = u - v3(0.5f);
-            float fx = floorf(u.x), fy = floorf(u.y), fz = floorf(u.z);
-            int i = int(fx), j = int(fy), k = int(fz);
-            float a = u.x - fx, b = u.y - fy, c = u.z - fz;
-            float c00 = lerpf(voxel(i, j, k), voxel(i + 1, j, k), a);
-            float c10 = lerpf(voxel(i, j + 1, k), voxel(i + 1, j + 1, k), a);
-            float c01 = lerpf(voxel(i, j, k + 1), voxel(i + 1, j, k + 1), a);
-            float c11 = lerpf(voxel(i, j + 1, k + 1), voxel(i + 1, j + 1, k + 1), a);
+            const float fx = floorf(u.x), fy = floorf(u.y), fz = floorf(u.z);
+            const int i = int(fx), j = int(fy), k = int(fz);
+            const float a = u.x - fx, b = u.y - fy, c = u.z - fz;
+            const float c00 = lerpf(voxel(i, j, k), voxel(i + 1, j, k), a);
+            const float c10 = lerpf(voxel(i, j + 1, k), voxel(i + 1, j + 1, k), a);
+            const float c01 = lerpf(voxel(i, j, k + 1), voxel(i + 1, j, k + 1), a);
+            const float c11 = lerpf(voxel(i, j + 1, k + 1), voxel(i + 1, j + 1, k + 1), a);
             return lerpf(lerpf(c00, c10, b), lerpf(c01, c11, b), c);
         }
-        if (P.interpolation == INTERP_STOCHASTIC) {
-            float j0 = rng.uniform() - 0.5f;
-            float j1 = rng.uniform() - 0.5f;
-            float j2 = rng.uniform() - 0.5f;
-            u = u + v3(j0, j1, j2);
-        }
+        if (P.interpolation == INTERP_STOCHASTIC) u = u + v3(jx, jy, jz);
         return voxel(int(floorf(u.x)), int(floorf(u.y)), int(floorf(u.z)));
+    }
+    CRVPT_DEV V3 toTex(const SceneParams& P, V3 pos) const {
+        return v3((pos.x - P.boxMin[0]) * P.worldToTex[0], (pos.y - P.boxMin[1]) * P.worldToTex[1],
+                  (pos.z - P.boxMin[2]) * P.worldToTex[2]);
+    }
+    CRVPT_DEV float lookupWorldJ(const SceneParams& P, V3 pos, float jx, float jy, float jz) const {
+        return lookupTex(P, toTex(P, pos), jx, jy, jz);
+    }
+    CRVPT_DEV float lookupWorld(const SceneParams& P, Rng& rng, V3 pos) const {
+        float jx = 0.0f, jy = 0.0f, jz = 0.0f;
+        if (P.interpolation == INTERP_STOCHASTIC) {
+            jx = rng.uniform() - 0.5f;
+            jy = rng.uniform() - 0.5f;
+            jz = rng.uniform() - 0.5f;
+        }
+        return lookupTex(P, toTex(P, pos), jx, jy, jz);
     }
 };
 

@@ -64,8 +64,9 @@ struct NanoAccess {
                   sx * g.invMat[6] + sy * g.invMat[7] + sz * g.invMat[8]);
     }
 
-    // sampleCloud for NanoVDB (VptUtils.glsl:515-580, 674-684): raw values, no normalisation.
-    CRVPT_DEV float lookupWorld(const SceneParams& P, Rng& rng, V3 pos) const {
+    // sampleCloud for NanoVDB (VptUtils.glsl:515-580, 674-684): raw values, no normalisation; (jx, jy, jz) in
+    // (-0.5, 0.5) is the stochastic sub-voxel jitter, ignored by the other filters.
+    CRVPT_DEV float lookupWorldJ(const SceneParams& P, V3 pos, float jx, float jy, float jz) const {
         V3 coord = (pos - v3(P.boxMin)) / (v3(P.boxMax) - v3(P.boxMin));
         coord = coord * (v3(P.gridMax) - v3(P.gridMin)) + v3(P.gridMin);
         V3 idx = worldToIndex(coord);
@@ -80,13 +81,17 @@ struct NanoAccess {
             const float c11 = lerpf(value(i, j + 1, k + 1), value(i + 1, j + 1, k + 1), a);
             return lerpf(lerpf(c00, c10, b), lerpf(c01, c11, b), c);
         }
-        if (P.interpolation == INTERP_STOCHASTIC) {
-            const float j0 = rng.uniform() - 0.5f;
-            const float j1 = rng.uniform() - 0.5f;
-            const float j2 = rng.uniform() - 0.5f;
-            idx = idx + v3(j0, j1, j2);
-        }
+        if (P.interpolation == INTERP_STOCHASTIC) idx = idx + v3(jx, jy, jz);
         return value(int(floorf(idx.x)), int(floorf(idx.y)), int(floorf(idx.z)));
+    }
+    CRVPT_DEV float lookupWorld(const SceneParams& P, Rng& rng, V3 pos) const {
+        float jx = 0.0f, jy = 0.0f, jz = 0.0f;
+        if (P.interpolation == INTERP_STOCHASTIC) {
+            jx = rng.uniform() - 0.5f;
+            jy = rng.uniform() - 0.5f;
+            jz = rng.uniform() - 0.5f;
+        }
+        return lookupWorldJ(P, pos, jx, jy, jz);
     }
 };
 

@@ -25,11 +25,15 @@ namespace crvpt {
 
 constexpr float kInf = 3.0e38f;
 
+#ifndef CRVPT_PATHPOOL_WAVES
+#define CRVPT_PATHPOOL_WAVES 3  // waves per SIMD the path pool is compiled for (caps VGPRs at 168)
+#endif
+
 // ------------------------------------------------------------------------------------------------------------
 // Supervoxel walker
 // ------------------------------------------------------------------------------------------------------------
 struct Walker {
-    uint32_t svReads;   // supervoxel majorant reads since the path started (work counter)
+    uint32_t svReads;   // supervoxel majorant reads since the launch started (work counter)
     V3 o, w;            // ray origin / direction (world)
     float t, tEnd;      // current distance along the ray, end of the segment
     int ix, iy, iz;     // current supervoxel
@@ -39,14 +43,25 @@ struct Walker {
     float tau;          // remaining optical length of the current Exp(1) sample; < 0 when none is drawn
 };
 
+CRVPT_DEV float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
+
 CRVPT_DEV float svMajorant(const SceneParams& P, const SupervoxelGrid& S, int ix, int iy, int iz) {
     return P.extinction * S.majorant[(size_t(iz) * S.grid[1] + iy) * S.grid[0] + ix];
 }
 
 // Starts a segment from x along w, clipped to the box and to [0, limit]. False when the segment is empty.
+// The slab test follows rayBoxIntersect (near-zero direction components replaced by 1e-6, tMin >= 0).
 CRVPT_DEV bool walkerStart(Walker& W, const SceneParams& P, const SupervoxelGrid& S, V3 x, V3 w, float limit) {
-    float tMin, tMax;
-    if (!rayBoxIntersect(v3(P.boxMin), v3(P.boxMax), x, w, tMin, tMax)) return false;
+    const float ax = fabsf(w.x) <= 0.000001f ? 0.000001f : w.x;
+    const float ay = fabsf(w.y) <= 0.000001f ? 0.000001f : w.y;
+    const float az = fabsf(w.z) <= 0.000001f ? 0.000001f : w.z;
+    const V3 inv = v3(rcpf(ax), rcpf(ay), rcpf(az));
+    const V3 c0 = (v3(P.boxMin) - x) * inv;
+    const V3 c1 = (v3(P.boxMax) - x) * inv;
+    float tMin = fmaxf(fmaxf(fminf(c0.x, c1.x), fminf(c0.y, c1.y)), fminf(c0.z, c1.z));
+    tMin = fmaxf(0.0f, tMin);
+    float tMax = fminf(fminf(fmaxf(c0.x, c1.x), fmaxf(c0.y, c1.y)), fmaxf(c0.z, c1.z));
+    if (tMax <= tMin || tMax <= 0.0f) return false;
     tMax = fminf(tMax, limit);
     if (!(tMax > tMin)) return false;
     W.o = x;
@@ -59,25 +74,24 @@ CRVPT_DEV bool walkerStart(Walker& W, const SceneParams& P, const SupervoxelGrid
         W.nx = W.ny = W.nz = kInf;
         return true;
     }
-    // Cell of the point slightly inside the segment (robust against an entry point on a cell plane).
+    // Cell of a point slightly inside the segment (robust against an entry point on a cell plane).
     const float tc = tMin + 1e-4f * (tMax - tMin);
     const V3 p = x + w * tc;
     int idx[3];
     float nxt[3], dlt[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        const float wa = comp(w, a), pa = comp(p, a);
-        // supervoxel edge in world units: S.size voxels of the density grid
-        const float cw = float(S.size) / P.worldToTex[a];
-        int i = int(floorf((pa - P.boxMin[a]) / cw));
+        const float wa = comp(w, a), pa = comp(p, a), ia = comp(inv, a);
+        const float cw = float(S.size) * rcpf(P.worldToTex[a]);  // supervoxel edge in world units
+        int i = int(floorf((pa - P.boxMin[a]) * (P.worldToTex[a] * (1.0f / float(S.size)))));
         i = min(max(i, 0), S.grid[a] - 1);
         idx[a] = i;
         if (wa > 0.0f) {
-            nxt[a] = tc + (P.boxMin[a] + float(i + 1) * cw - pa) / wa;
-            dlt[a] = cw / wa;
+            nxt[a] = tc + (P.boxMin[a] + float(i + 1) * cw - pa) * ia;
+            dlt[a] = cw * ia;
         } else if (wa < 0.0f) {
-            nxt[a] = tc + (P.boxMin[a] + float(i) * cw - pa) / wa;
-            dlt[a] = -cw / wa;
+            nxt[a] = tc + (P.boxMin[a] + float(i) * cw - pa) * ia;
+            dlt[a] = -cw * ia;
         } else {
             nxt[a] = kInf;
             dlt[a] = kInf;
@@ -94,12 +108,13 @@ CRVPT_DEV bool walkerStart(Walker& W, const SceneParams& P, const SupervoxelGrid
 enum : int { WALK_MOVED = 0, WALK_COLLIDE = 1, WALK_END = 2 };
 
 // One walker event: a tentative collision (W.t at the collision), a supervoxel crossing, or the segment end.
-CRVPT_DEV int walkStep(Walker& W, Rng& rng, const SceneParams& P, const SupervoxelGrid& S) {
-    if (W.tau < 0.0f) W.tau = rng.exponential();
+// e is an Exp(1) sample, consumed only when no optical budget is pending.
+CRVPT_DEV int walkAdvance(Walker& W, const SceneParams& P, const SupervoxelGrid& S, float e) {
+    if (W.tau < 0.0f) W.tau = e;
     const float tExit = fminf(fminf(W.nx, W.ny), fminf(W.nz, W.tEnd));
     const float opt = W.mu * (tExit - W.t);
     if (W.tau < opt) {
-        W.t += W.tau / W.mu;
+        W.t += W.tau * rcpf(W.mu);
         W.tau = -1.0f;
         return WALK_COLLIDE;
     }
@@ -124,6 +139,11 @@ CRVPT_DEV int walkStep(Walker& W, Rng& rng, const SceneParams& P, const Supervox
     return WALK_MOVED;
 }
 
+// Walker event with the exponential drawn from a buffered stream (transmittance queries).
+CRVPT_DEV int walkStep(Walker& W, Rng& rng, const SceneParams& P, const SupervoxelGrid& S) {
+    return walkAdvance(W, P, S, W.tau < 0.0f ? rng.exponential() : 0.0f);
+}
+
 template <class Grid>
 CRVPT_DEV float density(const Grid& g, const SceneParams& P, Rng& rng, V3 p) {
     return g.lookupWorld(P, rng, p);
@@ -132,136 +152,191 @@ CRVPT_DEV float density(const Grid& g, const SceneParams& P, Rng& rng, V3 p) {
 // ------------------------------------------------------------------------------------------------------------
 // Path pool for the delta (DeltaTracking.glsl:167-345), analog ratio (RatioTracking.glsl:25-96) and next-event
 // (NextEventTracking.glsl:223-394 with NeeTransmittance.glsl ratio-tracking shadow rays) estimators.
+//
+// Random numbers: every walker step of every lane evaluates one Philox block (counter: step index, pixel,
+// global frame, seed offset) in lockstep with the rest of the wave: one word for the free-path exponential, 16-bit
+// fields for the sub-voxel jitter, one word for the collision decision. Scatter events draw from a second counter
+// space, the camera jitter from a third, so a path's draws never depend on the lane schedule.
 // ------------------------------------------------------------------------------------------------------------
-struct PathSlot {
-    uint32_t taps;    // density lookups of this lane since the launch started (work counter)
-    uint32_t unit;
-    bool alive;
-    bool exhausted;
-    bool hasValue;
-    bool shadow;
-    float thr;        // analog ratio weight, or shadow-ray transmittance while shadow == true
-    V3 color;         // next-event accumulated radiance
-    V3 shadowScale;   // radiance factor of the pending shadow ray
-    V3 mainX, mainW;  // where the main path resumes after the shadow ray
+constexpr uint32_t kEventSpace = 0x80000000u;
+constexpr uint32_t kCameraBlock = 0xFFFFFFFFu;
+
+enum : uint32_t { PF_ALIVE = 1u, PF_EXHAUSTED = 2u, PF_HAS_VALUE = 4u, PF_SHADOW = 8u };
+
+struct PathState {
+    uint32_t unit;     // launch-local path index
+    uint32_t pix;      // pixel index
+    uint32_t frame;    // global frame (sample) number
+    uint32_t step;     // next per-step Philox block
+    uint32_t events;   // next event Philox block
+    uint32_t flags;
+    uint32_t taps;     // density lookups of this lane since the launch started (work counter)
+    float thr;         // analog ratio weight, or the shadow-ray transmittance while PF_SHADOW is set
+    V3 color;          // next-event accumulated radiance
+    V3 shadowScale;    // radiance factor of the pending shadow ray
+    V3 mainX, mainW;   // where the main path resumes after the shadow ray
 };
+
+CRVPT_DEV uint4 pathBlock(const SceneParams& P, const PathState& st, uint32_t c0) {
+    return philox4x32_10(make_uint4(c0, st.pix, st.frame, P.seedOffset), make_uint2(0x6A09E667u, 0x3C6EF372u));
+}
+CRVPT_DEV float u01(uint32_t x) { return float(x >> 8) * (1.0f / 16777216.0f); }
+CRVPT_DEV float uOpen(uint32_t x) { return float((x >> 8) + 1u) * (1.0f / 16777216.0f); }
+CRVPT_DEV float jitterHi(uint32_t x) { return (float(x >> 16) + 0.5f) * (1.0f / 65536.0f) - 0.5f; }
+CRVPT_DEV float jitterLo(uint32_t x) { return (float(x & 0xFFFFu) + 0.5f) * (1.0f / 65536.0f) - 0.5f; }
 
 CRVPT_DEV void writePath(float4* out, uint32_t unit, V3 c, bool hasValue) {
     out[unit] = make_float4(c.x, c.y, c.z, hasValue ? 1.0f : 0.0f);
 }
 
-// Camera ray of a unit: seeds the path's Philox stream with (pixel, global frame, seed offset) and jitters the
-// pixel position with two draws (Clouds.glsl:80-90).
-CRVPT_DEV void cameraRayOfUnit(const SceneParams& P, uint32_t unit, Rng& rng, V3& x, V3& w) {
-    const uint32_t npix = uint32_t(P.width) * uint32_t(P.height);
-    const uint32_t s = unit / npix;
-    const uint32_t p = unit - s * npix;
-    const uint32_t px = p % uint32_t(P.width), py = p / uint32_t(P.width);
-    rng.init(p, P.frameBase + s, P.seedOffset);
-    const float jx = rng.uniform(), jy = rng.uniform();
-    const float sx = 2.0f * (float(px) + jx) / float(P.width) - 1.0f;
-    const float sy = 2.0f * (float(py) + jy) / float(P.height) - 1.0f;
+// Uniform direction on the sphere from two uniforms, in the basis of d (randomDirection).
+CRVPT_DEV V3 sphereDirection(float r1, float r2u, V3 d) {
+    const float r2 = r2u * 2.0f - 1.0f;
+    const float s = sqrtf(fmaxf(0.0f, 1.0f - r2 * r2));
+    const float phi = kTwoPi * r1;
+    V3 t0, t1;
+    orthonormalBasis(d, t0, t1);
+    return t0 * (__cosf(phi) * s) + t1 * (__sinf(phi) * s) + d * r2;
+}
+
+// importanceSamplePhase from two uniforms (HG inversion; isotropic below |g| < 0.001).
+CRVPT_DEV V3 phaseDirection(float g, V3 d, float u0, float u1) {
+    if (fabsf(g) < 0.001f) return sphereDirection(u0, u1, -d);
+    const float phi = u0 * kTwoPi;
+    const float t = (1.0f - g * g) * rcpf(1.0f - g + 2.0f * g * u1);
+    const float cosTheta = (0.5f / g) * ((1.0f + g * g) - t * t);
+    const float sinTheta = sqrtf(fmaxf(0.0f, 1.0f - cosTheta * cosTheta));
+    V3 t0, t1;
+    orthonormalBasis(d, t0, t1);
+    return t0 * (sinTheta * __sinf(phi)) + t1 * (sinTheta * __cosf(phi)) + d * cosTheta;
+}
+
+// Camera ray of the path: pixel jitter from the camera block (Clouds.glsl:80-90).
+CRVPT_DEV void cameraRay(const SceneParams& P, const PathState& st, V3& x, V3& w) {
+    const uint4 b = pathBlock(P, st, kCameraBlock);
+    const uint32_t px = st.pix % uint32_t(P.width), py = st.pix / uint32_t(P.width);
+    const float sx = 2.0f * (float(px) + u01(b.x)) / float(P.width) - 1.0f;
+    const float sy = 2.0f * (float(py) + u01(b.y)) / float(P.height) - 1.0f;
     createCameraRay(P, sx, sy, x, w);
 }
 
-template <int MODE, class Grid>
-CRVPT_DEV bool startPath(PathSlot& st, Walker& W, Rng& rng, const SceneParams& P, const SupervoxelGrid& S,
-                         float4* out) {
-    V3 x, w;
-    cameraRayOfUnit(P, st.unit, rng, x, w);
-    st.hasValue = false;
-    st.shadow = false;
+template <int MODE>
+CRVPT_DEV bool startPath(PathState& st, Walker& W, const SceneParams& P, const SupervoxelGrid& S, float4* out) {
+    const uint32_t npix = uint32_t(P.width) * uint32_t(P.height);
+    const uint32_t s = st.unit / npix;
+    st.pix = st.unit - s * npix;
+    st.frame = P.frameBase + s;
+    st.step = 0;
+    st.events = 0;
+    st.flags = PF_ALIVE;
     st.thr = 1.0f;
     st.color = v3(0.0f);
+    V3 x, w;
+    cameraRay(P, st, x, w);
     if (!walkerStart(W, P, S, x, w, kInf)) {
         writePath(out, st.unit, skyRadiance(P, w), false);
+        st.flags = 0;
         return false;
     }
     return true;
 }
 
-// One event of a live path. Returns false when the path finished (its radiance is written).
-template <int MODE, class Grid>
-CRVPT_DEV bool stepPath(PathSlot& st, Walker& W, Rng& rng, const SceneParams& P, const SupervoxelGrid& S,
-                        const Grid& grid, float4* out) {
-    const int ev = walkStep(W, rng, P, S);
-    if (MODE == VPT_NEXT_EVENT_TRACKING && st.shadow) {
-        if (ev == WALK_END) {
-            st.color += st.shadowScale * st.thr;
-            st.shadow = false;
-            if (!walkerStart(W, P, S, st.mainX, st.mainW, kInf)) {
-                writePath(out, st.unit, st.color, true);
-                return false;
-            }
-        } else if (ev == WALK_COLLIDE) {
-            const V3 p = W.o + W.w * W.t;
-            const float rho = density(grid, P, rng, p);
-            st.taps++;
-            st.thr *= 1.0f - P.extinction * rho / W.mu;
-        }
-        return true;
-    }
-    if (ev == WALK_END) {
-        V3 c;
-        if (MODE == VPT_NEXT_EVENT_TRACKING) {
-            c = st.hasValue ? st.color : st.color + skyRadiance(P, W.w);
-        } else {
-            c = skyRadiance(P, W.w) * st.thr;
-        }
-        writePath(out, st.unit, c, st.hasValue);
-        return false;
-    }
-    if (ev != WALK_COLLIDE) return true;
-
-    const V3 p = W.o + W.w * W.t;
-    const float rho = density(grid, P, rng, p);
-    st.taps++;
-    const float xi = rng.uniform();
-    const float pReal = P.extinction * rho / W.mu;
-    const float pAbs = (1.0f - P.albedo) * pReal;
-    if (MODE != VPT_RATIO_TRACKING && xi < pAbs) {  // absorption: delta returns 0, NEE returns its sum
-        writePath(out, st.unit, MODE == VPT_NEXT_EVENT_TRACKING ? st.color : v3(0.0f), true);
-        return false;
-    }
-    if (!(xi < pReal)) return true;  // null collision
-
-    // Real scattering collision.
-    const V3 nw = samplePhase(rng, P.phaseG, W.w);
-    if (MODE == VPT_RATIO_TRACKING) st.thr *= 1.0f - pAbs / pReal;
-    st.hasValue = true;
+// Real scattering collision at p of the main path (rare compared with tentative collisions).
+template <int MODE>
+CRVPT_DEV bool scatterEvent(PathState& st, Walker& W, const SceneParams& P, const SupervoxelGrid& S, V3 p,
+                            float pAbsOverReal, float4* out) {
+    const uint4 e = pathBlock(P, st, kEventSpace | st.events++);
+    const V3 nw = phaseDirection(P.phaseG, W.w, u01(e.x), u01(e.y));
+    if (MODE == VPT_RATIO_TRACKING) st.thr *= 1.0f - pAbsOverReal;
+    st.flags |= PF_HAS_VALUE;
     if (MODE == VPT_NEXT_EVENT_TRACKING) {
-        const float pdfEnv = 1.0f / (4.0f * kPi);
         bool toLight = false;
         float probFactor = 1.0f, dist = 0.0f;
         int li = 0;
         V3 dir;
         if (P.numLights > 0) {
-            toLight = P.isEnvMapBlack ? true : (rng.uniform() > 0.5f);
+            toLight = P.isEnvMapBlack ? true : (u01(e.z) > 0.5f);
             probFactor = P.isEnvMapBlack ? 1.0f : 2.0f;
-            if (toLight) dir = lightDirection(P, rng, p, li, probFactor, dist);
-            else dir = uniformSphereDirection(rng, v3(1.0f, 0.0f, 0.0f));
+        }
+        if (toLight) {
+            if (P.numLights > 1) {
+                li = min(int(u01(e.w) * float(P.numLights)), P.numLights - 1);
+                probFactor *= float(P.numLights);
+            }
+            const Light& L = P.lights[li];
+            const V3 lp = lightPositionWorld(P, L);
+            const V3 d = (L.lightType == 0u || L.lightType == 1u) ? lp - p : lp;
+            dist = length(d);
+            dir = d * rcpf(dist + 1e-5f);
         } else {
-            dir = uniformSphereDirection(rng, v3(1.0f, 0.0f, 0.0f));
+            const uint4 e2 = pathBlock(P, st, kEventSpace | st.events++);
+            dir = sphereDirection(u01(e2.x), u01(e2.y), v3(1.0f, 0.0f, 0.0f));
         }
         const float ph = evalPhase(P.phaseG, W.w, dir);
         const V3 L = toLight ? lightRadiance(P, p, li) * (probFactor * ph)
-                             : skyRadiance(P, dir) * (probFactor * ph / pdfEnv);
+                             : skyRadiance(P, dir) * (probFactor * ph * (4.0f * kPi));
         st.mainX = p;
         st.mainW = nw;
         if ((L.x > 0.0f || L.y > 0.0f || L.z > 0.0f) && walkerStart(W, P, S, p, dir, toLight ? dist : kInf)) {
-            st.shadow = true;
+            st.flags |= PF_SHADOW;
             st.shadowScale = L;
             st.thr = 1.0f;
             return true;
         }
-        st.color += L;  // unoccluded (segment outside the box) or zero contribution
+        st.color += L;  // zero contribution, or the shadow segment lies outside the box (T = 1)
     }
     if (!walkerStart(W, P, S, p, nw, kInf)) {
-        V3 c = MODE == VPT_NEXT_EVENT_TRACKING ? st.color : skyRadiance(P, nw) * st.thr;
+        const V3 c = MODE == VPT_NEXT_EVENT_TRACKING ? st.color : skyRadiance(P, nw) * st.thr;
         writePath(out, st.unit, c, true);
         return false;
     }
     return true;
+}
+
+// End of the current segment: a finished shadow ray resumes its main path, a main path leaves the volume.
+template <int MODE>
+CRVPT_DEV bool endEvent(PathState& st, Walker& W, const SceneParams& P, const SupervoxelGrid& S, float4* out) {
+    if (MODE == VPT_NEXT_EVENT_TRACKING && (st.flags & PF_SHADOW)) {
+        st.color += st.shadowScale * st.thr;
+        st.flags &= ~PF_SHADOW;
+        if (walkerStart(W, P, S, st.mainX, st.mainW, kInf)) return true;
+        writePath(out, st.unit, st.color, true);
+        return false;
+    }
+    const bool hv = (st.flags & PF_HAS_VALUE) != 0;
+    V3 c;
+    if (MODE == VPT_NEXT_EVENT_TRACKING) c = hv ? st.color : st.color + skyRadiance(P, W.w);
+    else c = skyRadiance(P, W.w) * st.thr;
+    writePath(out, st.unit, c, hv);
+    return false;
+}
+
+// One walker step of a live path. Returns false when the path finished (its radiance is written).
+template <int MODE, class Grid>
+CRVPT_DEV bool stepPath(PathState& st, Walker& W, const SceneParams& P, const SupervoxelGrid& S, const Grid& grid,
+                        float4* out) {
+    const uint4 r = pathBlock(P, st, st.step++);
+    const int ev = walkAdvance(W, P, S, -__logf(uOpen(r.x)));
+    if (ev == WALK_MOVED) return true;
+    if (ev == WALK_END) return endEvent<MODE>(st, W, P, S, out);
+    const V3 p = W.o + W.w * W.t;
+    const float rho = grid.lookupWorldJ(P, p, jitterHi(r.y), jitterLo(r.y), jitterHi(r.z));
+    st.taps++;
+    const float pReal = P.extinction * rho * rcpf(W.mu);
+    if (MODE == VPT_NEXT_EVENT_TRACKING && (st.flags & PF_SHADOW)) {
+        st.thr *= 1.0f - pReal;  // ratio tracking (NeeTransmittance.glsl:107-113)
+        // Once the product is exactly 0 every later factor leaves it 0 and the shadow ray adds exactly 0.
+        if (st.thr == 0.0f) return endEvent<MODE>(st, W, P, S, out);
+        return true;
+    }
+    const float xi = u01(r.w);
+    if (!(xi < pReal)) return true;  // null collision
+    const float pAbs = (1.0f - P.albedo) * pReal;
+    if (MODE != VPT_RATIO_TRACKING && xi < pAbs) {  // absorption: delta returns 0, next-event its sum
+        writePath(out, st.unit, MODE == VPT_NEXT_EVENT_TRACKING ? st.color : v3(0.0f), true);
+        return false;
+    }
+    return scatterEvent<MODE>(st, W, P, S, p, pAbs / pReal, out);
 }
 
 // Wave-reduced 64-bit work counters: [0] density taps, [1] supervoxel majorant reads, [2] paths started.
@@ -281,21 +356,20 @@ CRVPT_DEV void flushCounters(unsigned long long* stats, uint32_t taps, uint32_t 
 }
 
 template <int MODE, class Grid>
-__global__ __launch_bounds__(256) void k_pathPool(SceneParams P, Grid grid, SupervoxelGrid S,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CRVPT_PATHPOOL_WAVES))) void k_pathPool(SceneParams P, Grid grid, SupervoxelGrid S,
                                                   float4* __restrict__ out, uint32_t* __restrict__ counter,
                                                   int stepsPerRefill, unsigned long long* __restrict__ stats) {
     const int lane = __lane_id();
-    PathSlot st;
-    st.alive = false;
-    st.exhausted = false;
+    PathState st;
+    st.flags = 0;
     st.unit = 0;
     st.taps = 0;
+    bool exhausted = false;
     uint32_t paths = 0;
-    Rng rng;
     Walker W;
     W.svReads = 0;
     while (true) {
-        const bool needs = !st.alive && !st.exhausted;
+        const bool needs = !(st.flags & PF_ALIVE) && !exhausted;
         const uint64_t needMask = __ballot(needs);
         if (needMask) {
             const int leader = __ffsll((unsigned long long)needMask) - 1;
@@ -305,19 +379,22 @@ __global__ __launch_bounds__(256) void k_pathPool(SceneParams P, Grid grid, Supe
             if (needs) {
                 st.unit = base + uint32_t(__popcll(needMask & ((1ull << lane) - 1ull)));
                 if (uint64_t(st.unit) >= P.numUnits) {
-                    st.exhausted = true;
+                    exhausted = true;
                 } else {
                     paths++;
-                    st.alive = startPath<MODE, Grid>(st, W, rng, P, S, out);
+                    startPath<MODE>(st, W, P, S, out);
                 }
             }
         }
-        if (__ballot(st.alive) == 0) {
-            if (__ballot(!st.exhausted) == 0) break;
+        const bool alive = (st.flags & PF_ALIVE) != 0;
+        if (__ballot(alive) == 0) {
+            if (__ballot(!exhausted) == 0) break;
             continue;
         }
         for (int k = 0; k < stepsPerRefill; k++) {
-            if (st.alive) st.alive = stepPath<MODE, Grid>(st, W, rng, P, S, grid, out);
+            if (st.flags & PF_ALIVE) {
+                if (!stepPath<MODE, Grid>(st, W, P, S, grid, out)) st.flags = 0;
+            }
         }
     }
     if (stats) flushCounters(stats, st.taps, W.svReads, paths);
@@ -528,9 +605,15 @@ __global__ __launch_bounds__(256) void k_wholePath(SceneParams P, Grid grid, Sup
         const uint32_t unit = base + uint32_t(__popcll(all & ((1ull << lane) - 1ull)));
         if (__ballot(uint64_t(unit) < P.numUnits) == 0) break;
         if (uint64_t(unit) < P.numUnits) {
-            Rng rng;
+            PathState st;
+            st.unit = unit;
+            const uint32_t npix = uint32_t(P.width) * uint32_t(P.height);
+            st.pix = unit % npix;
+            st.frame = P.frameBase + unit / npix;
             V3 x, w;
-            cameraRayOfUnit(P, unit, rng, x, w);
+            cameraRay(P, st, x, w);
+            Rng rng;
+            rng.init(st.pix, st.frame, P.seedOffset ^ 0x243F6A88u);
             V3 c = MODE == VPT_DECOMPOSITION_TRACKING ? decompositionPath(P, S, grid, rng, x, w)
                                                        : residualRatioPath(P, S, grid, rng, x, w);
             out[unit] = make_float4(c.x, c.y, c.z, 1.0f);
@@ -810,9 +893,9 @@ __global__ void k_fbm(float* __restrict__ out, int nx, int ny, int nz, FbmParams
         const int x = int(idx % nx);
         const int y = int((idx / nx) % ny);
         const int z = int(idx / (size_t(nx) * ny));
-        const float px = (float(x) + 0.5f) / float(nx);
-        const float py = (float(y) + 0.5f) / float(ny);
-        const float pz = (float(z) + 0.5f) / float(nz);
+        const float px = __fdiv_rn(float(x) + 0.5f, float(nx));
+        const float py = __fdiv_rn(float(y) + 0.5f, float(ny));
+        const float pz = __fdiv_rn(float(z) + 0.5f, float(nz));
         float sum = 0.0f, norm = 0.0f, amp = 1.0f, freq = F.baseFrequency;
         for (int o = 0; o < F.octaves; o++) {
             const uint32_t salt = F.seed * 2654435761u + uint32_t(o) * 0x9E3779B9u;
@@ -821,16 +904,16 @@ __global__ void k_fbm(float* __restrict__ out, int nx, int ny, int nz, FbmParams
             amp = amp * F.gain;
             freq = freq * F.lacunarity;
         }
-        float v = sum / norm;
+        float v = __fdiv_rn(sum, norm);
         if (F.heightFalloff) {
             // anvil profile: flat top, rounded base, in the normalised height py
             const float h = py;
-            const float base = fminf(fmaxf((h - 0.05f) / 0.25f, 0.0f), 1.0f);
-            const float top = fminf(fmaxf((0.95f - h) / 0.1f, 0.0f), 1.0f);
-            const float spread = 1.0f + 0.5f * fminf(fmaxf((h - 0.6f) / 0.3f, 0.0f), 1.0f);
+            const float base = fminf(fmaxf(__fdiv_rn(h - 0.05f, 0.25f), 0.0f), 1.0f);
+            const float top = fminf(fmaxf(__fdiv_rn(0.95f - h, 0.1f), 0.0f), 1.0f);
+            const float spread = 1.0f + 0.5f * fminf(fmaxf(__fdiv_rn(h - 0.6f, 0.3f), 0.0f), 1.0f);
             v = v * base * top * spread;
         }
-        float t = (v - F.edge0) / (F.edge1 - F.edge0);
+        float t = __fdiv_rn(v - F.edge0, F.edge1 - F.edge0);
         t = fminf(fmaxf(t, 0.0f), 1.0f);
         out[idx] = t * t * (3.0f - 2.0f * t);
     }
@@ -840,7 +923,7 @@ __global__ void k_fbm(float* __restrict__ out, int nx, int ny, int nz, FbmParams
 __global__ void k_normalize(const float* __restrict__ raw, float* __restrict__ out, size_t n, float vmin,
                             float vmax) {
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
-        out[i] = (raw[i] - vmin) / (vmax - vmin);
+        out[i] = __fdiv_rn(raw[i] - vmin, vmax - vmin);
 }
 
 // ------------------------------------------------------------------------------------------------------------
