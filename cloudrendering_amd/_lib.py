@@ -37,6 +37,7 @@ SIGNATURES = {
                                           _c_int]),
     "crvpt_get_density_field": (_c_int, [_p, _c_u64, _c_int]),
     "crvpt_get_nanovdb_buffer": (_c_int, [_p, _c_u64, _pu64]),
+    "crvpt_build_nanovdb_from_dense": (_c_int, [_p, _c_int, _c_int, _c_int, _c_d, _pd, _p, _c_u64, _pu64]),
     "crvpt_set_use_sparse_grid": (_c_int, [_c_int]),
     "crvpt_set_grid_interpolation_type": (_c_int, [_c_int]),
     "crvpt_set_vpt_mode": (_c_int, [_c_i64]),
@@ -70,6 +71,9 @@ SIGNATURES = {
     "crvpt_set_denoiser_property": (_c_int, [_s, _s]),
     "crvpt_transmittance_batch": (_c_int, [_c_int, _c_int, _p, _p, _p, _c_int, _c_u32, _p, _p, _c_int, _p]),
     "crvpt_set_use_majorant_grid": (_c_int, [_c_int]),
+    "crvpt_grid_lookup_batch": (_c_int, [_c_int, _p, _p]),
+    "crvpt_grid_lookup_world_batch": (_c_int, [_c_int, _p, _p, _p]),
+    "crvpt_get_supervoxel_tables": (_c_int, [_pi, _pi, _p, _p, _c_u64]),
     "crvpt_set_launch_config": (_c_int, [_c_int, _c_int, _c_int]),
     "crvpt_get_last_render_stats": (_c_int, [_pd, _pd, _pu64, _pi]),
     "crvpt_set_collect_stats": (_c_int, [_c_int]),

@@ -191,7 +191,8 @@ struct Renderer {
     bool collectStats = false;
 
     // ---- volume ----
-    int dims[3] = {0, 0, 0};
+    int dims[3] = {0, 0, 0};        // dense grid (or, for loaded NanoVDB grids, the index bounding box) size
+    int sparseDims[3] = {0, 0, 0};  // NanoVDB index bounding box size
     double voxelSize[3] = {1.0, 1.0, 1.0};
     bool hasDense = false;
     bool hasSparse = false;
@@ -315,6 +316,7 @@ int setSparseHost(Renderer& r, std::vector<uint8_t>&& buf, bool fromDense) {
     r.nvdbHost = std::move(buf);
     r.nano = info;
     r.hasSparse = true;
+    for (int a = 0; a < 3; a++) r.sparseDims[a] = info.indexBBoxMax[a] - info.indexBBoxMin[a] + 1;
     if (!fromDense) {
         r.hasDense = false;
         r.denseOnHostValid = false;
@@ -353,6 +355,9 @@ std::vector<float> fogSphere(int& n) {
 }
 
 bool sparseActive(const Renderer& r) { return r.useSparse && r.hasSparse; }
+
+// Texel grid of the active representation: dense voxels, or the NanoVDB active index box.
+const int* activeDims(const Renderer& r) { return sparseActive(r) ? r.sparseDims : r.dims; }
 
 // Builds the NanoVDB buffer from the dense field when the sparse grid is requested and missing
 // (CloudData::getSparseDensityField, CloudData.cpp:1367-1438).
@@ -434,10 +439,11 @@ int prepareScene(Renderer& r) {
     gridViews(r, dg, ng, sp);
     if (r.tablesDirty) {
         r.svSize = chooseSvSize(r);
-        for (int a = 0; a < 3; a++) r.svGrid[a] = (r.dims[a] + r.svSize - 1) / r.svSize;
+        const int* ad = activeDims(r);
+        for (int a = 0; a < 3; a++) r.svGrid[a] = (ad[a] + r.svSize - 1) / r.svSize;
         const size_t n = size_t(r.svGrid[0]) * r.svGrid[1] * r.svGrid[2];
         HIP_TRY(r.dMajorant.ensure(n * sizeof(float)));
-        HIP_TRY(launchMajorant(sp ? nullptr : &dg, sp ? &ng : nullptr, r.dims, r.svSize, r.svGrid,
+        HIP_TRY(launchMajorant(sp ? nullptr : &dg, sp ? &ng : nullptr, ad, r.svSize, r.svGrid,
                                r.dMajorant.as<float>(), r.stream));
         r.tablesDirty = false;
         r.refTablesValid = false;
@@ -447,17 +453,18 @@ int prepareScene(Renderer& r) {
                     r.refTablesExtinction != double(extinctionX(r)))) {
         // SuperVoxelGridResidualRatioTracking / ...DecompositionTracking constructors (SuperVoxelGrid.cpp:40-91,
         // 269-405): edge 8, halved until it divides grids smaller than the edge.
+        const int* ad = activeDims(r);
         int sv = 8;
-        if (r.dims[0] < sv || r.dims[1] < sv || r.dims[2] < sv) {
-            while (!(r.dims[0] % sv == 0 && r.dims[1] % sv == 0 && r.dims[2] % sv == 0)) sv /= 2;
+        if (ad[0] < sv || ad[1] < sv || ad[2] < sv) {
+            while (!(ad[0] % sv == 0 && ad[1] % sv == 0 && ad[2] % sv == 0)) sv /= 2;
         }
         r.refSvSize = sv;
-        for (int a = 0; a < 3; a++) r.refGrid[a] = (r.dims[a] + sv - 1) / sv;
+        for (int a = 0; a < 3; a++) r.refGrid[a] = (ad[a] + sv - 1) / sv;
         const size_t n = size_t(r.refGrid[0]) * r.refGrid[1] * r.refGrid[2];
         HIP_TRY(r.dSvMin.ensure(n * sizeof(float)));
         HIP_TRY(r.dSvMax.ensure(n * sizeof(float)));
         HIP_TRY(r.dSvAvg.ensure(n * sizeof(float)));
-        HIP_TRY(launchRefSupervoxelStats(sp ? nullptr : &dg, sp ? &ng : nullptr, r.dims, sv, r.refGrid,
+        HIP_TRY(launchRefSupervoxelStats(sp ? nullptr : &dg, sp ? &ng : nullptr, ad, sv, r.refGrid,
                                          r.interpolation == INTERP_NEAREST ? 0 : 1, r.dSvMin.as<float>(),
                                          r.dSvMax.as<float>(), r.dSvAvg.as<float>(), r.stream));
         std::vector<float> mn(n), mx(n), avg(n);
@@ -476,8 +483,8 @@ int prepareScene(Renderer& r) {
             dec[2 * i] = mn[i];
             dec[2 * i + 1] = mx[i];
             float dmin = mn[i];
-            if ((r.dims[0] % sv != 0 && sx == r.refGrid[0] - 1) || (r.dims[1] % sv != 0 && sy == r.refGrid[1] - 1) ||
-                (r.dims[2] % sv != 0 && sz == r.refGrid[2] - 1))
+            if ((ad[0] % sv != 0 && sx == r.refGrid[0] - 1) || (ad[1] % sv != 0 && sy == r.refGrid[1] - 1) ||
+                (ad[2] % sv != 0 && sz == r.refGrid[2] - 1))
                 dmin = 0.0f;
             const float muMin = dmin * ext, muMax = mx[i] * ext, muAvg = avg[i] * ext;
             const float muR = std::max(muMax - muMin, 0.1f);
@@ -513,7 +520,7 @@ void fillSceneParams(Renderer& r, int w, int h, SceneParams& P, SupervoxelGrid& 
         P.boxMax[a] = float(sparse ? r.boxMaxSparse[a] : r.boxMaxDense[a]);
         P.gridMin[a] = sparse ? float(r.gridMinSparse[a]) : 0.0f;
         P.gridMax[a] = sparse ? float(r.gridMaxSparse[a]) : 1.0f;
-        P.worldToTex[a] = float(r.dims[a]) / (P.boxMax[a] - P.boxMin[a]);
+        P.worldToTex[a] = float(activeDims(r)[a]) / (P.boxMax[a] - P.boxMin[a]);
     }
     P.extinction = extinctionX(r);
     P.albedo = float(r.albedo[0]);
@@ -896,6 +903,20 @@ int crvpt_get_density_field(float* out, uint64_t count, int normalized) {
         r.denseOnHostValid = true;
     }
     for (size_t i = 0; i < n; i++) out[i] = normalized ? (r.denseHost[i] - r.vmin) / (r.vmax - r.vmin) : r.denseHost[i];
+    return CRVPT_OK;
+}
+
+int crvpt_build_nanovdb_from_dense(const float* values, int nx, int ny, int nz, double dx, const double origin[3],
+                                   uint8_t* out, uint64_t capacity, uint64_t* size) {
+    if (!values || nx <= 0 || ny <= 0 || nz <= 0 || !(dx > 0.0) || !origin || !size)
+        return fail(CRVPT_ERR_INVALID_ARGUMENT, "build_nanovdb_from_dense: invalid arguments");
+    const int dims[3] = {nx, ny, nz};
+    std::vector<uint8_t> b = buildNanoVdbFromDense(values, dims, dx, origin, "tmp");
+    *size = b.size();
+    if (out) {
+        if (capacity < b.size()) return fail(CRVPT_ERR_INVALID_ARGUMENT, "build_nanovdb_from_dense: too small");
+        std::memcpy(out, b.data(), b.size());
+    }
     return CRVPT_OK;
 }
 
@@ -1287,6 +1308,75 @@ int crvpt_transmittance_batch(int estimator, int n_rays, const float* origins, c
         HIP_TRY(hipStreamSynchronize(s));
         HIP_TRY(hipMemcpy(mean, mo, size_t(n_rays) * sizeof(float), hipMemcpyDeviceToHost));
         if (m2) HIP_TRY(hipMemcpy(m2, m2o, size_t(n_rays) * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    return CRVPT_OK;
+}
+
+int crvpt_grid_lookup_batch(int n, const int32_t* ijk, float* out) {
+    if (int rc = requireInit()) return rc;
+    if (n < 0 || (n > 0 && (!ijk || !out))) return fail(CRVPT_ERR_INVALID_ARGUMENT, "grid_lookup_batch: bad arguments");
+    Renderer& r = *g_r;
+    if (int rc = prepareScene(r)) return rc;
+    if (n == 0) return CRVPT_OK;
+    DenseGrid dg;
+    NanoGrid ng;
+    bool sp;
+    gridViews(r, dg, ng, sp);
+    HIP_TRY(r.dRays.ensure(size_t(n) * 16));
+    int32_t* dIjk = r.dRays.as<int32_t>();
+    float* dOut = reinterpret_cast<float*>(dIjk + 3 * size_t(n));
+    HIP_TRY(hipStreamSynchronize(r.stream));
+    HIP_TRY(hipMemcpy(dIjk, ijk, 12 * size_t(n), hipMemcpyHostToDevice));
+    HIP_TRY(launchGridLookup(sp ? nullptr : &dg, sp ? &ng : nullptr, n, dIjk, dOut, r.stream));
+    HIP_TRY(hipStreamSynchronize(r.stream));
+    HIP_TRY(hipMemcpy(out, dOut, 4 * size_t(n), hipMemcpyDeviceToHost));
+    return CRVPT_OK;
+}
+
+int crvpt_grid_lookup_world_batch(int n, const float* xyz, float* out, int32_t* ijk_out) {
+    if (int rc = requireInit()) return rc;
+    if (n < 0 || (n > 0 && (!xyz || !out || !ijk_out)))
+        return fail(CRVPT_ERR_INVALID_ARGUMENT, "grid_lookup_world_batch: bad arguments");
+    Renderer& r = *g_r;
+    if (int rc = prepareScene(r)) return rc;
+    if (!sparseActive(r)) return fail(CRVPT_ERR_UNSUPPORTED, "grid_lookup_world_batch: needs the sparse grid");
+    if (n == 0) return CRVPT_OK;
+    DenseGrid dg;
+    NanoGrid ng;
+    bool sp;
+    gridViews(r, dg, ng, sp);
+    HIP_TRY(r.dRays.ensure(size_t(n) * 28));
+    float* dXyz = r.dRays.as<float>();
+    float* dOut = dXyz + 3 * size_t(n);
+    int32_t* dIjk = reinterpret_cast<int32_t*>(dOut + n);
+    HIP_TRY(hipStreamSynchronize(r.stream));
+    HIP_TRY(hipMemcpy(dXyz, xyz, 12 * size_t(n), hipMemcpyHostToDevice));
+    HIP_TRY(launchNanoLookupWorld(ng, n, dXyz, dOut, dIjk, r.stream));
+    HIP_TRY(hipStreamSynchronize(r.stream));
+    HIP_TRY(hipMemcpy(out, dOut, 4 * size_t(n), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(ijk_out, dIjk, 12 * size_t(n), hipMemcpyDeviceToHost));
+    return CRVPT_OK;
+}
+
+int crvpt_get_supervoxel_tables(int* supervoxel_size, int grid_size[3], float* decomposition_min_max,
+                                float* rrt_mu_c_mu_r, uint64_t capacity) {
+    if (int rc = requireInit()) return rc;
+    Renderer& r = *g_r;
+    const int prevMode = r.mode;
+    r.mode = VPT_DECOMPOSITION_TRACKING;
+    int rc = prepareScene(r);
+    r.mode = prevMode;
+    if (rc) return rc;
+    const size_t n = size_t(r.refGrid[0]) * r.refGrid[1] * r.refGrid[2];
+    if (supervoxel_size) *supervoxel_size = r.refSvSize;
+    if (grid_size)
+        for (int a = 0; a < 3; a++) grid_size[a] = r.refGrid[a];
+    if (decomposition_min_max || rrt_mu_c_mu_r) {
+        if (capacity < 2 * n) return fail(CRVPT_ERR_INVALID_ARGUMENT, "get_supervoxel_tables: output too small");
+        HIP_TRY(hipStreamSynchronize(r.stream));
+        if (decomposition_min_max)
+            HIP_TRY(hipMemcpy(decomposition_min_max, r.dDecomp.ptr, 2 * n * sizeof(float), hipMemcpyDeviceToHost));
+        if (rrt_mu_c_mu_r) HIP_TRY(hipMemcpy(rrt_mu_c_mu_r, r.dRrt.ptr, 2 * n * sizeof(float), hipMemcpyDeviceToHost));
     }
     return CRVPT_OK;
 }

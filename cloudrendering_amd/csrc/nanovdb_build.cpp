@@ -20,7 +20,7 @@ constexpr uint64_t kMagic = 0x304244566f6e614eULL;  // "NanoVDB0"
 constexpr uint32_t kVersion = (32u << 21) | (3u << 10) | 3u;
 constexpr uint32_t kGridSize = 672, kTreeSize = 64, kRootSize = 64, kTileSize = 32;
 constexpr uint32_t kUpperSize = 270400, kLowerSize = 33856, kLeafSize = 2144;
-constexpr uint32_t kGridTypeFloat = 1, kGridClassFog = 2;
+constexpr uint32_t kGridTypeFloat = 1;
 
 template <class T>
 void put(std::vector<uint8_t>& b, size_t off, T v) {
@@ -59,56 +59,80 @@ struct Upper {
     uint64_t offset = 0;
 };
 
+// Running statistics in the incremental form nanovdb::Stats uses (GridStats.h:243-297): min, max, mean and the
+// aggregated squared deviation, so node averages and deviations round exactly as the reference's.
 struct Stats {
     int32_t bmin[3] = {INT32_MAX, INT32_MAX, INT32_MAX};
     int32_t bmax[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
     float mn = 3.402823466e38f, mx = -3.402823466e38f;
-    double sum = 0.0, sum2 = 0.0;
     uint64_t count = 0;
-    void addVoxel(int x, int y, int z, float v) {
-        const int c[3] = {x, y, z};
-        for (int a = 0; a < 3; a++) {
-            bmin[a] = std::min(bmin[a], c[a]);
-            bmax[a] = std::max(bmax[a], c[a]);
-        }
+    double avgD = 0.0, aux = 0.0;
+    bool empty() const { return bmin[0] > bmax[0] || bmin[1] > bmax[1] || bmin[2] > bmax[2]; }
+    void extrema(float v) {
         mn = std::min(mn, v);
         mx = std::max(mx, v);
-        sum += v;
-        sum2 += double(v) * v;
-        count++;
     }
-    void addTile(const Coord& o, int dim, float v) {
-        const int lo[3] = {o.x, o.y, o.z};
+    void addValue(float v) {
+        extrema(v);
+        count += 1;
+        const double delta = double(v) - avgD;
+        avgD += delta / double(count);
+        aux += delta * (double(v) - avgD);
+    }
+    void addValue(float v, uint64_t n) {
+        const double denom = 1.0 / double(count + n);
+        const double delta = double(v) - avgD;
+        avgD += denom * delta * double(n);
+        aux += denom * delta * delta * double(count) * double(n);
+        extrema(v);
+        count += n;
+    }
+    void addStats(const Stats& o) {
+        if (o.count > 0) {
+            const double denom = 1.0 / double(count + o.count);
+            const double delta = o.avgD - avgD;
+            avgD += denom * delta * double(o.count);
+            aux += o.aux + denom * delta * delta * double(count) * double(o.count);
+            extrema(o.mn);
+            extrema(o.mx);
+            count += o.count;
+        }
+    }
+    void addBox(const int32_t lo[3], const int32_t hi[3]) {
         for (int a = 0; a < 3; a++) {
             bmin[a] = std::min(bmin[a], lo[a]);
-            bmax[a] = std::max(bmax[a], lo[a] + dim - 1);
+            bmax[a] = std::max(bmax[a], hi[a]);
         }
-        const double n = double(dim) * dim * dim;
-        mn = std::min(mn, v);
-        mx = std::max(mx, v);
-        sum += v * n;
-        sum2 += double(v) * v * n;
-        count += uint64_t(n);
     }
-    void merge(const Stats& o) {
-        if (!o.count) return;
-        for (int a = 0; a < 3; a++) {
-            bmin[a] = std::min(bmin[a], o.bmin[a]);
-            bmax[a] = std::max(bmax[a], o.bmax[a]);
-        }
-        mn = std::min(mn, o.mn);
-        mx = std::max(mx, o.mx);
-        sum += o.sum;
-        sum2 += o.sum2;
-        count += o.count;
+    void merge(const Stats& o) {  // NodeStats::add: bounding box union and statistics
+        if (!o.empty()) addBox(o.bmin, o.bmax);
+        addStats(o);
     }
-    float avg() const { return count ? float(sum / double(count)) : 0.0f; }
-    float stddev() const {
-        if (!count) return 0.0f;
-        const double m = sum / double(count);
-        return float(std::sqrt(std::max(0.0, sum2 / double(count) - m * m)));
-    }
+    float avg() const { return float(avgD); }
+    float stddev() const { return float(std::sqrt(count < 2 ? 0.0 : aux / double(count))); }
 };
+
+// nanovdb::Map::applyMap for the scale-translate map: fma(x, dx, T) per axis (NanoVDB.h:1433-1438).
+inline void applyMap(double dx, const double p0[3], const double in[3], double out[3]) {
+    for (int a = 0; a < 3; a++) out[a] = std::fma(in[a], dx, std::fma(0.0, 0.0, std::fma(0.0, 0.0, p0[a])));
+}
+
+// CRC-32 (reflected polynomial 0xEDB88320, as in nanovdb::CRC32, GridChecksum.h:62-96).
+uint32_t crc32(const uint8_t* data, size_t n) {
+    static uint32_t lut[256];
+    static bool init = false;
+    if (!init) {
+        for (uint32_t i = 0; i < 256; i++) {
+            uint32_t c = i;
+            for (int k = 0; k < 8; k++) c = (c >> 1) ^ ((c & 1u) ? 0xEDB88320u : 0u);
+            lut[i] = c;
+        }
+        init = true;
+    }
+    uint32_t c = 0xFFFFFFFFu;
+    for (size_t i = 0; i < n; i++) c = lut[(c ^ data[i]) & 0xFFu] ^ (c >> 8);
+    return ~c;
+}
 
 void putCoord(std::vector<uint8_t>& b, size_t off, const int32_t c[3]) {
     for (int a = 0; a < 3; a++) put<int32_t>(b, off + 4 * a, c[a]);
@@ -206,11 +230,12 @@ bool extractNanoVdbGridFromFile(const std::vector<uint8_t>& file, std::vector<ui
 
 std::vector<uint8_t> buildNanoVdbFromDense(const float* values, const int dims[3], double dx, const double p0[3],
                                            const std::string& name) {
-    // 1. Leaves: every 8^3 block with at least one non-background voxel (GridBuilder.h:251-300).
+    // 1. Leaves: every 8^3 block with at least one non-background voxel (GridBuilder.h:251-300); a fully active
+    //    constant leaf becomes an active tile of its lower node.
     std::vector<Leaf> leaves;
     std::map<Coord, int> leafIndex;
     const int lx = (dims[0] + 7) / 8, ly = (dims[1] + 7) / 8, lz = (dims[2] + 7) / 8;
-    std::vector<std::pair<Coord, float>> lowerTiles;  // pruned constant, fully active leaves
+    std::vector<std::pair<Coord, float>> leafTiles;
     for (int bz = 0; bz < lz; bz++)
         for (int by = 0; by < ly; by++)
             for (int bx = 0; bx < lx; bx++) {
@@ -236,14 +261,15 @@ std::vector<uint8_t> buildNanoVdbFromDense(const float* values, const int dims[3
                     bool constant = true;
                     for (int n = 1; n < 512 && constant; n++) constant = L.values[n] == L.values[0];
                     if (constant) {
-                        lowerTiles.push_back({L.origin, L.values[0]});
+                        leafTiles.push_back({L.origin, L.values[0]});
                         continue;
                     }
                 }
                 leafIndex[L.origin] = int(leaves.size());
                 leaves.push_back(L);
             }
-    // 2. Internal nodes.
+    // 2. Internal nodes, then the reference's pruning (GridBuilder.h:303-343): a lower node holding only tiles of
+    //    one value becomes an active upper tile, an upper node holding only tiles of one value a root tile.
     std::map<Coord, int> lowerIndex, upperIndex;
     std::vector<Lower> lowers;
     std::vector<Upper> uppers;
@@ -259,15 +285,21 @@ std::vector<uint8_t> buildNanoVdbFromDense(const float* values, const int dims[3
         std::fill(std::begin(n.child), std::end(n.child), -1);
         return n;
     };
+    auto lowerSlot = [](const Coord& c) {
+        return ((c.x & 127) >> 3) << 8 | ((c.y & 127) >> 3) << 4 | ((c.z & 127) >> 3);
+    };
+    auto upperSlot = [](const Coord& c) {
+        return ((c.x & 4095) >> 7) << 10 | ((c.y & 4095) >> 7) << 5 | ((c.z & 4095) >> 7);
+    };
     for (auto& kv : leafIndex) {
         Lower& n = lowerOf(kv.first);
-        const int i = ((kv.first.x & 127) >> 3) << 8 | ((kv.first.y & 127) >> 3) << 4 | ((kv.first.z & 127) >> 3);
+        const int i = lowerSlot(kv.first);
         n.child[i] = kv.second;
         n.childMask[i >> 6] |= 1ull << (i & 63);
     }
-    for (auto& t : lowerTiles) {
+    for (auto& t : leafTiles) {
         Lower& n = lowerOf(t.first);
-        const int i = ((t.first.x & 127) >> 3) << 8 | ((t.first.y & 127) >> 3) << 4 | ((t.first.z & 127) >> 3);
+        const int i = lowerSlot(t.first);
         n.tileValue[i] = t.second;
         n.valueMask[i >> 6] |= 1ull << (i & 63);
     }
@@ -284,16 +316,47 @@ std::vector<uint8_t> buildNanoVdbFromDense(const float* values, const int dims[3
         return n;
     };
     for (auto& kv : lowerIndex) {
+        Lower& l = lowers[size_t(kv.second)];
         Upper& n = upperOf(kv.first);
-        const int i = ((kv.first.x & 4095) >> 7) << 10 | ((kv.first.y & 4095) >> 7) << 5 | ((kv.first.z & 4095) >> 7);
+        const int i = upperSlot(kv.first);
+        bool onlyTiles = true;
+        for (int w = 0; w < 64 && onlyTiles; w++) onlyTiles = l.childMask[w] == 0ull;
+        bool same = onlyTiles;
+        for (int j = 1; j < 4096 && same; j++) same = l.tileValue[j] == l.tileValue[0];
+        if (same) {
+            n.tileValue[i] = l.tileValue[0];
+            n.valueMask[i >> 6] |= 1ull << (i & 63);
+            continue;
+        }
         n.child[i] = kv.second;
         n.childMask[i >> 6] |= 1ull << (i & 63);
     }
-    // 3. Breadth-first offsets in root-table / mask order.
-    std::vector<int> upperOrder, lowerOrder, leafOrder;
+    struct RootTile {
+        int upper;  // -1: tile
+        float value;
+        uint32_t state;
+    };
+    std::map<Coord, RootTile> rootTable;
     for (auto& kv : upperIndex) {
-        Upper& u = uppers[size_t(kv.second)];
-        upperOrder.push_back(kv.second);
+        const Upper& u = uppers[size_t(kv.second)];
+        bool onlyTiles = true;
+        for (int w = 0; w < 512 && onlyTiles; w++) onlyTiles = u.childMask[w] == 0ull;
+        bool same = onlyTiles;
+        for (int j = 1; j < 32768 && same; j++) same = u.tileValue[j] == u.tileValue[0];
+        if (same) {
+            bool allOn = true;
+            for (int w = 0; w < 512 && allOn; w++) allOn = u.valueMask[w] == ~0ull;
+            rootTable[kv.first] = RootTile{-1, u.tileValue[0], allOn ? 1u : 0u};
+        } else {
+            rootTable[kv.first] = RootTile{kv.second, 0.0f, 0u};
+        }
+    }
+    // 3. Breadth-first offsets in root-table / mask order (GridBuilder.h:346-378).
+    std::vector<int> upperOrder, lowerOrder, leafOrder;
+    for (auto& kv : rootTable) {
+        if (kv.second.upper < 0) continue;
+        Upper& u = uppers[size_t(kv.second.upper)];
+        upperOrder.push_back(kv.second.upper);
         for (int i = 0; i < 32768; i++)
             if (u.child[i] >= 0) {
                 lowerOrder.push_back(u.child[i]);
@@ -302,7 +365,7 @@ std::vector<uint8_t> buildNanoVdbFromDense(const float* values, const int dims[3
                     if (l.child[j] >= 0) leafOrder.push_back(l.child[j]);
             }
     }
-    const uint64_t tiles = upperIndex.size();
+    const uint64_t tiles = rootTable.size();
     const uint64_t rootOff = kGridSize + kTreeSize;
     const uint64_t upperBase = rootOff + kRootSize + kTileSize * tiles;
     const uint64_t lowerBase = upperBase + uint64_t(kUpperSize) * upperOrder.size();
@@ -313,22 +376,25 @@ std::vector<uint8_t> buildNanoVdbFromDense(const float* values, const int dims[3
     for (size_t i = 0; i < leafOrder.size(); i++) leaves[size_t(leafOrder[i])].offset = leafBase + kLeafSize * i;
 
     std::vector<uint8_t> b(total, 0);
-    // 4. Leaves.
+    // 4. Leaves: values, masks and the statistics of GridStats::process(Node0) (GridStats.h:685-708).
     std::vector<Stats> leafStats(leaves.size());
     uint64_t voxelCount = 0;
     for (size_t li = 0; li < leaves.size(); li++) {
         const Leaf& L = leaves[li];
         Stats& s = leafStats[li];
         for (int n = 0; n < 512; n++)
-            if ((L.mask[n >> 6] >> (n & 63)) & 1ull)
-                s.addVoxel(L.origin.x + (n >> 6), L.origin.y + ((n >> 3) & 7), L.origin.z + (n & 7), L.values[n]);
+            if ((L.mask[n >> 6] >> (n & 63)) & 1ull) {
+                const int32_t c[3] = {L.origin.x + (n >> 6), L.origin.y + ((n >> 3) & 7), L.origin.z + (n & 7)};
+                s.addBox(c, c);
+                s.addValue(L.values[n]);
+            }
         voxelCount += s.count;
         const size_t o = size_t(L.offset);
         putCoord(b, o, s.bmin);
         const uint8_t dif[3] = {uint8_t(s.bmax[0] - s.bmin[0]), uint8_t(s.bmax[1] - s.bmin[1]),
                                 uint8_t(s.bmax[2] - s.bmin[2])};
         std::memcpy(b.data() + o + 12, dif, 3);
-        b[o + 15] = 0;  // flags
+        b[o + 15] = 2;  // flags: the leaf holds active values
         std::memcpy(b.data() + o + 16, L.mask, 64);
         put<float>(b, o + 80, s.mn);
         put<float>(b, o + 84, s.mx);
@@ -336,43 +402,62 @@ std::vector<uint8_t> buildNanoVdbFromDense(const float* values, const int dims[3
         put<float>(b, o + 92, s.stddev());
         std::memcpy(b.data() + o + 96, L.values, 2048);
     }
-    // 5. Lower nodes.
+    // 5. Internal nodes (GridStats.h:628-680): active tiles first (value-mask order), then children.
+    uint32_t tileCount[3] = {0, 0, 0};
     std::vector<Stats> lowerStats(lowers.size());
-    uint32_t lowerTileCount = 0;
     for (size_t ni = 0; ni < lowers.size(); ni++) {
         const Lower& n = lowers[ni];
+        if (n.offset == 0) continue;  // pruned into an upper tile
         Stats& s = lowerStats[ni];
         const size_t o = size_t(n.offset);
         for (int i = 0; i < 4096; i++) {
-            const Coord c{n.origin.x + ((i >> 8) << 3), n.origin.y + (((i >> 4) & 15) << 3), n.origin.z + ((i & 15) << 3)};
+            if ((n.valueMask[i >> 6] >> (i & 63)) & 1ull) {
+                const int32_t lo[3] = {n.origin.x + ((i >> 8) << 3), n.origin.y + (((i >> 4) & 15) << 3),
+                                       n.origin.z + ((i & 15) << 3)};
+                const int32_t hi[3] = {lo[0] + 7, lo[1] + 7, lo[2] + 7};
+                s.addValue(n.tileValue[i], 512);
+                s.addBox(lo, hi);
+                tileCount[0]++;
+                voxelCount += 512;
+            }
+        }
+        for (int i = 0; i < 4096; i++) {
             if (n.child[i] >= 0) {
                 s.merge(leafStats[size_t(n.child[i])]);
                 put<int64_t>(b, o + 1088 + 8 * size_t(i), int64_t(leaves[size_t(n.child[i])].offset - n.offset));
             } else {
                 put<float>(b, o + 1088 + 8 * size_t(i), n.tileValue[i]);
-                if ((n.valueMask[i >> 6] >> (i & 63)) & 1ull) {
-                    s.addTile(c, 8, n.tileValue[i]);
-                    lowerTileCount++;
-                    voxelCount += 512;
-                }
             }
         }
         putCoord(b, o, s.bmin);
         putCoord(b, o + 12, s.bmax);
-        put<uint64_t>(b, o + 24, 0);
+        put<uint64_t>(b, o + 24, s.empty() ? 0u : 2u);
         std::memcpy(b.data() + o + 32, n.valueMask, 512);
         std::memcpy(b.data() + o + 544, n.childMask, 512);
-        put<float>(b, o + 1056, s.mn);
-        put<float>(b, o + 1060, s.mx);
-        put<float>(b, o + 1064, s.avg());
-        put<float>(b, o + 1068, s.stddev());
+        if (!s.empty()) {
+            put<float>(b, o + 1056, s.mn);
+            put<float>(b, o + 1060, s.mx);
+            put<float>(b, o + 1064, s.avg());
+            put<float>(b, o + 1068, s.stddev());
+        }
     }
-    // 6. Upper nodes.
     std::vector<Stats> upperStats(uppers.size());
     for (size_t ni = 0; ni < uppers.size(); ni++) {
         const Upper& n = uppers[ni];
+        if (n.offset == 0) continue;  // pruned into a root tile
         Stats& s = upperStats[ni];
         const size_t o = size_t(n.offset);
+        for (int i = 0; i < 32768; i++) {
+            if ((n.valueMask[i >> 6] >> (i & 63)) & 1ull) {
+                const int32_t lo[3] = {n.origin.x + ((i >> 10) << 7), n.origin.y + (((i >> 5) & 31) << 7),
+                                       n.origin.z + ((i & 31) << 7)};
+                const int32_t hi[3] = {lo[0] + 127, lo[1] + 127, lo[2] + 127};
+                s.addValue(n.tileValue[i], 4096 * 512);
+                s.addBox(lo, hi);
+                tileCount[1]++;
+                voxelCount += 4096ull * 512ull;
+            }
+        }
         for (int i = 0; i < 32768; i++) {
             if (n.child[i] >= 0) {
                 s.merge(lowerStats[size_t(n.child[i])]);
@@ -383,47 +468,68 @@ std::vector<uint8_t> buildNanoVdbFromDense(const float* values, const int dims[3
         }
         putCoord(b, o, s.bmin);
         putCoord(b, o + 12, s.bmax);
-        put<uint64_t>(b, o + 24, 0);
+        put<uint64_t>(b, o + 24, s.empty() ? 0u : 2u);
         std::memcpy(b.data() + o + 32, n.valueMask, 4096);
         std::memcpy(b.data() + o + 4128, n.childMask, 4096);
-        put<float>(b, o + 8224, s.mn);
-        put<float>(b, o + 8228, s.mx);
-        put<float>(b, o + 8232, s.avg());
-        put<float>(b, o + 8236, s.stddev());
+        if (!s.empty()) {
+            put<float>(b, o + 8224, s.mn);
+            put<float>(b, o + 8228, s.mx);
+            put<float>(b, o + 8232, s.avg());
+            put<float>(b, o + 8236, s.stddev());
+        }
     }
-    // 7. Root and tiles.
+    // 6. Root and its tiles (GridBuilder.h processRoot, GridStats.h:583-624).
     Stats rootStats;
     {
         size_t t = 0;
-        for (auto& kv : upperIndex) {
-            const Upper& u = uppers[size_t(kv.second)];
-            rootStats.merge(upperStats[size_t(kv.second)]);
+        for (auto& kv : rootTable) {
             const size_t o = size_t(rootOff + kRootSize + kTileSize * t);
-            const uint64_t key = uint64_t(uint32_t(u.origin.z) >> 12) | (uint64_t(uint32_t(u.origin.y) >> 12) << 21) |
-                                 (uint64_t(uint32_t(u.origin.x) >> 12) << 42);
+            const Coord& c = kv.first;
+            const uint64_t key = uint64_t(uint32_t(c.z) >> 12) | (uint64_t(uint32_t(c.y) >> 12) << 21) |
+                                 (uint64_t(uint32_t(c.x) >> 12) << 42);
             put<uint64_t>(b, o, key);
-            put<int64_t>(b, o + 8, int64_t(u.offset - rootOff));
-            put<uint32_t>(b, o + 16, 0);
-            put<float>(b, o + 20, 0.0f);
+            if (kv.second.upper >= 0) {
+                const Upper& u = uppers[size_t(kv.second.upper)];
+                rootStats.merge(upperStats[size_t(kv.second.upper)]);
+                put<int64_t>(b, o + 8, int64_t(u.offset - rootOff));
+                put<uint32_t>(b, o + 16, 0);
+                put<float>(b, o + 20, 0.0f);
+            } else {
+                put<int64_t>(b, o + 8, 0);
+                put<uint32_t>(b, o + 16, kv.second.state);
+                put<float>(b, o + 20, kv.second.value);
+                if (kv.second.state) {
+                    const int32_t lo[3] = {c.x, c.y, c.z}, hi[3] = {c.x + 4095, c.y + 4095, c.z + 4095};
+                    rootStats.addBox(lo, hi);
+                    rootStats.addValue(kv.second.value, 4096ull * 4096ull * 4096ull);
+                    tileCount[2]++;
+                    voxelCount += 4096ull * 4096ull * 4096ull;
+                }
+            }
             t++;
         }
     }
-    if (!rootStats.count) {
-        for (int a = 0; a < 3; a++) {
-            rootStats.bmin[a] = 0;
-            rootStats.bmax[a] = 0;
-        }
-        rootStats.mn = rootStats.mx = 0.0f;
+    const bool rootEmpty = rootStats.empty();
+    if (rootEmpty) {
+        const int32_t emptyMin[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, emptyMax[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+        putCoord(b, rootOff, emptyMin);
+        putCoord(b, rootOff + 12, emptyMax);
+    } else {
+        putCoord(b, rootOff, rootStats.bmin);
+        putCoord(b, rootOff + 12, rootStats.bmax);
     }
-    putCoord(b, rootOff, rootStats.bmin);
-    putCoord(b, rootOff + 12, rootStats.bmax);
     put<uint32_t>(b, rootOff + 24, uint32_t(tiles));
     put<float>(b, rootOff + 28, 0.0f);
-    put<float>(b, rootOff + 32, rootStats.mn);
-    put<float>(b, rootOff + 36, rootStats.mx);
-    put<float>(b, rootOff + 40, rootStats.avg());
-    put<float>(b, rootOff + 44, rootStats.stddev());
-    // 8. Tree.
+    if (tiles == 0) {
+        put<float>(b, rootOff + 32, 0.0f);
+        put<float>(b, rootOff + 36, 0.0f);
+    } else {
+        put<float>(b, rootOff + 32, rootStats.count ? rootStats.mn : 0.0f);
+        put<float>(b, rootOff + 36, rootStats.count ? rootStats.mx : 0.0f);
+        put<float>(b, rootOff + 40, rootStats.avg());
+        put<float>(b, rootOff + 44, rootStats.stddev());
+    }
+    // 7. Tree.
     const size_t tree = kGridSize;
     put<uint64_t>(b, tree + 0, leafOrder.empty() ? 0 : leafBase - tree);
     put<uint64_t>(b, tree + 8, lowerOrder.empty() ? 0 : lowerBase - tree);
@@ -432,15 +538,15 @@ std::vector<uint8_t> buildNanoVdbFromDense(const float* values, const int dims[3
     put<uint32_t>(b, tree + 32, uint32_t(leafOrder.size()));
     put<uint32_t>(b, tree + 36, uint32_t(lowerOrder.size()));
     put<uint32_t>(b, tree + 40, uint32_t(upperOrder.size()));
-    put<uint32_t>(b, tree + 44, lowerTileCount);
-    put<uint32_t>(b, tree + 48, 0);
-    put<uint32_t>(b, tree + 52, 0);
+    put<uint32_t>(b, tree + 44, tileCount[0]);
+    put<uint32_t>(b, tree + 48, tileCount[1]);
+    put<uint32_t>(b, tree + 52, tileCount[2]);
     put<uint64_t>(b, tree + 56, voxelCount);
-    // 9. Grid.
+    // 8. Grid (GridBuilder.h processGrid; GridStats.h:531-570). The reference passes GridClass::FogVolume in the
+    //    oracle slot of getHandle (CloudData.cpp:1398-1400), so the class stays the builder default, Unknown.
     put<uint64_t>(b, 0, kMagic);
-    put<uint64_t>(b, 8, 0);
     put<uint32_t>(b, 16, kVersion);
-    put<uint32_t>(b, 20, (1u << 1) | (1u << 2) | (1u << 5));  // HasBBox | HasMinMax | IsBreadthFirst
+    put<uint32_t>(b, 20, rootEmpty ? (4u | 8u | 16u | 32u) : (2u | 4u | 8u | 16u | 32u));
     put<uint32_t>(b, 24, 0);
     put<uint32_t>(b, 28, 1);
     put<uint64_t>(b, 32, total);
@@ -459,16 +565,45 @@ std::vector<uint8_t> buildNanoVdbFromDense(const float* values, const int dims[3
     }
     put<float>(b, map + 84, 1.0f);
     put<double>(b, map + 256, 1.0);
-    // World bounding box of the active index box [min, max + 1) through the map.
-    for (int a = 0; a < 3; a++) {
-        put<double>(b, 560 + 8 * a, double(rootStats.bmin[a]) * dx + p0[a]);
-        put<double>(b, 584 + 8 * a, double(rootStats.bmax[a] + 1) * dx + p0[a]);
-        put<double>(b, 608 + 8 * a, dx);
+    // World bounding box: the eight corners of the index box [min, max + 1) through the map.
+    if (!rootEmpty) {
+        double wmin[3] = {1e308, 1e308, 1e308}, wmax[3] = {-1e308, -1e308, -1e308};
+        for (int k = 0; k < 8; k++) {
+            const double c[3] = {double((k & 1) ? rootStats.bmax[0] + 1 : rootStats.bmin[0]),
+                                 double((k & 2) ? rootStats.bmax[1] + 1 : rootStats.bmin[1]),
+                                 double((k & 4) ? rootStats.bmax[2] + 1 : rootStats.bmin[2])};
+            double w[3];
+            applyMap(dx, p0, c, w);
+            for (int a = 0; a < 3; a++) {
+                wmin[a] = std::min(wmin[a], w[a]);
+                wmax[a] = std::max(wmax[a], w[a]);
+            }
+        }
+        for (int a = 0; a < 3; a++) {
+            put<double>(b, 560 + 8 * a, wmin[a]);
+            put<double>(b, 584 + 8 * a, wmax[a]);
+        }
+    } else {
+        for (int a = 0; a < 3; a++) {
+            put<double>(b, 560 + 8 * a, 1.7976931348623157e308);
+            put<double>(b, 584 + 8 * a, -1.7976931348623157e308);
+        }
     }
-    put<uint32_t>(b, 632, kGridClassFog);
+    {
+        const double one[3] = {1.0, 1.0, 1.0}, zero[3] = {0.0, 0.0, 0.0};
+        double m1[3], m0[3];
+        applyMap(dx, p0, one, m1);
+        applyMap(dx, p0, zero, m0);
+        for (int a = 0; a < 3; a++) put<double>(b, 608 + 8 * a, m1[a] - m0[a]);
+    }
+    put<uint32_t>(b, 632, 0u);  // GridClass::Unknown
     put<uint32_t>(b, 636, kGridTypeFloat);
     put<int64_t>(b, 640, 0);
     put<uint32_t>(b, 648, 0);
+    // 9. Partial checksum (GridChecksum.h:135-150): CRC-32 of grid, tree and root bytes after the checksum field in
+    //    the low word, the empty tail CRC in the high word.
+    const uint32_t head = crc32(b.data() + 16, size_t(rootOff + kRootSize + kTileSize * tiles) - 16);
+    put<uint64_t>(b, 8, (uint64_t(0xFFFFFFFFu) << 32) | head);
     return b;
 }
 

@@ -927,6 +927,28 @@ __global__ void k_normalize(const float* __restrict__ raw, float* __restrict__ o
 }
 
 // ------------------------------------------------------------------------------------------------------------
+// Grid lookup probes (exactness checks of the accessors)
+// ------------------------------------------------------------------------------------------------------------
+template <class Grid>
+__global__ void k_gridLookup(Grid grid, int n, const int32_t* __restrict__ ijk, float* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = grid.voxelAt(ijk[3 * i], ijk[3 * i + 1], ijk[3 * i + 2]);
+}
+
+// NanoVDB world (grid) coordinates -> world_to_indexf -> floor -> value, the nearest sampleCloudRaw path.
+__global__ void k_nanoLookupWorld(NanoAccess grid, int n, const float* __restrict__ xyz, float* __restrict__ out,
+                                  int32_t* __restrict__ ijkOut) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const V3 idx = grid.worldToIndex(v3(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]));
+    const int a = int(floorf(idx.x)), b = int(floorf(idx.y)), c = int(floorf(idx.z));
+    ijkOut[3 * i] = a;
+    ijkOut[3 * i + 1] = b;
+    ijkOut[3 * i + 2] = c;
+    out[i] = grid.value(a, b, c);
+}
+
+// ------------------------------------------------------------------------------------------------------------
 // Host-side launchers
 // ------------------------------------------------------------------------------------------------------------
 template <class Grid>
@@ -1035,6 +1057,27 @@ hipError_t launchFbm(float* out, int nx, int ny, int nz, const FbmParams& F, hip
 hipError_t launchNormalize(const float* raw, float* out, size_t n, float vmin, float vmax, hipStream_t stream) {
     const size_t blocks = std::min<size_t>((n + 255) / 256, 65536);
     hipLaunchKernelGGL(k_normalize, dim3(unsigned(blocks)), dim3(256), 0, stream, raw, out, n, vmin, vmax);
+    return hipGetLastError();
+}
+
+hipError_t launchGridLookup(const DenseGrid* dense, const NanoGrid* nano, int n, const int32_t* ijk, float* out,
+                            hipStream_t stream) {
+    const dim3 grid((n + 255) / 256), block(256);
+    if (nano) {
+        // index-space coordinates: voxelAt adds the index minimum, so pass raw NanoVDB coordinates through value()
+        NanoGrid g = *nano;
+        g.indexMin[0] = g.indexMin[1] = g.indexMin[2] = 0;
+        hipLaunchKernelGGL(k_gridLookup<NanoAccess>, grid, block, 0, stream, NanoAccess{g}, n, ijk, out);
+    } else {
+        hipLaunchKernelGGL(k_gridLookup<DenseAccess>, grid, block, 0, stream, DenseAccess{*dense}, n, ijk, out);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launchNanoLookupWorld(const NanoGrid& nano, int n, const float* xyz, float* out, int32_t* ijk,
+                                 hipStream_t stream) {
+    hipLaunchKernelGGL(k_nanoLookupWorld, dim3((n + 255) / 256), dim3(256), 0, stream, NanoAccess{nano}, n, xyz, out,
+                       ijk);
     return hipGetLastError();
 }
 

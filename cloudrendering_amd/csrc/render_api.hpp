@@ -46,6 +46,10 @@ hipError_t launchRefSupervoxelStats(const DenseGrid* dense, const NanoGrid* nano
 hipError_t launchFbm(float* out, int nx, int ny, int nz, const FbmParams& F, hipStream_t stream);
 hipError_t launchNormalize(const float* raw, float* out, size_t n, float vmin, float vmax, hipStream_t stream);
 int occupancyBlocksPerCU(int mode, bool sparse);
+hipError_t launchGridLookup(const DenseGrid* dense, const NanoGrid* nano, int n, const int32_t* ijk, float* out,
+                            hipStream_t stream);
+hipError_t launchNanoLookupWorld(const NanoGrid& nano, int n, const float* xyz, float* out, int32_t* ijk,
+                                 hipStream_t stream);
 
 // Denoisers (denoise.hip)
 struct EawParams {

@@ -30,7 +30,8 @@ __all__ = [
     "set_use_sparse_grid", "set_grid_interpolation_type", "set_density_field", "set_nanovdb_buffer",
     "get_density_field", "get_nanovdb_buffer", "generate_fbm_cloud", "transmittance_batch",
     "set_use_majorant_grid", "set_launch_config", "get_last_render_stats", "get_num_lights",
-    "set_collect_stats", "get_last_render_counters",
+    "set_collect_stats", "get_last_render_counters", "build_nanovdb_from_dense", "grid_lookup",
+    "grid_lookup_world", "get_supervoxel_tables",
     "VPT_MODE_NAMES", "CrvptError",
 ]
 
@@ -370,3 +371,48 @@ def get_last_render_counters() -> dict:
     v = (ctypes.c_uint64 * 3)()
     call("crvpt_get_last_render_counters", v)
     return {"density_taps": int(v[0]), "supervoxel_reads": int(v[1]), "paths": int(v[2])}
+
+
+def build_nanovdb_from_dense(density_zyx, dx: float, origin=(0.0, 0.0, 0.0)) -> bytes:
+    """Host-side dense -> NanoVDB conversion (CloudData::getSparseDensityField semantics); no GPU needed."""
+    a = np.ascontiguousarray(np.asarray(density_zyx, dtype=np.float32))
+    nz, ny, nx = a.shape
+    o = dvec(origin, 3)
+    size = ctypes.c_uint64()
+    call("crvpt_build_nanovdb_from_dense", a.ctypes.data_as(ctypes.c_void_p), nx, ny, nz, float(dx), o, None, 0,
+         ctypes.byref(size))
+    out = np.empty(size.value, dtype=np.uint8)
+    call("crvpt_build_nanovdb_from_dense", a.ctypes.data_as(ctypes.c_void_p), nx, ny, nz, float(dx), o,
+         out.ctypes.data_as(ctypes.c_void_p), out.size, ctypes.byref(size))
+    return out.tobytes()
+
+
+def grid_lookup(ijk) -> np.ndarray:
+    """Device accessor values at integer coordinates of the current grid (see include/crvpt.h)."""
+    a = np.ascontiguousarray(np.asarray(ijk, dtype=np.int32).reshape(-1, 3))
+    out = np.empty(a.shape[0], np.float32)
+    call("crvpt_grid_lookup_batch", a.shape[0], a.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p))
+    return out
+
+
+def grid_lookup_world(xyz):
+    """Nearest device lookups at NanoVDB world coordinates of the current sparse grid: (values, voxels)."""
+    a = np.ascontiguousarray(np.asarray(xyz, dtype=np.float32).reshape(-1, 3))
+    out = np.empty(a.shape[0], np.float32)
+    ijk = np.empty((a.shape[0], 3), np.int32)
+    call("crvpt_grid_lookup_world_batch", a.shape[0], a.ctypes.data_as(ctypes.c_void_p),
+         out.ctypes.data_as(ctypes.c_void_p), ijk.ctypes.data_as(ctypes.c_void_p))
+    return out, ijk
+
+
+def get_supervoxel_tables():
+    """(edge, grid size, decomposition (min, max) pairs, RRT (mu_c, mu_r_bar) pairs) built on the device."""
+    sv = ctypes.c_int()
+    g = (ctypes.c_int * 3)()
+    call("crvpt_get_supervoxel_tables", ctypes.byref(sv), g, None, None, 0)
+    n = g[0] * g[1] * g[2]
+    dec = np.empty((n, 2), np.float32)
+    rrt = np.empty((n, 2), np.float32)
+    call("crvpt_get_supervoxel_tables", ctypes.byref(sv), g, dec.ctypes.data_as(ctypes.c_void_p),
+         rrt.ctypes.data_as(ctypes.c_void_p), 2 * n)
+    return sv.value, tuple(g), dec, rrt

@@ -56,6 +56,11 @@ int crvpt_set_nanovdb_buffer(const uint8_t* data, uint64_t size);
  * height profile). */
 int crvpt_generate_fbm_cloud(int nx, int ny, int nz, uint32_t seed, int octaves, float lacunarity, float gain,
                              float base_frequency, float edge0, float edge1, int height_falloff);
+/* No reference counterpart, host only (no GPU needed): the dense -> NanoVDB conversion of
+ * CloudData::getSparseDensityField (CloudData.cpp:1367-1438) on caller data: FP32 fog volume, background 0,
+ * voxel size dx, index (0,0,0) at world origin[3], grid name "tmp". With out == NULL only *size is written. */
+int crvpt_build_nanovdb_from_dense(const float* values, int nx, int ny, int nz, double dx, const double origin[3],
+                                   uint8_t* out, uint64_t capacity, uint64_t* size);
 /* Copies the current dense field (normalised or raw) or the current NanoVDB buffer to host memory. */
 int crvpt_get_density_field(float* out, uint64_t count, int normalized);
 int crvpt_get_nanovdb_buffer(uint8_t* out, uint64_t capacity, uint64_t* size);
@@ -124,6 +129,17 @@ int crvpt_set_denoiser_property(const char* key, const char* value);
 int crvpt_transmittance_batch(int estimator, int n_rays, const float* origins, const float* directions,
                               const float* t_max, int n_samples, uint32_t seed, float* mean, float* m2,
                               int on_device, void* stream);
+
+/* No reference counterpart: exactness probes of the device grid accessors on the current volume.
+ * grid_lookup_batch: voxel values at integer coordinates (NanoVDB index space for the sparse grid, normalised texels
+ * for the dense grid); grid_lookup_world_batch (sparse grid only): pnanovdb_grid_world_to_indexf + floor + lookup of
+ * NanoVDB world coordinates, also returning the selected voxels. Host arrays. */
+int crvpt_grid_lookup_batch(int n, const int32_t* ijk, float* out);
+int crvpt_grid_lookup_world_batch(int n, const float* xyz, float* out, int32_t* ijk_out);
+/* No reference counterpart: the supervoxel tables the device built for the residual-ratio and decomposition trackers
+ * (SuperVoxelGrid.cpp:40-405): edge, grid size, (min, max) and (mu_c, mu_r_bar) pairs (capacity in floats). */
+int crvpt_get_supervoxel_tables(int* supervoxel_size, int grid_size[3], float* decomposition_min_max,
+                                float* rrt_mu_c_mu_r, uint64_t capacity);
 
 /* No reference counterpart: performance controls of the gfx950 path pool. use_majorant_grid: local majorants
  * from the supervoxel grid (1, default) or the reference's global majorant (0); supervoxel_size: edge in voxels of

@@ -1,0 +1,127 @@
+"""GPU checks of the sparse (NanoVDB) path: device lookups bit-exact against the reference accessor's golden vectors,
+supervoxel tables bit-exact against the oracle's restatement of SuperVoxelGrid.cpp, and sparse renders and
+transmittance estimates statistically equal to the dense path and the FP64 optical depth."""
+import glob
+import math
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = sorted(glob.glob(os.path.join(REPO, "tests", "golden", "nanovdb_*.npz")))
+
+
+def _cpu(t):
+    return t.detach().cpu().numpy()
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p) for p in GOLDEN])
+def test_device_lookups_bit_exact_vs_reference(vpt, path):
+    print(f"case: device NanoVDB lookups {os.path.basename(path)}", flush=True)
+    g = np.load(path)
+    vpt.set_nanovdb_buffer(g["grid"].tobytes())
+    got = vpt.grid_lookup(g["ijk"])
+    np.testing.assert_array_equal(got.view(np.uint32), g["values"].view(np.uint32))
+    vals, ijk = vpt.grid_lookup_world(g["xyz"])
+    np.testing.assert_array_equal(ijk, g["world_ijk"])
+    np.testing.assert_array_equal(vals.view(np.uint32), g["world_values"].view(np.uint32))
+
+
+@pytest.mark.parametrize("interp", ["nearest", "stochastic"])
+@pytest.mark.parametrize("sparse", [False, True])
+def test_supervoxel_tables_bit_exact(vpt, oracle, interp, sparse):
+    print(f"case: supervoxel tables interp={interp} sparse={sparse}", flush=True)
+    d = oracle.fbm_cloud(44, 36, 28, seed=12)
+    vpt.set_density_field(d)
+    vpt.set_grid_interpolation_type(interp)
+    vpt.set_extinction_scale(300.0)
+    vpt.set_extinction_base([1.0, 1.0, 1.0])
+    vpt.set_use_sparse_grid(sparse)
+    sv, g, dec, rrt = vpt.get_supervoxel_tables()
+    vmin, vmax = float(d.min()), float(d.max())
+    norm = ((d - vmin) / (vmax - vmin)).astype(np.float32)
+    if sparse:
+        nz_, ny_, nx_ = np.nonzero(norm != 0)
+        crop = norm[nz_.min():nz_.max() + 1, ny_.min():ny_.max() + 1, nx_.min():nx_.max() + 1]
+        sc = oracle.Scene(crop, interpolation=interp, extinction=300.0, voxel_range=(0.0, 1.0))
+    else:
+        sc = oracle.Scene(d, interpolation=interp, extinction=300.0)
+    osv, og, orrt, odec, _ = sc.supervoxel_tables()
+    vpt.set_use_sparse_grid(False)
+    assert (sv, g) == (osv, og)
+    np.testing.assert_array_equal(dec.view(np.uint32), odec.view(np.uint32))
+    np.testing.assert_array_equal(rrt.view(np.uint32), orrt.view(np.uint32))
+
+
+def _stats(vpt, frames, res, batches=4):
+    import torch
+
+    means = []
+    for b in range(batches):
+        vpt.set_seed_offset(7717 * (b + 1))
+        img = vpt.render_frame(torch.zeros((3, res, res), dtype=torch.float32, device="cuda"), frames)
+        means.append(_cpu(img).reshape(3, -1).mean(1))
+    vpt.set_seed_offset(0)
+    m = np.array(means, np.float64)
+    return m.mean(0), m.std(0, ddof=1) / math.sqrt(batches)
+
+
+@pytest.mark.parametrize("mode", [0, 2, 5, 3, 4])
+def test_sparse_render_equals_dense(vpt, oracle, mode):
+    """TestVolumetricPathTracing.cpp:187-197 (dense vs sparse grid types) for each estimator."""
+    print(f"case: sparse vs dense render mode={mode}", flush=True)
+    d = oracle.fbm_cloud(32, 32, 32, seed=13)
+    vpt.set_density_field(d)
+    vpt.set_vpt_mode(mode)
+    vpt.set_grid_interpolation_type("nearest")
+    vpt.set_extinction_scale(200.0)
+    vpt.set_scattering_albedo([0.9, 0.9, 0.9])
+    vpt.set_phase_g(0.5)
+    vpt.set_use_builtin_environment_map("Default")
+    vpt.set_use_lights(False)
+    vpt.set_use_sparse_grid(False)
+    md, sd = _stats(vpt, 64, 32)
+    vpt.set_use_sparse_grid(True)
+    ms, ss = _stats(vpt, 64, 32)
+    vpt.set_use_sparse_grid(False)
+    tol = 3.0 * np.sqrt(sd ** 2 + ss ** 2) + 1e-5
+    assert np.all(np.abs(md - ms) <= tol), (md, ms, tol)
+
+
+@pytest.mark.parametrize("estimator", ["delta", "ratio", "residual_ratio", "decomposition"])
+@pytest.mark.parametrize("interp", ["nearest", "stochastic"])
+def test_sparse_transmittance_vs_fp64(vpt, oracle, estimator, interp):
+    print(f"case: sparse transmittance {estimator} {interp}", flush=True)
+    d = oracle.fbm_cloud(40, 40, 40, seed=14)
+    vpt.set_density_field(d)
+    vpt.set_grid_interpolation_type(interp)
+    vpt.set_extinction_scale(40.0)
+    vpt.set_extinction_base([1.0, 1.0, 1.0])
+    vpt.set_use_sparse_grid(True)
+    box = vpt.get_render_bounding_box()
+    bmin = np.array(box[0::2], np.float32)
+    bmax = np.array(box[1::2], np.float32)
+    rng = np.random.default_rng(15)
+    n_rays, n = 512, 2048
+    c = 0.5 * (bmin + bmax)
+    half = 0.5 * (bmax - bmin)
+    o = rng.normal(size=(n_rays, 3))
+    o = (c + o / np.linalg.norm(o, axis=1, keepdims=True) * 1.5 * np.linalg.norm(half)).astype(np.float32)
+    tgt = rng.uniform(bmin, bmax, (n_rays, 3))
+    w = (tgt - o) / np.linalg.norm(tgt - o, axis=1, keepdims=True)
+    w = w.astype(np.float32)
+    mean, m2 = vpt.transmittance_batch(estimator, o, w, n_samples=n, seed=9)
+    vpt.set_use_sparse_grid(False)
+    vmin, vmax = float(d.min()), float(d.max())
+    norm = ((d - vmin) / (vmax - vmin)).astype(np.float32)
+    nz_, ny_, nx_ = np.nonzero(norm != 0)
+    crop = norm[nz_.min():nz_.max() + 1, ny_.min():ny_.max() + 1, nx_.min():nx_.max() + 1]
+    sc = oracle.Scene(crop, interpolation=interp, extinction=40.0, voxel_range=(0.0, 1.0), box=(bmin, bmax))
+    exact = np.exp(-sc.optical_depth(o, w))
+    var = np.maximum(m2.astype(np.float64) - mean.astype(np.float64) ** 2, 1.0 / n)
+    z = (mean - exact) / np.sqrt(var / n)
+    assert np.max(np.abs(z)) < 5.5, (np.argmax(np.abs(z)), np.max(np.abs(z)))
+    assert abs(z.mean()) * math.sqrt(z.size) < 4.0
