@@ -1,14 +1,19 @@
 """Throughput benchmark of the gfx950 volumetric path tracer on its headline workload.
 
-Workload (one GPU): dense 128^3 FP32 seeded fBm cloud (5 octaves, seed 0, lacunarity 2, gain 0.5, base frequency
-4, smoothstep(0.45, 0.6)) filling the unit cube, extinction 1024, albedo 0.9, Henyey-Greenstein g = 0.85, constant
-white environment image plus one directional sun with next-event estimation, 256 x 256 pinhole camera at the
-reference module's default pose, 64 samples per pixel per step, unbounded path length (the reference trackers have
-no bounce limit). A step is one render_frame pass of 256 * 256 * 64 camera paths with the volume resident in HBM.
+Default workload "c2_sparse512" (one GPU): a NanoVDB FP32 sparse grid of a 512^3 domain holding a seeded fBm cloud
+(5 octaves, seed 0, lacunarity 2, gain 0.5, base frequency 4) thresholded by smoothstep(0.645, 0.795) so that about
+12 % of the voxels are active in 8^3 leaves, 16^3-voxel majorant supervoxels, extinction 1024, albedo 0.9, the
+reference's default sky, 1024 x 1024 pinhole camera at the reference module's default pose, 256 samples per pixel,
+at most 128 scattering events per path. Step k renders the image once with estimator
+and phase g taken from the 12 combinations (delta, ratio, residual ratio, decomposition) x g in {0, 0.5, 0.85}.
 
-Prints one JSON line: Mpaths/s, the roofline of the dominant kernel (algorithmic bytes from device work counters of
-an instrumented pass with identical seeds / HIP-event kernel time of the timed passes), and the CPU oracle timed
-on a bounded sample of the same scene on the host cores.
+Workload "c1_dense128": dense 128^3 FP32 fBm (smoothstep(0.45, 0.6)) filling the unit cube, next-event tracking with
+a constant white environment image and one directional sun, g = 0.85, 256 x 256, 64 spp per step, at most 64
+scattering events per path.
+
+Prints one JSON line: Mpaths/s over the timed steps, the roofline of the dominant kernel (algorithmic bytes from
+device work counters of an instrumented pass with identical seeds, divided by the HIP-event time of that kernel in
+the timed passes), and the CPU oracle timed on a bounded sample of the same scene on the host cores.
 """
 from __future__ import annotations
 
@@ -23,15 +28,20 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
-W = H = 256
-SPP = 64
-GRID = 128
+
+C2 = {"max_bounces": 128, "grid": 512, "edge0": 0.645, "edge1": 0.795, "width": 1024, "height": 1024, "spp": 256,
+      "modes": [(0, "delta tracking"), (2, "ratio tracking"), (4, "residual ratio tracking"),
+                (3, "decomposition tracking")], "gs": [0.0, 0.5, 0.85]}
+C1 = {"max_bounces": 64, "grid": 128, "width": 256, "height": 256, "spp": 64}
+MODE_KERNELS = {0: "k_pathPool<DELTA_TRACKING, NanoAccess>", 2: "k_pathPool<RATIO_TRACKING, NanoAccess>",
+                4: "k_wholePath<RESIDUAL_RATIO_TRACKING, NanoAccess>",
+                3: "k_wholePath<DECOMPOSITION_TRACKING, NanoAccess>"}
 
 
-def setup_scene(vpt, torch):
+def setup_c1(vpt, torch):
     vpt.set_max_grid_extent(0.5)
-    vpt.generate_fbm_cloud(GRID, GRID, GRID, seed=0, octaves=5, lacunarity=2.0, gain=0.5, base_frequency=4.0,
-                           edge0=0.45, edge1=0.6)
+    vpt.generate_fbm_cloud(C1["grid"], C1["grid"], C1["grid"], seed=0, octaves=5, lacunarity=2.0, gain=0.5,
+                           base_frequency=4.0, edge0=0.45, edge1=0.6)
     vpt.set_vpt_mode_from_name("Next Event Tracking")
     vpt.set_grid_interpolation_type(1)
     vpt.set_extinction_base([1.0, 1.0, 1.0])
@@ -50,24 +60,82 @@ def setup_scene(vpt, torch):
     vpt.set_light_property(0, "position", " ".join(f"{100.0 * v / n:.6f}" for v in d))
     vpt.set_light_property(0, "color", "1 0.961538462 0.884615385")
     vpt.set_light_property(0, "intensity", "2.6")
+    vpt.set_max_bounces(C1["max_bounces"])
 
 
-def cpu_baseline(budget_s: float = 12.0):
-    """The oracle (FP32 restatement of the reference shaders, hybrid-Taus RNG) on the same scene and camera, timed on
-    the host cores on a bounded sample: a 64 x 64 image of the same view at growing spp until the budget is used."""
+def setup_c2(vpt):
+    g = C2["grid"]
+    vpt.generate_fbm_cloud(g, g, g, seed=0, octaves=5, lacunarity=2.0, gain=0.5, base_frequency=4.0,
+                           edge0=C2["edge0"], edge1=C2["edge1"])
+    vpt.set_grid_interpolation_type(1)
+    vpt.set_extinction_base([1.0, 1.0, 1.0])
+    vpt.set_extinction_scale(1024.0)
+    vpt.set_scattering_albedo([0.9, 0.9, 0.9])
+    vpt.set_use_builtin_environment_map("Default")
+    vpt.set_use_lights(False)
+    vpt.set_use_sparse_grid(True)
+    vpt.set_launch_config(16, 0, 8)
+    vpt.set_max_bounces(C2["max_bounces"])
+
+
+def cpu_baseline_c1(budget_s: float = 12.0):
+    """The oracle (FP32 restatement of the reference shaders, hybrid-Taus RNG) on the C1 scene and camera."""
     import numpy as np
 
     from oracle import oracle as O
 
     threads = max(1, min(os.cpu_count() or 1, 16))
-    d = O.fbm_cloud(GRID, GRID, GRID, seed=0)
+    d = O.fbm_cloud(C1["grid"], C1["grid"], C1["grid"], seed=0)
     sun = np.array([0.5826, 0.7660, 0.2717])
     sun = 100.0 * sun / np.linalg.norm(sun)
     L = O.make_light(light_type=2, light_space=0, position=tuple(sun), intensity=2.6,
                      color=(1.0, 0.961538462, 0.884615385))
     res = 64
     sc = O.Scene(d, mode="next_event", interpolation="stochastic", extinction=1024.0, albedo=0.9, g=0.85,
-                 env_image=np.ones((3, 16, 32), np.float32), lights=[L], width=res, height=res, max_grid_extent=0.5)
+                 env_image=np.ones((3, 16, 32), np.float32), lights=[L], width=res, height=res, max_grid_extent=0.5,
+                 max_bounces=C1["max_bounces"])
+    return _time_oracle(sc, res, threads, budget_s, "C1 scene and camera")
+
+
+def cpu_baseline_c2(vpt, budget_s: float = 15.0):
+    """The oracle on the C2 volume (its dense equivalent: the oracle samples the same normalised voxels) and camera,
+    cycling through the same estimator / g combinations, on a bounded sample of 64 x 64 pixels."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    threads = max(1, min(os.cpu_count() or 1, 16))
+    g = C2["grid"]
+    d = vpt.get_density_field((g, g, g), normalized=True)
+    # The sparse grid spans the active voxels only: crop the dense field to them and use the same world box.
+    nzm = d != 0
+    zs, ys, xs = (np.nonzero(nzm.any(axis=ax))[0] for ax in ((1, 2), (0, 2), (0, 1)))
+    d = np.ascontiguousarray(d[zs[0]:zs[-1] + 1, ys[0]:ys[-1] + 1, xs[0]:xs[-1] + 1])
+    del nzm
+    box = vpt.get_render_bounding_box()
+    box = (np.array(box[0::2], np.float32), np.array(box[1::2], np.float32))
+    res = 64
+    names = {0: "delta", 2: "ratio", 4: "residual_ratio", 3: "decomposition"}
+    paths, elapsed, frames = 0, 0.0, 0
+    combos = [(m, gg) for m, _ in C2["modes"] for gg in C2["gs"]]
+    scenes = {}
+    while elapsed < budget_s and frames < 4 * len(combos):
+        m, gg = combos[frames % len(combos)]
+        if (m, gg) not in scenes:
+            scenes[(m, gg)] = O.Scene(d, mode=names[m], interpolation="stochastic", extinction=1024.0, albedo=0.9,
+                                      g=gg, width=res, height=res, voxel_range=(0.0, 1.0), box=box,
+                                      super_voxel_size=16, max_bounces=C2["max_bounces"])
+        t0 = time.perf_counter()
+        scenes[(m, gg)].render(1, frame_start=frames, threads=threads)
+        elapsed += time.perf_counter() - t0
+        paths += res * res
+        frames += 1
+    return {"value": paths / elapsed / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
+            "sample": f"{res}x{res} px x {frames} spp ({paths} paths) of the C2 volume and camera, estimator/g "
+                      f"cycling like the timed steps, {elapsed:.1f} s on {threads} threads"}
+
+
+def _time_oracle(sc, res, threads, budget_s, what):
     paths, elapsed, frames = 0, 0.0, 0
     acc = None
     while elapsed < budget_s and frames < 64:
@@ -77,12 +145,12 @@ def cpu_baseline(budget_s: float = 12.0):
         paths += res * res
         frames += 1
     return {"value": paths / elapsed / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
-            "sample": f"{res}x{res} px x {frames} spp ({paths} paths) of the same scene and camera, "
-                      f"{elapsed:.1f} s on {threads} threads"}
+            "sample": f"{res}x{res} px x {frames} spp ({paths} paths) of the {what}, {elapsed:.1f} s on "
+                      f"{threads} threads"}
 
 
 def load_traffic(workload: str):
-    """HBM bytes per launch of the path kernel from the committed rocprofv3 PMC summary, when one exists."""
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 PMC summary, when one exists."""
     path = os.path.join(REPO, "profiles", f"pmc_{workload}.json")
     if not os.path.exists(path):
         return None
@@ -91,12 +159,26 @@ def load_traffic(workload: str):
     return pmc.get("traffic_bytes_per_launch")
 
 
+def copy_calibration(torch):
+    """A plain device copy of 256 MiB (FETCH_SIZE / WRITE_SIZE scale factor for the PMC summaries)."""
+    n = 256 * 1024 * 1024 // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": 4 * n}), flush=True)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=12)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["c2_sparse512", "c1_dense128"], default="c2_sparse512")
+    ap.add_argument("--spp", type=int, default=0, help="samples per pixel per step (0: the workload's)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path tracer is single-GPU here",
@@ -105,39 +187,94 @@ def main() -> int:
 
     import torch
 
+    if args.copy_calibration:
+        copy_calibration(torch)
+        return 0
+
     from cloudrendering_amd import vpt
 
     vpt.initialize()
-    setup_scene(vpt, torch)
     dev = torch.device("cuda:0")
+    c2 = args.workload == "c2_sparse512"
+    if c2:
+        setup_c2(vpt)
+        W, H, spp = C2["width"], C2["height"], args.spp or C2["spp"]
+        combos = [(m, g) for m, _ in C2["modes"] for g in C2["gs"]]
+    else:
+        setup_c1(vpt, torch)
+        W, H, spp = C1["width"], C1["height"], args.spp or C1["spp"]
+        combos = [(5, 0.85)]
     target = torch.zeros((3, H, W), dtype=torch.float32, device=dev)
 
-    # Instrumented pass (untimed): algorithmic work counters of one step.
-    vpt.set_collect_stats(True)
-    vpt.render_frame(target, SPP)
-    torch.cuda.synchronize()
-    counters = vpt.get_last_render_counters()
-    vpt.set_collect_stats(False)
+    def apply(k):
+        m, g = combos[k % len(combos)]
+        vpt.set_vpt_mode(m)
+        vpt.set_phase_g(g)
+        return m
 
-    for _ in range(args.warmup):
-        vpt.render_frame(target, SPP)
+    for k in range(args.warmup):
+        apply(k)
+        vpt.render_frame(target, spp)
     torch.cuda.synchronize()
-    kernel_ms = []
+    per_combo_ms = {}
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        vpt.render_frame(target, SPP)
-        kernel_ms.append(vpt.get_last_render_stats()["path_kernel_ms"])
+    for k in range(args.steps):
+        apply(k)
+        vpt.render_frame(target, spp)
+        st = vpt.get_last_render_stats()
+        per_combo_ms.setdefault(combos[k % len(combos)], []).append(st["path_kernel_ms"])
+        print(f"step {k}: {combos[k % len(combos)]} path kernel {st['path_kernel_ms']:.1f} ms", file=sys.stderr,
+              flush=True)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1e3 / args.steps
-    paths_per_step = W * H * SPP
+    paths_per_step = W * H * spp
     value = paths_per_step / (wall / args.steps) / 1e6
 
-    # Algorithmic bytes per launch of the path kernel: FP32 density taps, 4-byte majorant reads, 16-byte path record.
-    b_alg = 4 * counters["density_taps"] + 4 * counters["supervoxel_reads"] + 16 * counters["paths"]
-    k_ms = sum(kernel_ms) / len(kernel_ms)
-    achieved = b_alg / (k_ms * 1e-3) / 1e9
-    workload = "c1_dense128_nee_256x256_64spp"
+    # Dominant kernel: the estimator whose path kernel takes the most time over the timed steps.
+    by_mode = {}
+    for (m, g), ms in per_combo_ms.items():
+        by_mode.setdefault(m, []).extend(ms)
+    dom = max(by_mode, key=lambda m: sum(by_mode[m]))
+    dom_ms = sum(by_mode[dom]) / len(by_mode[dom])
+    # Instrumented passes (untimed) of the dominant estimator's combinations: device work counters over the same
+    # image and sample count (the counters add wave-reduced atomics, so these passes are not timed).
+    taps = svr = paths = desc = 0
+    n_c = 0
+    vpt.set_collect_stats(True)
+    for k in range(len(combos)):
+        if combos[k][0] != dom or combos[k] not in per_combo_ms:
+            continue
+        apply(k)
+        vpt.render_frame(target, spp)
+        torch.cuda.synchronize()
+        c = vpt.get_last_render_counters()
+        taps += c["density_taps"]
+        svr += c["supervoxel_reads"]
+        paths += c["paths"]
+        desc += c["tree_descents"]
+        n_c += 1
+    vpt.set_collect_stats(False)
+    taps, svr, paths, desc = (v / max(n_c, 1) for v in (taps, svr, paths, desc))
+    # Algorithmic bytes per launch: 4-byte FP32 density taps, 4-byte majorant / 8-byte (control, residual) table
+    # reads, 40 bytes per sparse-tree descent (root tile key and child 8 + 8, upper and lower child-mask words 4 + 4,
+    # upper and lower table entries 8 + 8), and the 16-byte radiance record of each path.
+    b_alg = 4 * taps + (4 if dom in (0, 2, 5) else 8) * svr + 40 * desc + 16 * paths
+    achieved = b_alg / (dom_ms * 1e-3) / 1e9
+    if c2:
+        kernel = MODE_KERNELS[dom]
+        config = {"workload": args.workload, "volume": "NanoVDB fp32 512^3 fBm seed 0, ~12% active, 8^3 leaves",
+                  "image": f"{W}x{H}", "spp_per_step": spp,
+                  "estimators": [n for _, n in C2["modes"]], "phase_g": C2["gs"], "albedo": 0.9,
+                  "extinction": 1024.0, "environment": "default sky", "max_bounces": C2["max_bounces"],
+                  "majorant": "16^3 supervoxel grid", "steps_cycle": "estimator x g (12 combinations)"}
+    else:
+        kernel = "k_pathPool<NEXT_EVENT_TRACKING, DenseAccess>"
+        config = {"workload": args.workload, "volume": "dense 128^3 fp32 fBm seed 0", "image": f"{W}x{H}",
+                  "spp_per_step": spp, "estimator": "next event tracking", "phase_g": 0.85, "albedo": 0.9,
+                  "extinction": 1024.0, "environment": "constant white image + directional sun",
+                  "max_bounces": C1["max_bounces"], "majorant": "8^3 supervoxel grid"}
+    per_combo = {f"{m}/{g}": round(sum(v) / len(v), 3) for (m, g), v in per_combo_ms.items()}
     line = {
         "metric": "Mpaths/s on 1 MI355X",
         "value": round(value, 3),
@@ -151,19 +288,17 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded fBm procedural cloud generated on the GPU)",
-        "config": {"workload": workload, "volume": "dense 128^3 fp32 fBm seed 0", "image": f"{W}x{H}",
-                   "spp_per_step": SPP, "estimator": "next event tracking", "phase_g": 0.85, "albedo": 0.9,
-                   "extinction": 1024.0, "environment": "constant white image + directional sun",
-                   "max_bounces": "unbounded", "majorant": "8^3 supervoxel grid"},
+        "config": config,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": load_traffic(workload),
-                     "kernel": "k_pathPool<NEXT_EVENT_TRACKING, DenseAccess>", "kernel_ms": round(k_ms, 3),
-                     "algorithmic_bytes": b_alg, "density_taps": counters["density_taps"],
-                     "supervoxel_reads": counters["supervoxel_reads"], "paths": counters["paths"]},
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": load_traffic(args.workload),
+                     "kernel": kernel, "kernel_ms": round(dom_ms, 3), "algorithmic_bytes": int(b_alg),
+                     "density_taps": int(taps), "supervoxel_reads": int(svr), "tree_descents": int(desc),
+                     "paths": int(paths)},
+        "kernel_ms_per_mode_g": per_combo,
     }
-    vpt.cleanup()
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline()
+        line["cpu_baseline"] = cpu_baseline_c2(vpt) if c2 else cpu_baseline_c1()
+    vpt.cleanup()
     print(json.dumps(line), flush=True)
     return 0
 

@@ -169,6 +169,7 @@ struct Renderer {
     double envFactorRgb[3] = {1.0, 1.0, 1.0};
     int useLinearRGB = 1;
     uint32_t seedOffset = 0;
+    uint32_t maxBounces = 0;  // 0: unbounded like the reference trackers
     bool useLights = false;
     std::vector<Light> lights;
     // camera (VolumetricPathTracingModuleRenderer.cpp:124-131)
@@ -558,6 +559,7 @@ void fillSceneParams(Renderer& r, int w, int h, SceneParams& P, SupervoxelGrid& 
     for (int i = 0; i < P.numLights; i++) P.lights[i] = r.lights[i];
     P.sparse = sparse ? 1 : 0;
     P.seedOffset = r.seedOffset;
+    P.maxBounces = r.maxBounces;
     P.width = w;
     P.height = h;
     S.majorant = r.dMajorant.as<float>();
@@ -795,16 +797,16 @@ int crvpt_set_collect_stats(int collect) {
     return CRVPT_OK;
 }
 
-int crvpt_get_last_render_counters(uint64_t out[3]) {
+int crvpt_get_last_render_counters(uint64_t out[4]) {
     if (int rc = requireInit()) return rc;
     Renderer& r = *g_r;
     if (!r.collectStats || !r.dStats.ptr) return fail(CRVPT_ERR_INVALID_ARGUMENT, "get_last_render_counters: "
                                                       "enable crvpt_set_collect_stats before rendering");
-    unsigned long long v[3];
+    unsigned long long v[4];
     HIP_TRY(hipStreamSynchronize(r.stream));
     HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMemcpy(v, r.dStats.ptr, sizeof(v), hipMemcpyDeviceToHost));
-    for (int i = 0; i < 3; i++) out[i] = v[i];
+    for (int i = 0; i < 4; i++) out[i] = v[i];
     return CRVPT_OK;
 }
 
@@ -1007,6 +1009,14 @@ int crvpt_set_phase_g(double g) {
 int crvpt_set_seed_offset(int64_t offset) {
     if (int rc = requireInit()) return rc;
     g_r->seedOffset = uint32_t(offset);
+    g_r->resetAccumulation();
+    return CRVPT_OK;
+}
+
+int crvpt_set_max_bounces(int32_t max_bounces) {
+    if (int rc = requireInit()) return rc;
+    if (max_bounces < 0) return fail(CRVPT_ERR_INVALID_ARGUMENT, "set_max_bounces: negative bounce limit");
+    g_r->maxBounces = uint32_t(max_bounces);
     g_r->resetAccumulation();
     return CRVPT_OK;
 }

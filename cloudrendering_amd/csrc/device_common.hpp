@@ -272,8 +272,14 @@ CRVPT_DEV V3 lightRadiance(const SceneParams& P, V3 pos, int lightIdx) {
 // Grid lookups. Both grids return the normalised density the reference's sampleCloud returns; STOCHASTIC adds a
 // uniform sub-voxel jitter (three draws) before the nearest-voxel read (VptUtils.glsl:515-591, 674-684).
 // ------------------------------------------------------------------------------------------------------------
+// Dense grids need no lookup memo: a voxel is one load.
+struct NoCache {
+    CRVPT_DEV uint32_t descents() const { return 0u; }
+};
+
 struct DenseAccess {
     DenseGrid g;
+    using Cache = NoCache;
     CRVPT_DEV float voxel(int i, int j, int k) const {
         if ((unsigned)i >= (unsigned)g.dim[0] || (unsigned)j >= (unsigned)g.dim[1] ||
             (unsigned)k >= (unsigned)g.dim[2])
@@ -281,6 +287,7 @@ struct DenseAccess {
         return g.values[(size_t(k) * g.dim[1] + j) * g.dim[0] + i];
     }
     CRVPT_DEV float voxelAt(int i, int j, int k) const { return voxel(i, j, k); }
+    CRVPT_DEV float voxelAt(int i, int j, int k, NoCache&) const { return voxel(i, j, k); }
     // u: texel coordinates (grid dim * normalised texture coordinate); (jx, jy, jz) in (-0.5, 0.5) is the
     // stochastic sub-voxel jitter, ignored by the other filters.
     CRVPT_DEV float lookupTex(const SceneParams& P, V3 u, float jx, float jy, float jz) const {
@@ -305,6 +312,9 @@ struct DenseAccess {
     CRVPT_DEV float lookupWorldJ(const SceneParams& P, V3 pos, float jx, float jy, float jz) const {
         return lookupTex(P, toTex(P, pos), jx, jy, jz);
     }
+    CRVPT_DEV float lookupWorldJ(const SceneParams& P, V3 pos, float jx, float jy, float jz, NoCache&) const {
+        return lookupTex(P, toTex(P, pos), jx, jy, jz);
+    }
     CRVPT_DEV float lookupWorld(const SceneParams& P, Rng& rng, V3 pos) const {
         float jx = 0.0f, jy = 0.0f, jz = 0.0f;
         if (P.interpolation == INTERP_STOCHASTIC) {
@@ -313,6 +323,9 @@ struct DenseAccess {
             jz = rng.uniform() - 0.5f;
         }
         return lookupTex(P, toTex(P, pos), jx, jy, jz);
+    }
+    CRVPT_DEV float lookupWorld(const SceneParams& P, Rng& rng, V3 pos, NoCache&) const {
+        return lookupWorld(P, rng, pos);
     }
 };
 

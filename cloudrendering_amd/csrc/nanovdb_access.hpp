@@ -16,8 +16,21 @@
 
 namespace crvpt {
 
+// Per-lane memo of the last 8^3 block a lookup descended to: the block coordinate (index >> 3), and either the
+// leaf's byte offset or, when the block is covered by a tile (no leaf), the tile value. A path's consecutive
+// lookups are about a voxel apart, so most of them hit the memo and skip the root-upper-lower descent (six dependent
+// loads); a hit returns the same value the descent would.
+struct LeafCache {
+    int bi = 0x7FFFFFFF, bj = 0, bk = 0;
+    uint32_t leaf = 0;  // 0: the block is a tile of value `tile`
+    float tile = 0.0f;
+    uint32_t misses = 0;  // descents (work counter)
+    CRVPT_DEV uint32_t descents() const { return misses; }
+};
+
 struct NanoAccess {
     NanoGrid g;
+    using Cache = LeafCache;
 
     CRVPT_DEV uint32_t u32(uint32_t off) const { return *reinterpret_cast<const uint32_t*>(g.buf + off); }
     CRVPT_DEV float f32(uint32_t off) const { return *reinterpret_cast<const float*>(g.buf + off); }
@@ -26,34 +39,67 @@ struct NanoAccess {
         return uint64_t(v.x) | (uint64_t(v.y) << 32);
     }
 
-    // Root tile lookup: the tile count is 1 for every domain that fits one 4096^3 tile, so the scan is short.
-    CRVPT_DEV float value(int i, int j, int k) const {
+    // Descends to the 8^3 block holding (i, j, k): returns the leaf offset, or 0 with the tile value in `tile`.
+    CRVPT_DEV uint32_t locateBlock(int i, int j, int k, float& tile) const {
         const uint64_t key = uint64_t(uint32_t(k) >> 12) | (uint64_t(uint32_t(j) >> 12) << 21) |
                              (uint64_t(uint32_t(i) >> 12) << 42);
-        uint32_t tile = g.rootOffset + 64u;
+        uint32_t rt = g.rootOffset + 64u;
         uint32_t t = 0;
-        for (; t < g.tileCount; t++, tile += 32u) {
-            if (u64(tile) == key) break;
+        for (; t < g.tileCount; t++, rt += 32u) {
+            if (u64(rt) == key) break;
         }
-        if (t == g.tileCount) return g.background;
-        const uint64_t child = u64(tile + 8u);
-        if (child == 0ull) return f32(tile + 20u);
+        if (t == g.tileCount) {
+            tile = g.background;
+            return 0u;
+        }
+        const uint64_t child = u64(rt + 8u);
+        if (child == 0ull) {
+            tile = f32(rt + 20u);
+            return 0u;
+        }
         const uint32_t upper = g.rootOffset + uint32_t(child);
         const uint32_t nu = ((uint32_t(i) & 4095u) >> 7) << 10 | ((uint32_t(j) & 4095u) >> 7) << 5 |
                             ((uint32_t(k) & 4095u) >> 7);
-        if (((u32(upper + 4128u + 4u * (nu >> 5)) >> (nu & 31u)) & 1u) == 0u) return f32(upper + 8256u + 8u * nu);
+        if (((u32(upper + 4128u + 4u * (nu >> 5)) >> (nu & 31u)) & 1u) == 0u) {
+            tile = f32(upper + 8256u + 8u * nu);
+            return 0u;
+        }
         const uint32_t lower = upper + uint32_t(u64(upper + 8256u + 8u * nu));
         const uint32_t nl = ((uint32_t(i) & 127u) >> 3) << 8 | ((uint32_t(j) & 127u) >> 3) << 4 |
                             ((uint32_t(k) & 127u) >> 3);
-        if (((u32(lower + 544u + 4u * (nl >> 5)) >> (nl & 31u)) & 1u) == 0u) return f32(lower + 1088u + 8u * nl);
-        const uint32_t leaf = lower + uint32_t(u64(lower + 1088u + 8u * nl));
+        if (((u32(lower + 544u + 4u * (nl >> 5)) >> (nl & 31u)) & 1u) == 0u) {
+            tile = f32(lower + 1088u + 8u * nl);
+            return 0u;
+        }
+        return lower + uint32_t(u64(lower + 1088u + 8u * nl));
+    }
+
+    CRVPT_DEV float value(int i, int j, int k, LeafCache& c) const {
+        const int bi = i >> 3, bj = j >> 3, bk = k >> 3;  // arithmetic shifts: blocks of negative indices too
+        if (bi != c.bi || bj != c.bj || bk != c.bk) {
+            c.leaf = locateBlock(i, j, k, c.tile);
+            c.misses++;
+            c.bi = bi;
+            c.bj = bj;
+            c.bk = bk;
+        }
+        if (c.leaf == 0u) return c.tile;
         const uint32_t n = (uint32_t(i) & 7u) << 6 | (uint32_t(j) & 7u) << 3 | (uint32_t(k) & 7u);
-        return f32(leaf + 96u + 4u * n);
+        return f32(c.leaf + 96u + 4u * n);
+    }
+
+    // Root tile scan (one tile for every domain inside one 4096^3 tile), then upper, lower and leaf.
+    CRVPT_DEV float value(int i, int j, int k) const {
+        LeafCache c;
+        return value(i, j, k, c);
     }
 
     // Voxel (i, j, k) of the dense-equivalent texel grid: index coordinate = index bbox min + (i, j, k).
     CRVPT_DEV float voxelAt(int i, int j, int k) const {
         return value(i + g.indexMin[0], j + g.indexMin[1], k + g.indexMin[2]);
+    }
+    CRVPT_DEV float voxelAt(int i, int j, int k, LeafCache& c) const {
+        return value(i + g.indexMin[0], j + g.indexMin[1], k + g.indexMin[2], c);
     }
 
     // pnanovdb_grid_world_to_indexf in FP32 with the reference's operation order (PNanoVDB.glsl:2008-2018).
@@ -66,7 +112,7 @@ struct NanoAccess {
 
     // sampleCloud for NanoVDB (VptUtils.glsl:515-580, 674-684): raw values, no normalisation; (jx, jy, jz) in
     // (-0.5, 0.5) is the stochastic sub-voxel jitter, ignored by the other filters.
-    CRVPT_DEV float lookupWorldJ(const SceneParams& P, V3 pos, float jx, float jy, float jz) const {
+    CRVPT_DEV float lookupWorldJ(const SceneParams& P, V3 pos, float jx, float jy, float jz, LeafCache& cache) const {
         V3 coord = (pos - v3(P.boxMin)) / (v3(P.boxMax) - v3(P.boxMin));
         coord = coord * (v3(P.gridMax) - v3(P.gridMin)) + v3(P.gridMin);
         V3 idx = worldToIndex(coord);
@@ -75,14 +121,28 @@ struct NanoAccess {
             const float fx = floorf(idx.x), fy = floorf(idx.y), fz = floorf(idx.z);
             const int i = int(fx), j = int(fy), k = int(fz);
             const float a = idx.x - fx, b = idx.y - fy, c = idx.z - fz;
-            const float c00 = lerpf(value(i, j, k), value(i + 1, j, k), a);
-            const float c10 = lerpf(value(i, j + 1, k), value(i + 1, j + 1, k), a);
-            const float c01 = lerpf(value(i, j, k + 1), value(i + 1, j, k + 1), a);
-            const float c11 = lerpf(value(i, j + 1, k + 1), value(i + 1, j + 1, k + 1), a);
+            LeafCache& m = cache;
+            const float c00 = lerpf(value(i, j, k, m), value(i + 1, j, k, m), a);
+            const float c10 = lerpf(value(i, j + 1, k, m), value(i + 1, j + 1, k, m), a);
+            const float c01 = lerpf(value(i, j, k + 1, m), value(i + 1, j, k + 1, m), a);
+            const float c11 = lerpf(value(i, j + 1, k + 1, m), value(i + 1, j + 1, k + 1, m), a);
             return lerpf(lerpf(c00, c10, b), lerpf(c01, c11, b), c);
         }
         if (P.interpolation == INTERP_STOCHASTIC) idx = idx + v3(jx, jy, jz);
-        return value(int(floorf(idx.x)), int(floorf(idx.y)), int(floorf(idx.z)));
+        return value(int(floorf(idx.x)), int(floorf(idx.y)), int(floorf(idx.z)), cache);
+    }
+    CRVPT_DEV float lookupWorldJ(const SceneParams& P, V3 pos, float jx, float jy, float jz) const {
+        LeafCache c;
+        return lookupWorldJ(P, pos, jx, jy, jz, c);
+    }
+    CRVPT_DEV float lookupWorld(const SceneParams& P, Rng& rng, V3 pos, LeafCache& cache) const {
+        float jx = 0.0f, jy = 0.0f, jz = 0.0f;
+        if (P.interpolation == INTERP_STOCHASTIC) {
+            jx = rng.uniform() - 0.5f;
+            jy = rng.uniform() - 0.5f;
+            jz = rng.uniform() - 0.5f;
+        }
+        return lookupWorldJ(P, pos, jx, jy, jz, cache);
     }
     CRVPT_DEV float lookupWorld(const SceneParams& P, Rng& rng, V3 pos) const {
         float jx = 0.0f, jy = 0.0f, jz = 0.0f;

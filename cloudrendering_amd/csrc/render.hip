@@ -148,6 +148,10 @@ template <class Grid>
 CRVPT_DEV float density(const Grid& g, const SceneParams& P, Rng& rng, V3 p) {
     return g.lookupWorld(P, rng, p);
 }
+template <class Grid>
+CRVPT_DEV float density(const Grid& g, const SceneParams& P, Rng& rng, V3 p, typename Grid::Cache& gc) {
+    return g.lookupWorld(P, rng, p, gc);
+}
 
 // ------------------------------------------------------------------------------------------------------------
 // Path pool for the delta (DeltaTracking.glsl:167-345), analog ratio (RatioTracking.glsl:25-96) and next-event
@@ -162,6 +166,7 @@ constexpr uint32_t kEventSpace = 0x80000000u;
 constexpr uint32_t kCameraBlock = 0xFFFFFFFFu;
 
 enum : uint32_t { PF_ALIVE = 1u, PF_EXHAUSTED = 2u, PF_HAS_VALUE = 4u, PF_SHADOW = 8u };
+constexpr uint32_t kBounceShift = 8;  // flags bits 8..31 count the scattering events of the main path
 
 struct PathState {
     uint32_t unit;     // launch-local path index
@@ -246,6 +251,7 @@ template <int MODE>
 CRVPT_DEV bool scatterEvent(PathState& st, Walker& W, const SceneParams& P, const SupervoxelGrid& S, V3 p,
                             float pAbsOverReal, float4* out) {
     const uint4 e = pathBlock(P, st, kEventSpace | st.events++);
+    st.flags += 1u << kBounceShift;
     const V3 nw = phaseDirection(P.phaseG, W.w, u01(e.x), u01(e.y));
     if (MODE == VPT_RATIO_TRACKING) st.thr *= 1.0f - pAbsOverReal;
     st.flags |= PF_HAS_VALUE;
@@ -314,13 +320,13 @@ CRVPT_DEV bool endEvent(PathState& st, Walker& W, const SceneParams& P, const Su
 // One walker step of a live path. Returns false when the path finished (its radiance is written).
 template <int MODE, class Grid>
 CRVPT_DEV bool stepPath(PathState& st, Walker& W, const SceneParams& P, const SupervoxelGrid& S, const Grid& grid,
-                        float4* out) {
+                        typename Grid::Cache& gc, float4* out) {
     const uint4 r = pathBlock(P, st, st.step++);
     const int ev = walkAdvance(W, P, S, -__logf(uOpen(r.x)));
     if (ev == WALK_MOVED) return true;
     if (ev == WALK_END) return endEvent<MODE>(st, W, P, S, out);
     const V3 p = W.o + W.w * W.t;
-    const float rho = grid.lookupWorldJ(P, p, jitterHi(r.y), jitterLo(r.y), jitterHi(r.z));
+    const float rho = grid.lookupWorldJ(P, p, jitterHi(r.y), jitterLo(r.y), jitterHi(r.z), gc);
     st.taps++;
     const float pReal = P.extinction * rho * rcpf(W.mu);
     if (MODE == VPT_NEXT_EVENT_TRACKING && (st.flags & PF_SHADOW)) {
@@ -332,26 +338,32 @@ CRVPT_DEV bool stepPath(PathState& st, Walker& W, const SceneParams& P, const Su
     const float xi = u01(r.w);
     if (!(xi < pReal)) return true;  // null collision
     const float pAbs = (1.0f - P.albedo) * pReal;
-    if (MODE != VPT_RATIO_TRACKING && xi < pAbs) {  // absorption: delta returns 0, next-event its sum
+    const bool bounceLimit = P.maxBounces != 0u && (st.flags >> kBounceShift) >= P.maxBounces;
+    if ((MODE != VPT_RATIO_TRACKING && xi < pAbs) || bounceLimit) {
+        // absorption or a scattering event beyond the bounce limit: delta / ratio end with 0, next-event with its sum
         writePath(out, st.unit, MODE == VPT_NEXT_EVENT_TRACKING ? st.color : v3(0.0f), true);
         return false;
     }
     return scatterEvent<MODE>(st, W, P, S, p, pAbs / pReal, out);
 }
 
-// Wave-reduced 64-bit work counters: [0] density taps, [1] supervoxel majorant reads, [2] paths started.
-CRVPT_DEV void flushCounters(unsigned long long* stats, uint32_t taps, uint32_t svs, uint32_t paths) {
-    unsigned long long a = taps, b = svs, c = paths;
+// Wave-reduced 64-bit work counters: [0] density taps, [1] supervoxel majorant / table reads, [2] paths started,
+// [3] sparse-tree descents (lookup memo misses).
+CRVPT_DEV void flushCounters(unsigned long long* stats, uint32_t taps, uint32_t svs, uint32_t paths,
+                             uint32_t descents) {
+    unsigned long long a = taps, b = svs, c = paths, d = descents;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         a += __shfl_down(a, off);
         b += __shfl_down(b, off);
         c += __shfl_down(c, off);
+        d += __shfl_down(d, off);
     }
     if (__lane_id() == 0) {
         atomicAdd(&stats[0], a);
         atomicAdd(&stats[1], b);
         atomicAdd(&stats[2], c);
+        atomicAdd(&stats[3], d);
     }
 }
 
@@ -368,6 +380,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CRVPT_PATHP
     uint32_t paths = 0;
     Walker W;
     W.svReads = 0;
+    typename Grid::Cache gc;
     while (true) {
         const bool needs = !(st.flags & PF_ALIVE) && !exhausted;
         const uint64_t needMask = __ballot(needs);
@@ -393,11 +406,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CRVPT_PATHP
         }
         for (int k = 0; k < stepsPerRefill; k++) {
             if (st.flags & PF_ALIVE) {
-                if (!stepPath<MODE, Grid>(st, W, P, S, grid, out)) st.flags = 0;
+                if (!stepPath<MODE, Grid>(st, W, P, S, grid, gc, out)) st.flags = 0;
             }
         }
     }
-    if (stats) flushCounters(stats, st.taps, W.svReads, paths);
+    if (stats) flushCounters(stats, st.taps, W.svReads, paths, gc.descents());
 }
 
 // ------------------------------------------------------------------------------------------------------------
@@ -405,6 +418,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CRVPT_PATHP
 // (ResidualRatioTracking.glsl:33-224) on the reference's supervoxel tables. One lane runs a whole path; lanes
 // claim paths from the same per-wave ballot queue.
 // ------------------------------------------------------------------------------------------------------------
+// Per-lane work counters of the whole-path and transmittance estimators (density taps, supervoxel table reads).
+struct WorkCount {
+    uint32_t taps = 0;
+    uint32_t svReads = 0;
+};
+
 CRVPT_DEV float signf(float v) { return v > 0.0f ? 1.0f : (v < 0.0f ? -1.0f : 0.0f); }
 CRVPT_DEV float fractf(float v) { return v - floorf(v); }
 
@@ -421,7 +440,8 @@ CRVPT_DEV V3 gridDims(const SceneParams& P) {
 
 template <class Grid>
 CRVPT_DEV V3 decompositionPath(const SceneParams& P, const SupervoxelGrid& S, const Grid& grid, Rng& rng, V3 x,
-                               V3 w) {
+                               V3 w, WorkCount& wc, typename Grid::Cache& gc) {
+    uint32_t bounces = 0;
     const V3 eps = v3(1e-6f);
     float tMinVal, tMaxVal;
     if (rayBoxIntersect(v3(P.boxMin) + eps, v3(P.boxMax) - eps, x, w, tMinVal, tMaxVal)) {
@@ -442,6 +462,7 @@ CRVPT_DEV V3 decompositionPath(const SceneParams& P, const SupervoxelGrid& S, co
             x += w * a;
             const size_t si = refIndex(S, ix, iy, iz);
             const float mmMin = S.decomp[2 * si], mmMax = S.decomp[2 * si + 1];
+            wc.svReads++;
             if (mmMax < 1e-5f) {
                 x += w * dMax;
             } else {
@@ -460,12 +481,14 @@ CRVPT_DEV V3 decompositionPath(const SceneParams& P, const SupervoxelGrid& S, co
                     if (tc <= tr) {
                         collide = true;
                     } else {
-                        const float rho = density(grid, P, rng, xs);
+                        const float rho = density(grid, P, rng, xs, gc);
+                        wc.taps++;
                         collide = rng.uniform() * muR < P.extinction * rho - muC;
                     }
                     if (collide) {
                         x = xs;
                         if (rng.uniform() < absorptionAlbedo) return v3(0.0f);
+                        if (P.maxBounces != 0u && ++bounces > P.maxBounces) return v3(0.0f);
                         w = samplePhase(rng, P.phaseG, w);
                         tr = 0.0f;
                         tc = rng.exponential() / muC;
@@ -529,7 +552,8 @@ CRVPT_DEV void refSvBounds(const SceneParams& P, const SupervoxelGrid& S, const 
 
 template <class Grid>
 CRVPT_DEV float rrtEstimator(const SceneParams& P, const Grid& grid, Rng& rng, V3 x, V3 w, float dStart, float dEnd,
-                             float T, float& wSum, float& resT, float& resDist, float muC, float muR) {
+                             float T, float& wSum, float& resT, float& resDist, float muC, float muR,
+                             WorkCount& wc, typename Grid::Cache& gc) {
     const float Tc = __expf(-muC * (dEnd - dStart));
     float Tr = 1.0f;
     float dist = dStart;
@@ -538,7 +562,8 @@ CRVPT_DEV float rrtEstimator(const SceneParams& P, const Grid& grid, Rng& rng, V
         x += w * t;
         dist += t;
         if (dist >= dEnd) break;
-        const float rho = density(grid, P, rng, x);
+        const float rho = density(grid, P, rng, x, gc);
+        wc.taps++;
         Tr *= 1.0f - (P.extinction * rho - muC) / muR;
         const float Tlocal = T * Tr * __expf(-muC * (dist - dStart));
         wSum += Tlocal;
@@ -552,7 +577,7 @@ CRVPT_DEV float rrtEstimator(const SceneParams& P, const Grid& grid, Rng& rng, V
 
 template <class Grid>
 CRVPT_DEV V3 residualRatioPath(const SceneParams& P, const SupervoxelGrid& S, const Grid& grid, Rng& rng, V3 x,
-                               V3 w) {
+                               V3 w, WorkCount& wc, typename Grid::Cache& gc) {
     float wSum = 0.0f, resT = 0.0f, resDist = 0.0f;
     float T = 1.0f;
     V3 acc = v3(0.0f);
@@ -569,12 +594,13 @@ CRVPT_DEV V3 residualRatioPath(const SceneParams& P, const SupervoxelGrid& S, co
         while (refInside(S, q.idx[0], q.idx[1], q.idx[2])) {
             const size_t si = refIndex(S, q.idx[0], q.idx[1], q.idx[2]);
             const float muC = S.rrt[2 * si], muR = S.rrt[2 * si + 1];
+            wc.svReads++;
             V3 mn, mx;
             refSvBounds(P, S, q.idx, mn, mx);
             float a = 0.0f, b = 0.0f;
             rayBoxIntersect(mn, mx, oldX, w, a, b);
             x = oldX + w * a;
-            T *= rrtEstimator(P, grid, rng, x, w, a, b, T, wSum, resT, resDist, muC, muR);
+            T *= rrtEstimator(P, grid, rng, x, w, a, b, T, wSum, resT, resDist, muC, muR, wc, gc);
             refDdaAdvance(q);
         }
         const float xi = rng.uniform();
@@ -594,8 +620,12 @@ CRVPT_DEV V3 residualRatioPath(const SceneParams& P, const SupervoxelGrid& S, co
 
 template <int MODE, class Grid>
 __global__ __launch_bounds__(256) void k_wholePath(SceneParams P, Grid grid, SupervoxelGrid S,
-                                                   float4* __restrict__ out, uint32_t* __restrict__ counter) {
+                                                   float4* __restrict__ out, uint32_t* __restrict__ counter,
+                                                   unsigned long long* __restrict__ stats) {
     const int lane = __lane_id();
+    WorkCount wc;
+    typename Grid::Cache gc;
+    uint32_t paths = 0;
     while (true) {
         const uint64_t all = __ballot(true);
         const int leader = __ffsll((unsigned long long)all) - 1;
@@ -614,11 +644,13 @@ __global__ __launch_bounds__(256) void k_wholePath(SceneParams P, Grid grid, Sup
             cameraRay(P, st, x, w);
             Rng rng;
             rng.init(st.pix, st.frame, P.seedOffset ^ 0x243F6A88u);
-            V3 c = MODE == VPT_DECOMPOSITION_TRACKING ? decompositionPath(P, S, grid, rng, x, w)
-                                                       : residualRatioPath(P, S, grid, rng, x, w);
+            V3 c = MODE == VPT_DECOMPOSITION_TRACKING ? decompositionPath(P, S, grid, rng, x, w, wc, gc)
+                                                       : residualRatioPath(P, S, grid, rng, x, w, wc, gc);
             out[unit] = make_float4(c.x, c.y, c.z, 1.0f);
+            paths++;
         }
     }
+    if (stats) flushCounters(stats, wc.taps, wc.svReads, paths, gc.descents());
 }
 
 // ------------------------------------------------------------------------------------------------------------
@@ -675,12 +707,13 @@ CRVPT_DEV float transmittanceWalker(const SceneParams& P, const SupervoxelGrid& 
                                     V3 d, float limit, bool binary) {
     Walker W;
     if (!walkerStart(W, P, S, o, d, limit)) return 1.0f;
+    typename Grid::Cache gc;
     float T = 1.0f;
     while (true) {
         const int ev = walkStep(W, rng, P, S);
         if (ev == WALK_END) break;
         if (ev != WALK_COLLIDE) continue;
-        const float rho = density(grid, P, rng, W.o + W.w * W.t);
+        const float rho = density(grid, P, rng, W.o + W.w * W.t, gc);
         const float pReal = P.extinction * rho / W.mu;
         if (binary) {
             if (rng.uniform() < pReal) return 0.0f;
@@ -703,6 +736,8 @@ CRVPT_DEV float transmittanceRrt(const SceneParams& P, const SupervoxelGrid& S, 
     RefDda q;
     if (!refDdaInit(P, S, x, w, tMaxVal - tMinVal, q)) return 1.0f;
     float T = 1.0f, wSum = 0.0f, resT = 0.0f, resDist = 0.0f;
+    WorkCount wc;
+    typename Grid::Cache gc;
     while (refInside(S, q.idx[0], q.idx[1], q.idx[2])) {
         const size_t si = refIndex(S, q.idx[0], q.idx[1], q.idx[2]);
         const float muC = S.rrt[2 * si], muR = S.rrt[2 * si + 1];
@@ -711,7 +746,7 @@ CRVPT_DEV float transmittanceRrt(const SceneParams& P, const SupervoxelGrid& S, 
         float a = 0.0f, b = 0.0f;
         rayBoxIntersect(mn, mx, oldX, w, a, b);
         b = fminf(b, lim);
-        if (b > a) T *= rrtEstimator(P, grid, rng, oldX + w * a, w, a, b, T, wSum, resT, resDist, muC, muR);
+        if (b > a) T *= rrtEstimator(P, grid, rng, oldX + w * a, w, a, b, T, wSum, resT, resDist, muC, muR, wc, gc);
         refDdaAdvance(q);
     }
     return T;
@@ -728,6 +763,7 @@ CRVPT_DEV float transmittanceDecomposition(const SceneParams& P, const Supervoxe
     const V3 svSize = v3(float(S.refSize)) * boxDelta / dims;
     x += w * tMinVal;
     float travelled = tMinVal;
+    typename Grid::Cache gc;
     const V3 sp = (x - v3(P.boxMin)) / boxDelta * dims / v3(float(S.refSize));
     int ix = int(floorf(sp.x)), iy = int(floorf(sp.y)), iz = int(floorf(sp.z));
     while (refInside(S, ix, iy, iz)) {
@@ -752,7 +788,7 @@ CRVPT_DEV float transmittanceDecomposition(const SceneParams& P, const Supervoxe
                 tr += rng.exponential() / muR;
                 if (tc >= dUse && tr >= dUse) break;
                 if (tc <= tr) return 0.0f;
-                const float rho = density(grid, P, rng, x + w * tr);
+                const float rho = density(grid, P, rng, x + w * tr, gc);
                 if (rng.uniform() * muR < P.extinction * rho - muC) return 0.0f;
             }
         }
@@ -929,10 +965,16 @@ __global__ void k_normalize(const float* __restrict__ raw, float* __restrict__ o
 // ------------------------------------------------------------------------------------------------------------
 // Grid lookup probes (exactness checks of the accessors)
 // ------------------------------------------------------------------------------------------------------------
+constexpr int kLookupRun = 8;
+
 template <class Grid>
 __global__ void k_gridLookup(Grid grid, int n, const int32_t* __restrict__ ijk, float* __restrict__ out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = grid.voxelAt(ijk[3 * i], ijk[3 * i + 1], ijk[3 * i + 2]);
+    // Each lane answers a run of kLookupRun consecutive queries through one lookup memo, the way a path's
+    // consecutive density taps do, so coherent query lists exercise memo hits as well as misses.
+    const int first = (blockIdx.x * blockDim.x + threadIdx.x) * kLookupRun;
+    const int last = min(first + kLookupRun, n);
+    typename Grid::Cache c;
+    for (int q = first; q < last; q++) out[q] = grid.voxelAt(ijk[3 * q], ijk[3 * q + 1], ijk[3 * q + 2], c);
 }
 
 // NanoVDB world (grid) coordinates -> world_to_indexf -> floor -> value, the nearest sampleCloudRaw path.
@@ -969,11 +1011,11 @@ static hipError_t launchRenderT(const SceneParams& P, const Grid& grid, const Su
             break;
         case VPT_DECOMPOSITION_TRACKING:
             hipLaunchKernelGGL((k_wholePath<VPT_DECOMPOSITION_TRACKING, Grid>), gridDim, block, 0, stream, P, grid,
-                               S, paths, counter);
+                               S, paths, counter, stats);
             break;
         case VPT_RESIDUAL_RATIO_TRACKING:
             hipLaunchKernelGGL((k_wholePath<VPT_RESIDUAL_RATIO_TRACKING, Grid>), gridDim, block, 0, stream, P,
-                               grid, S, paths, counter);
+                               grid, S, paths, counter, stats);
             break;
         default:
             hipLaunchKernelGGL((k_pathPool<VPT_DELTA_TRACKING, Grid>), gridDim, block, 0, stream, P, grid, S, paths,
@@ -1062,7 +1104,7 @@ hipError_t launchNormalize(const float* raw, float* out, size_t n, float vmin, f
 
 hipError_t launchGridLookup(const DenseGrid* dense, const NanoGrid* nano, int n, const int32_t* ijk, float* out,
                             hipStream_t stream) {
-    const dim3 grid((n + 255) / 256), block(256);
+    const dim3 grid((n + 256 * kLookupRun - 1) / (256 * kLookupRun)), block(256);
     if (nano) {
         // index-space coordinates: voxelAt adds the index minimum, so pass raw NanoVDB coordinates through value()
         NanoGrid g = *nano;

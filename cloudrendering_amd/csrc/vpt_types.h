@@ -111,7 +111,7 @@ struct SceneParams {
     int32_t width, height;
     uint32_t frameBase;     // global frame number of sample 0 of this launch
     uint32_t numFrames;     // samples per pixel in this launch
-    uint32_t pad1;
+    uint32_t maxBounces;    // scattering events per path before it ends with zero radiance; 0 = unbounded
     uint64_t numUnits;      // width * height * numFrames
 };
 

@@ -31,7 +31,7 @@ __all__ = [
     "get_density_field", "get_nanovdb_buffer", "generate_fbm_cloud", "transmittance_batch",
     "set_use_majorant_grid", "set_launch_config", "get_last_render_stats", "get_num_lights",
     "set_collect_stats", "get_last_render_counters", "build_nanovdb_from_dense", "grid_lookup",
-    "grid_lookup_world", "get_supervoxel_tables",
+    "grid_lookup_world", "get_supervoxel_tables", "set_max_bounces",
     "VPT_MODE_NAMES", "CrvptError",
 ]
 
@@ -233,6 +233,12 @@ def set_seed_offset(offset: int) -> None:
     call("crvpt_set_seed_offset", int(offset))
 
 
+def set_max_bounces(max_bounces: int) -> None:
+    """Scattering events per path before it ends with zero further radiance; 0 (the default) is unbounded, as in
+    the reference trackers."""
+    call("crvpt_set_max_bounces", int(max_bounces))
+
+
 def get_volume_voxel_size() -> List[int]:
     v = (ctypes.c_int64 * 3)()
     call("crvpt_get_volume_voxel_size", v)
@@ -368,9 +374,10 @@ def set_collect_stats(collect: bool) -> None:
 
 
 def get_last_render_counters() -> dict:
-    v = (ctypes.c_uint64 * 3)()
+    v = (ctypes.c_uint64 * 4)()
     call("crvpt_get_last_render_counters", v)
-    return {"density_taps": int(v[0]), "supervoxel_reads": int(v[1]), "paths": int(v[2])}
+    return {"density_taps": int(v[0]), "supervoxel_reads": int(v[1]), "paths": int(v[2]),
+            "tree_descents": int(v[3])}
 
 
 def build_nanovdb_from_dense(density_zyx, dx: float, origin=(0.0, 0.0, 0.0)) -> bytes:

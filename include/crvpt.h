@@ -81,6 +81,10 @@ int crvpt_set_extinction_scale(double extinction_scale);
 int crvpt_set_extinction_base(const double extinction_base[3]);
 int crvpt_set_phase_g(double phase_g);
 int crvpt_set_seed_offset(int64_t offset);
+/* Scattering events per path before the path ends with zero further radiance (0 = unbounded). No counterpart in
+ * the reference, whose trackers never cut a path (DeltaTracking.glsl:167-345 loops until the path leaves or is
+ * absorbed); it bounds path length for benchmark workloads that state a maximum bounce count. */
+int crvpt_set_max_bounces(int32_t max_bounces);
 
 /* Module.cpp camera API: get_camera_position, get_camera_view_matrix (16, column-major like glm),
  * get_camera_fovy (radians), set_camera_position, set_camera_target, overwrite_camera_view_matrix,
@@ -153,10 +157,11 @@ int crvpt_set_launch_config(int supervoxel_size, int blocks, int steps_per_refil
 int crvpt_get_last_render_stats(double* path_kernel_ms, double* total_ms, uint64_t* num_paths, int* launches);
 
 /* No reference counterpart: work counters of the path kernels for the algorithmic-byte roofline. When enabled,
- * each crvpt_render_frame accumulates [density taps, supervoxel majorant reads, paths] (wave-reduced atomics at
- * kernel exit); crvpt_get_last_render_counters reads the totals of the last call. */
+ * each crvpt_render_frame accumulates [density taps, supervoxel majorant / table reads, paths, NanoVDB tree
+ * descents (lookups that missed the per-lane 8^3-block memo)] (wave-reduced atomics at kernel exit);
+ * crvpt_get_last_render_counters reads the totals of the last call. */
 int crvpt_set_collect_stats(int collect);
-int crvpt_get_last_render_counters(uint64_t out[3]);
+int crvpt_get_last_render_counters(uint64_t out[4]);
 
 #ifdef __cplusplus
 }

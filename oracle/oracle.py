@@ -50,6 +50,7 @@ class OracleScene(ctypes.Structure):
         ("num_lights", ctypes.c_int), ("lights", ctypes.POINTER(OracleLight)),
         ("inverse_view_proj", ctypes.c_float * 16), ("inverse_view", ctypes.c_float * 16),
         ("mode", ctypes.c_int), ("custom_seed_offset", ctypes.c_uint32), ("super_voxel_size", ctypes.c_int),
+        ("max_bounces", ctypes.c_int),
     ]
 
 
@@ -122,7 +123,7 @@ class Scene:
                  camera=None, width=64, height=64, seed_offset=0, max_grid_extent=0.25, box=None,
                  sun_direction=(0.5826, 0.7660, 0.2717), sun_intensity=2.6,
                  sun_color=(1.0, 0.961538462, 0.884615385), super_voxel_size=8, use_linear_rgb=True,
-                 voxel_range=None):
+                 voxel_range=None, max_bounces=0):
         self.density = np.ascontiguousarray(np.asarray(density_zyx, np.float32))
         nz, ny, nx = self.density.shape
         sc = OracleScene()
@@ -181,6 +182,7 @@ class Scene:
         sc.mode = MODES[mode] if isinstance(mode, str) else int(mode)
         sc.custom_seed_offset = seed_offset & 0xFFFFFFFF
         sc.super_voxel_size = super_voxel_size
+        sc.max_bounces = int(max_bounces)
         self.sc = sc
         self.width, self.height = width, height
 

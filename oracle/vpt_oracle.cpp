@@ -363,6 +363,7 @@ vec3 deltaTracking(const Ctx& c, Rng& rng, vec3 x, vec3 w) {  // DeltaTracking.g
     float scatteringAlbedo = s->scattering_albedo[0];
     float PA = absorptionAlbedo * s->extinction[0];
     float PS = scatteringAlbedo * s->extinction[0];
+    int bounces = 0;
     float tMin, tMax;
     if (rayBoxIntersect(c.boxMin, c.boxMax, x, w, tMin, tMax)) {
         x += w * tMin;
@@ -380,6 +381,7 @@ vec3 deltaTracking(const Ctx& c, Rng& rng, vec3 x, vec3 w) {  // DeltaTracking.g
             float xi = rng.random();
             if (xi < Pa) return vec3(0.0f);
             if (xi < 1 - Pn) {
+                if (s->max_bounces > 0 && ++bounces > s->max_bounces) return vec3(0.0f);
                 float pdf_w;
                 w = importanceSamplePhase(rng, s->phase_g, w, pdf_w);
                 if (rayBoxIntersect(c.boxMin, c.boxMax, x, w, tMin, tMax)) {
@@ -401,6 +403,7 @@ vec3 ratioTracking(const Ctx& c, Rng& rng, vec3 x, vec3 w) {  // RatioTracking.g
     float PA = (1.0f - s->scattering_albedo[0]) * s->extinction[0];
     float PS = s->scattering_albedo[0] * s->extinction[0];
     float transmittance = 1.0f;
+    int bounces = 0;
     float tMin, tMax;
     if (rayBoxIntersect(c.boxMin, c.boxMax, x, w, tMin, tMax)) {
         x += w * tMin;
@@ -418,6 +421,7 @@ vec3 ratioTracking(const Ctx& c, Rng& rng, vec3 x, vec3 w) {  // RatioTracking.g
             float Pn = sigma_n / majorant;
             float xi = rng.random();
             if (xi < 1 - Pn) {
+                if (s->max_bounces > 0 && ++bounces > s->max_bounces) return vec3(0.0f);
                 transmittance *= 1.f - Pa / (Pa + Ps);
                 float pdf_w;
                 w = importanceSamplePhase(rng, s->phase_g, w, pdf_w);
@@ -439,6 +443,7 @@ vec3 nextEventTracking(const Ctx& c, Rng& rng, vec3 x, vec3 w) {  // NextEventTr
     float PA = (1.0f - s->scattering_albedo[0]) * s->extinction[0];
     vec3 color(0.0f);
     bool hasValue = false;
+    int bounces = 0;
     float tMin, tMax;
     if (rayBoxIntersect(c.boxMin, c.boxMax, x, w, tMin, tMax)) {
         x += w * tMin;
@@ -456,6 +461,7 @@ vec3 nextEventTracking(const Ctx& c, Rng& rng, vec3 x, vec3 w) {  // NextEventTr
             float xi = rng.random();
             if (xi < Pa) return color;
             if (xi < 1 - Pn) {
+                if (s->max_bounces > 0 && ++bounces > s->max_bounces) return color;
                 float pdf_w;
                 vec3 next_w = importanceSamplePhase(rng, s->phase_g, w, pdf_w);
                 hasValue = true;
@@ -528,6 +534,7 @@ vec3 analogDecompositionTracking(const Ctx& c, Rng& rng, vec3 x, vec3 w) {  // D
         vec3 startPoint = (x - c.boxMin) / boxDelta * voxelGridSize / vec3(float(c.svSize));
         int idx[3] = {int(std::floor(startPoint.x)), int(std::floor(startPoint.y)), int(std::floor(startPoint.z))};
         int cached[3] = {-1, -1, -1};
+        int bounces = 0;
         float mmMin = 0.0f, mmMax = 0.0f;
         while (svInside(c, idx[0], idx[1], idx[2])) {
             vec3 minP = c.boxMin + vec3(float(idx[0]), float(idx[1]), float(idx[2])) * superVoxelSize;
@@ -566,6 +573,7 @@ vec3 analogDecompositionTracking(const Ctx& c, Rng& rng, vec3 x, vec3 w) {  // D
                     if (isCollision) {
                         x = xs;
                         if (rng.random() < absorptionAlbedo) return vec3(0.0f);
+                        if (s->max_bounces > 0 && ++bounces > s->max_bounces) return vec3(0.0f);
                         float pdf_w;
                         w = importanceSamplePhase(rng, s->phase_g, w, pdf_w);
                         t_r = 0.0f;

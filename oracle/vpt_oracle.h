@@ -69,6 +69,8 @@ typedef struct OracleScene {
     int mode;
     uint32_t custom_seed_offset;
     int super_voxel_size;      /* requested supervoxel edge for RRT / decomposition (default 8) */
+    int max_bounces;           /* scattering events before a path ends with zero further radiance; 0 = unbounded
+                                  (a benchmark-workload bound with no counterpart in the reference) */
 } OracleScene;
 
 /* Camera matrices of the build's camera model (pos, target-or-front, up, fovy in radians, near, far). */

@@ -136,6 +136,24 @@ def test_render_vs_oracle(vpt, oracle, name, mode, interp):
     _check_equal(m_gpu, s_gpu, m_ref, s_ref, f"{name}/{interp}")
 
 
+@pytest.mark.parametrize("name,mode", [("delta", 0), ("ratio", 2), ("next_event", 5), ("decomposition", 3)])
+def test_max_bounces_vs_oracle(vpt, oracle, name, mode):
+    print(f"case: max bounces 1 mode={name}", flush=True)
+    d = oracle.fbm_cloud(24, 20, 16, seed=5)
+    _set_volume(vpt, d)
+    _common(vpt, mode, ext=96.0, g=0.5)
+    vpt.set_max_bounces(1)
+    try:
+        m_gpu, s_gpu, _ = _render_stats(vpt, SPP)
+    finally:
+        vpt.set_max_bounces(0)
+    sc = oracle.Scene(d, mode=name, extinction=96.0, g=0.5, width=RES, height=RES, max_bounces=1)
+    m_ref, s_ref = _oracle_stats(sc, SPP)
+    _check_equal(m_gpu, s_gpu, m_ref, s_ref, f"{name}/max_bounces=1")
+    free, _ = _oracle_stats(oracle.Scene(d, mode=name, extinction=96.0, g=0.5, width=RES, height=RES), SPP)
+    assert np.any(np.abs(free - m_ref) > 3.0 * s_ref), "the bounce limit should change this scene's image"
+
+
 @pytest.mark.parametrize("g", [0.0, 0.85])
 def test_next_event_env_image_and_sun_vs_oracle(vpt, oracle, g):
     print(f"case: next-event env image + directional sun g={g}", flush=True)

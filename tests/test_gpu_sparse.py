@@ -30,6 +30,27 @@ def test_device_lookups_bit_exact_vs_reference(vpt, path):
     np.testing.assert_array_equal(vals.view(np.uint32), g["world_values"].view(np.uint32))
 
 
+def test_coherent_lookups_through_the_memo(vpt, oracle):
+    """Queries in x-fastest order over the whole index box plus a random walk: consecutive queries mostly fall in the
+    same 8^3 block, so the per-lane lookup memo answers most of them; every answer must equal the voxel written."""
+    print("case: coherent NanoVDB lookups 40x36x28", flush=True)
+    d = oracle.fbm_cloud(40, 36, 28, seed=13)
+    vmin, vmax = float(d.min()), float(d.max())
+    norm = ((d - vmin) / (vmax - vmin)).astype(np.float32)
+    vpt.set_nanovdb_buffer(vpt.build_nanovdb_from_dense(norm, 0.01))
+    k, j, i = np.meshgrid(np.arange(-3, 31), np.arange(-3, 39), np.arange(-3, 43), indexing="ij")
+    ijk = np.stack([i.ravel(), j.ravel(), k.ravel()], 1).astype(np.int32)
+    rng = np.random.default_rng(3)
+    walk = np.cumsum(rng.integers(-1, 2, size=(20000, 3)), 0) + np.array([20, 18, 14])
+    walk = np.clip(walk, -2, 44).astype(np.int32)
+    q = np.concatenate([ijk, walk])
+    got = vpt.grid_lookup(q)
+    inside = np.all((q >= 0) & (q < np.array([40, 36, 28])), 1)
+    expect = np.zeros(len(q), np.float32)
+    expect[inside] = norm[q[inside, 2], q[inside, 1], q[inside, 0]]
+    np.testing.assert_array_equal(got.view(np.uint32), expect.view(np.uint32))
+
+
 @pytest.mark.parametrize("interp", ["nearest", "stochastic"])
 @pytest.mark.parametrize("sparse", [False, True])
 def test_supervoxel_tables_bit_exact(vpt, oracle, interp, sparse):

@@ -273,3 +273,23 @@ def test_supervoxel_tables_min_max(oracle, interp):
         mu_r = max(np.float32(block.max()) * np.float32(100.0) - np.float32(mu_min), np.float32(0.1))
         assert rrt[si, 1] == np.float32(mu_r)
         assert mu_min - 1e-4 <= rrt[si, 0] <= 100.0 * block.mean() + 1e-3
+
+
+@pytest.mark.parametrize("mode", ["delta", "ratio", "next_event", "decomposition"])
+def test_max_bounces_truncates_paths(oracle, mode):
+    """The bounce limit (a workload bound the reference does not have) only cuts paths: with the same random stream a
+    limit far above any path's length renders bit for bit what the unbounded tracker renders, and for the estimators
+    whose radiance is only collected at the path's end (delta, ratio, decomposition) a cut path gives 0, so every
+    pixel of a limited render is at most the unbounded one and the limits are ordered."""
+    d = oracle.fbm_cloud(16, 16, 16, seed=21)
+    kw = dict(mode=mode, extinction=96.0, g=0.5, width=16, height=16)
+    free, _ = oracle.Scene(d, **kw).render(8)
+    high, _ = oracle.Scene(d, max_bounces=100000, **kw).render(8)
+    np.testing.assert_array_equal(free, high)
+    one, _ = oracle.Scene(d, max_bounces=1, **kw).render(8)
+    two, _ = oracle.Scene(d, max_bounces=2, **kw).render(8)
+    assert not np.array_equal(one, free)
+    if mode != "next_event":
+        # running means in FP32: allow the last-ulp difference of the 1/(frame+1) mix
+        assert np.all(one <= two * (1 + 1e-6) + 1e-7)
+        assert np.all(two <= free * (1 + 1e-6) + 1e-7)
