@@ -34,8 +34,8 @@ C2 = {"max_bounces": 128, "grid": 512, "edge0": 0.645, "edge1": 0.795, "width": 
                 (3, "decomposition tracking")], "gs": [0.0, 0.5, 0.85]}
 C1 = {"max_bounces": 64, "grid": 128, "width": 256, "height": 256, "spp": 64}
 MODE_KERNELS = {0: "k_pathPool<DELTA_TRACKING, NanoAccess>", 2: "k_pathPool<RATIO_TRACKING, NanoAccess>",
-                4: "k_wholePath<RESIDUAL_RATIO_TRACKING, NanoAccess>",
-                3: "k_wholePath<DECOMPOSITION_TRACKING, NanoAccess>"}
+                4: "k_pathPool<RESIDUAL_RATIO_TRACKING, NanoAccess>",
+                3: "k_pathPool<DECOMPOSITION_TRACKING, NanoAccess>"}
 
 
 def setup_c1(vpt, torch):
@@ -75,6 +75,7 @@ def setup_c2(vpt):
     vpt.set_use_lights(False)
     vpt.set_use_sparse_grid(True)
     vpt.set_launch_config(16, 0, 8)
+    vpt.set_super_voxel_size(16)
     vpt.set_max_bounces(C2["max_bounces"])
 
 
@@ -160,14 +161,15 @@ def load_traffic(workload: str):
 
 
 def copy_calibration(torch):
-    """A plain device copy of 256 MiB (FETCH_SIZE / WRITE_SIZE scale factor for the PMC summaries)."""
+    """A plain streaming copy kernel of 256 MiB read + 256 MiB written (an elementwise multiply by 1, so it runs as
+    a kernel dispatch the PMC counters see): the FETCH_SIZE / WRITE_SIZE scale factor for the PMC summaries."""
     n = 256 * 1024 * 1024 // 4
     a = torch.ones(n, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
     for _ in range(3):
-        b.copy_(a)
+        torch.mul(a, 1.0, out=b)
     torch.cuda.synchronize()
-    print(json.dumps({"copy_bytes": 4 * n}), flush=True)
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}), flush=True)
 
 
 def main() -> int:
@@ -223,6 +225,7 @@ def main() -> int:
         vpt.render_frame(target, spp)
         st = vpt.get_last_render_stats()
         per_combo_ms.setdefault(combos[k % len(combos)], []).append(st["path_kernel_ms"])
+        launches = st["launches"]
         print(f"step {k}: {combos[k % len(combos)]} path kernel {st['path_kernel_ms']:.1f} ms", file=sys.stderr,
               flush=True)
     torch.cuda.synchronize()
@@ -256,7 +259,7 @@ def main() -> int:
         n_c += 1
     vpt.set_collect_stats(False)
     taps, svr, paths, desc = (v / max(n_c, 1) for v in (taps, svr, paths, desc))
-    # Algorithmic bytes per launch: 4-byte FP32 density taps, 4-byte majorant / 8-byte (control, residual) table
+    # Algorithmic bytes per render: 4-byte FP32 density taps, 4-byte majorant / 8-byte (control, residual) table
     # reads, 40 bytes per sparse-tree descent (root tile key and child 8 + 8, upper and lower child-mask words 4 + 4,
     # upper and lower table entries 8 + 8), and the 16-byte radiance record of each path.
     b_alg = 4 * taps + (4 if dom in (0, 2, 5) else 8) * svr + 40 * desc + 16 * paths
@@ -291,7 +294,8 @@ def main() -> int:
         "config": config,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": load_traffic(args.workload),
-                     "kernel": kernel, "kernel_ms": round(dom_ms, 3), "algorithmic_bytes": int(b_alg),
+                     "kernel": kernel, "launches_per_step": launches,
+                     "launch_ms": round(dom_ms / launches, 4), "algorithmic_bytes_per_launch": int(b_alg / launches),
                      "density_taps": int(taps), "supervoxel_reads": int(svr), "tree_descents": int(desc),
                      "paths": int(paths)},
         "kernel_ms_per_mode_g": per_combo,

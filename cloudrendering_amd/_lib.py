@@ -48,6 +48,7 @@ SIGNATURES = {
     "crvpt_set_phase_g": (_c_int, [_c_d]),
     "crvpt_set_seed_offset": (_c_int, [_c_i64]),
     "crvpt_set_max_bounces": (_c_int, [_c_int]),
+    "crvpt_set_super_voxel_size": (_c_int, [_c_int]),
     "crvpt_get_camera_position": (_c_int, [_pd]),
     "crvpt_get_camera_view_matrix": (_c_int, [_pd]),
     "crvpt_get_camera_fovy": (_c_int, [_pd]),

@@ -213,6 +213,7 @@ struct Renderer {
     bool envDirty = true;
     int svSize = 8;
     int svGrid[3] = {1, 1, 1};
+    int superVoxelSize = 8;  // requested edge of the reference tables (VolumetricPathTracingPass.hpp:254)
     int refSvSize = 8;
     int refGrid[3] = {1, 1, 1};
     bool refTablesValid = false;
@@ -453,9 +454,9 @@ int prepareScene(Renderer& r) {
     if (needRef && (!r.refTablesValid || r.refTablesInterp != r.interpolation ||
                     r.refTablesExtinction != double(extinctionX(r)))) {
         // SuperVoxelGridResidualRatioTracking / ...DecompositionTracking constructors (SuperVoxelGrid.cpp:40-91,
-        // 269-405): edge 8, halved until it divides grids smaller than the edge.
+        // 269-405): the requested edge, halved until it divides grids smaller than the edge.
         const int* ad = activeDims(r);
-        int sv = 8;
+        int sv = std::max(r.superVoxelSize, 1);
         if (ad[0] < sv || ad[1] < sv || ad[2] < sv) {
             while (!(ad[0] % sv == 0 && ad[1] % sv == 0 && ad[2] % sv == 0)) sv /= 2;
         }
@@ -525,6 +526,11 @@ void fillSceneParams(Renderer& r, int w, int h, SceneParams& P, SupervoxelGrid& 
     }
     P.extinction = extinctionX(r);
     P.albedo = float(r.albedo[0]);
+    for (int a = 0; a < 3; a++) {
+        P.extinctionRGB[a] = float(r.extinctionBase[a]) * float(r.extinctionScale);
+        P.albedoRGB[a] = float(r.albedo[a]);
+    }
+    P.extinctionMax = std::max(std::max(P.extinctionRGB[0], P.extinctionRGB[1]), P.extinctionRGB[2]);
     P.phaseG = float(r.phaseG);
     P.mode = r.mode;
     P.interpolation = r.interpolation;
@@ -960,10 +966,9 @@ int crvpt_set_grid_interpolation_type(int type) {
 
 int crvpt_set_vpt_mode(int64_t mode) {
     if (int rc = requireInit()) return rc;
-    if (mode != VPT_DELTA_TRACKING && mode != VPT_RATIO_TRACKING && mode != VPT_DECOMPOSITION_TRACKING &&
-        mode != VPT_RESIDUAL_RATIO_TRACKING && mode != VPT_NEXT_EVENT_TRACKING)
+    if (mode < VPT_DELTA_TRACKING || mode > VPT_NEXT_EVENT_TRACKING_SPECTRAL)
         return fail(CRVPT_ERR_UNSUPPORTED, "set_vpt_mode: mode " + std::to_string(mode) +
-                                               " is not on the gfx950 hot path (supported: 0, 2, 3, 4, 5)");
+                                               " is not on the gfx950 hot path (supported: 0-6)");
     g_r->mode = int(mode);
     g_r->resetAccumulation();
     return CRVPT_OK;
@@ -1009,6 +1014,16 @@ int crvpt_set_phase_g(double g) {
 int crvpt_set_seed_offset(int64_t offset) {
     if (int rc = requireInit()) return rc;
     g_r->seedOffset = uint32_t(offset);
+    g_r->resetAccumulation();
+    return CRVPT_OK;
+}
+
+int crvpt_set_super_voxel_size(int32_t super_voxel_size) {
+    if (int rc = requireInit()) return rc;
+    if (super_voxel_size < 1 || super_voxel_size > 64)
+        return fail(CRVPT_ERR_INVALID_ARGUMENT, "set_super_voxel_size: the edge must lie in [1, 64]");
+    g_r->superVoxelSize = super_voxel_size;
+    g_r->refTablesValid = false;
     g_r->resetAccumulation();
     return CRVPT_OK;
 }

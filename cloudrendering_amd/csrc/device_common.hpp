@@ -34,6 +34,8 @@ CRVPT_DEV V3 cross(V3 a, V3 b) { return {a.y * b.z - b.y * a.z, a.z * b.x - b.z 
 CRVPT_DEV float length(V3 a) { return sqrtf(dot(a, a)); }
 CRVPT_DEV V3 normalize(V3 a) { return a * rsqrtf(dot(a, a)); }
 CRVPT_DEV float comp(V3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+CRVPT_DEV V3 vmin(V3 a, float b) { return {fminf(a.x, b), fminf(a.y, b), fminf(a.z, b)}; }
+CRVPT_DEV float avg3(V3 a) { return (a.x + a.y + a.z) * (1.0f / 3.0f); }
 CRVPT_DEV float lerpf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
 CRVPT_DEV V3 lerp3(V3 a, V3 b, float t) { return {lerpf(a.x, b.x, t), lerpf(a.y, b.y, t), lerpf(a.z, b.z, t)}; }
 CRVPT_DEV float smoothstepf(float e0, float e1, float x) {

@@ -30,38 +30,122 @@ constexpr float kInf = 3.0e38f;
 #endif
 
 // ------------------------------------------------------------------------------------------------------------
-// Supervoxel walker
+// Cell walker
+//
+// A ray segment is walked cell by cell through a grid of supervoxels with a piecewise-constant collision rate:
+//  * delta, ratio and next-event tracking (and the transmittance queries) walk the majorant grid: rate
+//    extinction * (max density reachable by a lookup in the cell);
+//  * decomposition tracking walks the reference's (min, max) table: control rate mu_c = extinction * min and
+//    residual rate mu_r = extinction * max - mu_c (DecompositionTracking.glsl:67-75). The competing control and
+//    residual exponentials of the reference are one Poisson process of rate mu_c + mu_r whose events are control
+//    collisions with probability mu_c / (mu_c + mu_r);
+//  * residual ratio tracking walks the reference's (mu_c, mu_r_bar) table: tentative collisions at rate mu_r_bar,
+//    and the control transmittance exp(-integral of mu_c) accumulated along the way (ResidualRatioTracking.glsl:
+//    33-76).
+// An Exp(1) optical budget carries over cell boundaries (the process is memoryless, so this samples the same
+// collision distribution as the reference's fresh draw per supervoxel). One call crosses up to kMaxCrossings
+// cells before it returns, so empty space costs cell reads but no random-number blocks.
 // ------------------------------------------------------------------------------------------------------------
+#ifndef CRVPT_MAX_CROSSINGS
+#define CRVPT_MAX_CROSSINGS 8
+#endif
+constexpr int kMaxCrossings = CRVPT_MAX_CROSSINGS;
+
+template <int MODE>
+constexpr bool walksReferenceTable() {
+    return MODE == VPT_DECOMPOSITION_TRACKING || MODE == VPT_RESIDUAL_RATIO_TRACKING;
+}
+template <int MODE>
+constexpr bool isSpectral() {
+    return MODE == VPT_SPECTRAL_DELTA_TRACKING || MODE == VPT_NEXT_EVENT_TRACKING_SPECTRAL;
+}
+template <int MODE>
+constexpr bool isNee() {
+    return MODE == VPT_NEXT_EVENT_TRACKING || MODE == VPT_NEXT_EVENT_TRACKING_SPECTRAL;
+}
+
 struct Walker {
-    uint32_t svReads;   // supervoxel majorant reads since the launch started (work counter)
+    uint32_t svReads;   // cell reads since the launch started (work counter)
     V3 o, w;            // ray origin / direction (world)
     float t, tEnd;      // current distance along the ray, end of the segment
-    int ix, iy, iz;     // current supervoxel
-    float nx, ny, nz;   // distance of the next supervoxel plane per axis
+    int ix, iy, iz;     // current cell
+    float nx, ny, nz;   // distance of the next cell plane per axis
     float dx, dy, dz;   // plane spacing along the ray per axis
-    float mu;           // majorant of the current supervoxel, extinction units
+    float mu;           // tentative-collision rate of the current cell, extinction units
+    float muC;          // control rate of the current cell (decomposition and residual ratio tracking)
+    float ctrl;         // control optical depth walked since the caller last cleared it (residual ratio tracking)
+    float ext;          // extinction scale of the segment's majorants (spectral trackers: max component on the
+                        // main path, extinction.x on shadow rays)
     float tau;          // remaining optical length of the current Exp(1) sample; < 0 when none is drawn
 };
 
 CRVPT_DEV float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
 
-CRVPT_DEV float svMajorant(const SceneParams& P, const SupervoxelGrid& S, int ix, int iy, int iz) {
-    return P.extinction * S.majorant[(size_t(iz) * S.grid[1] + iy) * S.grid[0] + ix];
+template <int MODE>
+CRVPT_DEV int cellEdge(const SupervoxelGrid& S) {
+    return walksReferenceTable<MODE>() ? S.refSize : S.size;
+}
+template <int MODE>
+CRVPT_DEV int cellGridDim(const SupervoxelGrid& S, int a) {
+    return walksReferenceTable<MODE>() ? S.refGrid[a] : S.grid[a];
+}
+
+// Rates of the cell (W.ix, W.iy, W.iz).
+template <int MODE>
+CRVPT_DEV void cellRates(Walker& W, const SceneParams& P, const SupervoxelGrid& S) {
+    if (walksReferenceTable<MODE>()) {
+        const size_t c = (size_t(W.iz) * S.refGrid[1] + W.iy) * S.refGrid[0] + W.ix;
+        if (MODE == VPT_DECOMPOSITION_TRACKING) {
+            const float2 m = reinterpret_cast<const float2*>(S.decomp)[c];
+            // supervoxels whose max is below 1e-5 are skipped (DecompositionTracking.glsl:67-69)
+            const float muC = fmaxf(1e-10f, P.extinction * m.x);
+            const float muR = fmaxf(1e-10f, P.extinction * m.y - muC);
+            W.muC = muC;
+            W.mu = m.y < 1e-5f ? 0.0f : muC + muR;
+        } else {
+            const float2 m = reinterpret_cast<const float2*>(S.rrt)[c];
+            W.muC = m.x;
+            W.mu = m.y;
+        }
+    } else {
+        const float ext = isSpectral<MODE>() ? W.ext : P.extinction;
+        W.mu = ext * S.majorant[(size_t(W.iz) * S.grid[1] + W.iy) * S.grid[0] + W.ix];
+    }
+    W.svReads++;
 }
 
 // Starts a segment from x along w, clipped to the box and to [0, limit]. False when the segment is empty.
 // The slab test follows rayBoxIntersect (near-zero direction components replaced by 1e-6, tMin >= 0).
-CRVPT_DEV bool walkerStart(Walker& W, const SceneParams& P, const SupervoxelGrid& S, V3 x, V3 w, float limit) {
+//
+// Decomposition and residual ratio tracking walk the reference's supervoxel grid until the ray leaves the grid
+// (DecompositionTracking.glsl:55-117, ResidualRatioTracking.glsl:111-194), and that grid overhangs the box when the
+// grid size is not a multiple of the supervoxel edge: their segments end at the exit from the grid's box, where
+// lookups read the zero border (or, with stochastic filtering, the last voxels half a voxel beyond the box). A
+// decomposition path that scatters keeps walking the grid (gridBox), residual ratio tracking and camera rays enter
+// through the box.
+template <int MODE>
+CRVPT_DEV bool walkerStart(Walker& W, const SceneParams& P, const SupervoxelGrid& S, V3 x, V3 w, float limit,
+                           bool shadow = false, bool gridBox = false) {
     const float ax = fabsf(w.x) <= 0.000001f ? 0.000001f : w.x;
     const float ay = fabsf(w.y) <= 0.000001f ? 0.000001f : w.y;
     const float az = fabsf(w.z) <= 0.000001f ? 0.000001f : w.z;
     const V3 inv = v3(rcpf(ax), rcpf(ay), rcpf(az));
+    V3 hi = v3(P.boxMax);
+    if (walksReferenceTable<MODE>()) {
+        hi = v3(P.boxMin[0] + float(S.refGrid[0] * S.refSize) * rcpf(P.worldToTex[0]),
+                P.boxMin[1] + float(S.refGrid[1] * S.refSize) * rcpf(P.worldToTex[1]),
+                P.boxMin[2] + float(S.refGrid[2] * S.refSize) * rcpf(P.worldToTex[2]));
+    }
     const V3 c0 = (v3(P.boxMin) - x) * inv;
-    const V3 c1 = (v3(P.boxMax) - x) * inv;
+    const V3 c1 = ((gridBox ? hi : v3(P.boxMax)) - x) * inv;
     float tMin = fmaxf(fmaxf(fminf(c0.x, c1.x), fminf(c0.y, c1.y)), fminf(c0.z, c1.z));
     tMin = fmaxf(0.0f, tMin);
     float tMax = fminf(fminf(fmaxf(c0.x, c1.x), fmaxf(c0.y, c1.y)), fmaxf(c0.z, c1.z));
     if (tMax <= tMin || tMax <= 0.0f) return false;
+    if (walksReferenceTable<MODE>()) {
+        const V3 c2 = (hi - x) * inv;
+        tMax = fmaxf(tMax, fminf(fminf(fmaxf(c0.x, c2.x), fmaxf(c0.y, c2.y)), fmaxf(c0.z, c2.z)));
+    }
     tMax = fminf(tMax, limit);
     if (!(tMax > tMin)) return false;
     W.o = x;
@@ -69,11 +153,14 @@ CRVPT_DEV bool walkerStart(Walker& W, const SceneParams& P, const SupervoxelGrid
     W.t = tMin;
     W.tEnd = tMax;
     W.tau = -1.0f;
-    if (!P.useMajorantGrid) {
-        W.mu = P.extinction;
+    W.ctrl = 0.0f;
+    W.ext = (isSpectral<MODE>() && !shadow) ? P.extinctionMax : P.extinction;
+    if (!walksReferenceTable<MODE>() && !P.useMajorantGrid) {
+        W.mu = isSpectral<MODE>() ? W.ext : P.extinction;
         W.nx = W.ny = W.nz = kInf;
         return true;
     }
+    const int edge = cellEdge<MODE>(S);
     // Cell of a point slightly inside the segment (robust against an entry point on a cell plane).
     const float tc = tMin + 1e-4f * (tMax - tMin);
     const V3 p = x + w * tc;
@@ -82,11 +169,17 @@ CRVPT_DEV bool walkerStart(Walker& W, const SceneParams& P, const SupervoxelGrid
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         const float wa = comp(w, a), pa = comp(p, a), ia = comp(inv, a);
-        const float cw = float(S.size) * rcpf(P.worldToTex[a]);  // supervoxel edge in world units
-        int i = int(floorf((pa - P.boxMin[a]) * (P.worldToTex[a] * (1.0f / float(S.size)))));
-        i = min(max(i, 0), S.grid[a] - 1);
+        const float cw = float(edge) * rcpf(P.worldToTex[a]);  // cell edge in world units
+        int i = int(floorf((pa - P.boxMin[a]) * (P.worldToTex[a] * (1.0f / float(edge)))));
+        i = min(max(i, 0), cellGridDim<MODE>(S, a) - 1);
         idx[a] = i;
-        if (wa > 0.0f) {
+        // Components the slab test clamps to 1e-6 never reach a cell plane within the box (a segment moves less
+        // than 1e-5 world units along them); the clamped inverse would also carry the wrong sign for tiny negative
+        // components.
+        if (fabsf(wa) <= 0.000001f) {
+            nxt[a] = kInf;
+            dlt[a] = kInf;
+        } else if (wa > 0.0f) {
             nxt[a] = tc + (P.boxMin[a] + float(i + 1) * cw - pa) * ia;
             dlt[a] = cw * ia;
         } else if (wa < 0.0f) {
@@ -100,48 +193,55 @@ CRVPT_DEV bool walkerStart(Walker& W, const SceneParams& P, const SupervoxelGrid
     W.ix = idx[0]; W.iy = idx[1]; W.iz = idx[2];
     W.nx = nxt[0]; W.ny = nxt[1]; W.nz = nxt[2];
     W.dx = dlt[0]; W.dy = dlt[1]; W.dz = dlt[2];
-    W.mu = svMajorant(P, S, W.ix, W.iy, W.iz);
-    W.svReads++;
+    cellRates<MODE>(W, P, S);
     return true;
 }
 
 enum : int { WALK_MOVED = 0, WALK_COLLIDE = 1, WALK_END = 2 };
 
-// One walker event: a tentative collision (W.t at the collision), a supervoxel crossing, or the segment end.
-// e is an Exp(1) sample, consumed only when no optical budget is pending.
+// Advances to the next tentative collision (W.t at the collision), the segment end, or, after kMaxCrossings cell
+// crossings without either, returns WALK_MOVED. e is an Exp(1) sample, consumed only when no budget is pending.
+template <int MODE>
 CRVPT_DEV int walkAdvance(Walker& W, const SceneParams& P, const SupervoxelGrid& S, float e) {
     if (W.tau < 0.0f) W.tau = e;
-    const float tExit = fminf(fminf(W.nx, W.ny), fminf(W.nz, W.tEnd));
-    const float opt = W.mu * (tExit - W.t);
-    if (W.tau < opt) {
-        W.t += W.tau * rcpf(W.mu);
-        W.tau = -1.0f;
-        return WALK_COLLIDE;
+    for (int crossings = 0;;) {
+        const float tExit = fminf(fminf(W.nx, W.ny), fminf(W.nz, W.tEnd));
+        const float opt = W.mu * (tExit - W.t);
+        if (W.tau < opt) {
+            const float dt = W.tau * rcpf(W.mu);
+            if (MODE == VPT_RESIDUAL_RATIO_TRACKING) W.ctrl += W.muC * dt;
+            W.t += dt;
+            W.tau = -1.0f;
+            return WALK_COLLIDE;
+        }
+        W.tau -= opt;
+        if (MODE == VPT_RESIDUAL_RATIO_TRACKING) W.ctrl += W.muC * (tExit - W.t);
+        W.t = tExit;
+        if (tExit >= W.tEnd) return WALK_END;
+        if (W.nx <= W.ny && W.nx <= W.nz) {
+            W.ix += W.w.x > 0.0f ? 1 : -1;
+            W.nx += W.dx;
+        } else if (W.ny <= W.nz) {
+            W.iy += W.w.y > 0.0f ? 1 : -1;
+            W.ny += W.dy;
+        } else {
+            W.iz += W.w.z > 0.0f ? 1 : -1;
+            W.nz += W.dz;
+        }
+        if ((unsigned)W.ix >= (unsigned)cellGridDim<MODE>(S, 0) ||
+            (unsigned)W.iy >= (unsigned)cellGridDim<MODE>(S, 1) ||
+            (unsigned)W.iz >= (unsigned)cellGridDim<MODE>(S, 2)) {
+            W.tEnd = W.t;
+            return WALK_END;
+        }
+        cellRates<MODE>(W, P, S);
+        if (++crossings >= kMaxCrossings) return WALK_MOVED;
     }
-    W.tau -= opt;
-    W.t = tExit;
-    if (tExit >= W.tEnd) return WALK_END;
-    if (W.nx <= W.ny && W.nx <= W.nz) {
-        W.ix += W.w.x > 0.0f ? 1 : -1;
-        W.nx += W.dx;
-    } else if (W.ny <= W.nz) {
-        W.iy += W.w.y > 0.0f ? 1 : -1;
-        W.ny += W.dy;
-    } else {
-        W.iz += W.w.z > 0.0f ? 1 : -1;
-        W.nz += W.dz;
-    }
-    if ((unsigned)W.ix >= (unsigned)S.grid[0] || (unsigned)W.iy >= (unsigned)S.grid[1] ||
-        (unsigned)W.iz >= (unsigned)S.grid[2])
-        return WALK_END;
-    W.mu = svMajorant(P, S, W.ix, W.iy, W.iz);
-    W.svReads++;
-    return WALK_MOVED;
 }
 
 // Walker event with the exponential drawn from a buffered stream (transmittance queries).
 CRVPT_DEV int walkStep(Walker& W, Rng& rng, const SceneParams& P, const SupervoxelGrid& S) {
-    return walkAdvance(W, P, S, W.tau < 0.0f ? rng.exponential() : 0.0f);
+    return walkAdvance<VPT_DELTA_TRACKING>(W, P, S, W.tau < 0.0f ? rng.exponential() : 0.0f);
 }
 
 template <class Grid>
@@ -154,19 +254,25 @@ CRVPT_DEV float density(const Grid& g, const SceneParams& P, Rng& rng, V3 p, typ
 }
 
 // ------------------------------------------------------------------------------------------------------------
-// Path pool for the delta (DeltaTracking.glsl:167-345), analog ratio (RatioTracking.glsl:25-96) and next-event
-// (NextEventTracking.glsl:223-394 with NeeTransmittance.glsl ratio-tracking shadow rays) estimators.
+// Path pool: every estimator as a per-lane state machine advanced one walker event per step.
+//   delta tracking          DeltaTracking.glsl:167-345
+//   analog ratio tracking   RatioTracking.glsl:25-96
+//   next-event tracking     NextEventTracking.glsl:223-394, shadow rays by ratio tracking (NeeTransmittance.glsl)
+//   decomposition tracking  DecompositionTracking.glsl:35-123
+//   residual ratio tracking ResidualRatioTracking.glsl:33-224 (reservoir-sampled in-scattering, <= 10 iterations)
 //
-// Random numbers: every walker step of every lane evaluates one Philox block (counter: step index, pixel,
-// global frame, seed offset) in lockstep with the rest of the wave: one word for the free-path exponential, 16-bit
-// fields for the sub-voxel jitter, one word for the collision decision. Scatter events draw from a second counter
-// space, the camera jitter from a third, so a path's draws never depend on the lane schedule.
+// Random numbers: every step of every lane evaluates one Philox block (counter: step index, pixel, global frame,
+// seed offset) in lockstep with the rest of the wave: one word for the free-path exponential, 16-bit fields for
+// the sub-voxel jitter, one word for the collision decision. Scatter events draw from a second counter space, the
+// camera jitter from a third, so a path's draws never depend on the lane schedule.
 // ------------------------------------------------------------------------------------------------------------
 constexpr uint32_t kEventSpace = 0x80000000u;
 constexpr uint32_t kCameraBlock = 0xFFFFFFFFu;
 
 enum : uint32_t { PF_ALIVE = 1u, PF_EXHAUSTED = 2u, PF_HAS_VALUE = 4u, PF_SHADOW = 8u };
-constexpr uint32_t kBounceShift = 8;  // flags bits 8..31 count the scattering events of the main path
+// flags bits 8..31 count the scattering events of the main path (in-scattering iterations for residual ratio
+// tracking)
+constexpr uint32_t kBounceShift = 8;
 
 struct PathState {
     uint32_t unit;     // launch-local path index
@@ -176,10 +282,12 @@ struct PathState {
     uint32_t events;   // next event Philox block
     uint32_t flags;
     uint32_t taps;     // density lookups of this lane since the launch started (work counter)
-    float thr;         // analog ratio weight, or the shadow-ray transmittance while PF_SHADOW is set
-    V3 color;          // next-event accumulated radiance
+    float thr;         // ratio weight; shadow-ray transmittance while PF_SHADOW; running T of residual ratio
+    V3 color;          // next-event accumulated radiance; residual ratio tracking's accumulated radiance
     V3 shadowScale;    // radiance factor of the pending shadow ray
     V3 mainX, mainW;   // where the main path resumes after the shadow ray
+    float wSum, resT, resDist;  // residual ratio tracking reservoir
+    V3 weights;        // spectral tracking path weights
 };
 
 CRVPT_DEV uint4 pathBlock(const SceneParams& P, const PathState& st, uint32_t c0) {
@@ -236,9 +344,13 @@ CRVPT_DEV bool startPath(PathState& st, Walker& W, const SceneParams& P, const S
     st.flags = PF_ALIVE;
     st.thr = 1.0f;
     st.color = v3(0.0f);
+    st.wSum = 0.0f;
+    st.resT = 0.0f;
+    st.resDist = 0.0f;
+    st.weights = v3(1.0f);
     V3 x, w;
     cameraRay(P, st, x, w);
-    if (!walkerStart(W, P, S, x, w, kInf)) {
+    if (!walkerStart<MODE>(W, P, S, x, w, kInf)) {
         writePath(out, st.unit, skyRadiance(P, w), false);
         st.flags = 0;
         return false;
@@ -255,7 +367,7 @@ CRVPT_DEV bool scatterEvent(PathState& st, Walker& W, const SceneParams& P, cons
     const V3 nw = phaseDirection(P.phaseG, W.w, u01(e.x), u01(e.y));
     if (MODE == VPT_RATIO_TRACKING) st.thr *= 1.0f - pAbsOverReal;
     st.flags |= PF_HAS_VALUE;
-    if (MODE == VPT_NEXT_EVENT_TRACKING) {
+    if (isNee<MODE>()) {
         bool toLight = false;
         float probFactor = 1.0f, dist = 0.0f;
         int li = 0;
@@ -279,11 +391,13 @@ CRVPT_DEV bool scatterEvent(PathState& st, Walker& W, const SceneParams& P, cons
             dir = sphereDirection(u01(e2.x), u01(e2.y), v3(1.0f, 0.0f, 0.0f));
         }
         const float ph = evalPhase(P.phaseG, W.w, dir);
-        const V3 L = toLight ? lightRadiance(P, p, li) * (probFactor * ph)
-                             : skyRadiance(P, dir) * (probFactor * ph * (4.0f * kPi));
+        V3 L = toLight ? lightRadiance(P, p, li) * (probFactor * ph)
+                       : skyRadiance(P, dir) * (probFactor * ph * (4.0f * kPi));
+        if (isSpectral<MODE>()) L = L * vmin(st.weights, 100000.0f);
         st.mainX = p;
         st.mainW = nw;
-        if ((L.x > 0.0f || L.y > 0.0f || L.z > 0.0f) && walkerStart(W, P, S, p, dir, toLight ? dist : kInf)) {
+        if ((L.x > 0.0f || L.y > 0.0f || L.z > 0.0f) &&
+            walkerStart<MODE>(W, P, S, p, dir, toLight ? dist : kInf, true)) {
             st.flags |= PF_SHADOW;
             st.shadowScale = L;
             st.thr = 1.0f;
@@ -291,8 +405,17 @@ CRVPT_DEV bool scatterEvent(PathState& st, Walker& W, const SceneParams& P, cons
         }
         st.color += L;  // zero contribution, or the shadow segment lies outside the box (T = 1)
     }
-    if (!walkerStart(W, P, S, p, nw, kInf)) {
-        const V3 c = MODE == VPT_NEXT_EVENT_TRACKING ? st.color : skyRadiance(P, nw) * st.thr;
+    if (MODE == VPT_DECOMPOSITION_TRACKING) {
+        // a real collision absorbs with probability 1 - albedo (DecompositionTracking.glsl:92-98)
+        if (u01(e.z) < 1.0f - P.albedo) {
+            writePath(out, st.unit, v3(0.0f), true);
+            return false;
+        }
+    }
+    if (!walkerStart<MODE>(W, P, S, p, nw, kInf, false, MODE == VPT_DECOMPOSITION_TRACKING)) {
+        const V3 c = isNee<MODE>() ? st.color
+                                   : skyRadiance(P, nw) * (isSpectral<MODE>() ? vmin(st.weights, 100000.0f)
+                                                                              : v3(st.thr));
         writePath(out, st.unit, c, true);
         return false;
     }
@@ -302,46 +425,141 @@ CRVPT_DEV bool scatterEvent(PathState& st, Walker& W, const SceneParams& P, cons
 // End of the current segment: a finished shadow ray resumes its main path, a main path leaves the volume.
 template <int MODE>
 CRVPT_DEV bool endEvent(PathState& st, Walker& W, const SceneParams& P, const SupervoxelGrid& S, float4* out) {
-    if (MODE == VPT_NEXT_EVENT_TRACKING && (st.flags & PF_SHADOW)) {
+    if (isNee<MODE>() && (st.flags & PF_SHADOW)) {
         st.color += st.shadowScale * st.thr;
         st.flags &= ~PF_SHADOW;
-        if (walkerStart(W, P, S, st.mainX, st.mainW, kInf)) return true;
+        if (walkerStart<MODE>(W, P, S, st.mainX, st.mainW, kInf)) return true;
         writePath(out, st.unit, st.color, true);
         return false;
     }
     const bool hv = (st.flags & PF_HAS_VALUE) != 0;
     V3 c;
-    if (MODE == VPT_NEXT_EVENT_TRACKING) c = hv ? st.color : st.color + skyRadiance(P, W.w);
-    else c = skyRadiance(P, W.w) * st.thr;
+    const V3 wts = isSpectral<MODE>() ? vmin(st.weights, 100000.0f) : v3(st.thr);
+    if (isNee<MODE>()) c = hv ? st.color : st.color + skyRadiance(P, W.w) * wts;
+    else c = skyRadiance(P, W.w) * wts;
     writePath(out, st.unit, c, hv);
     return false;
 }
 
-// One walker step of a live path. Returns false when the path finished (its radiance is written).
+// Residual ratio tracking: end of one traversal of the box (ResidualRatioTracking.glsl:196-219). The traversal's
+// sky term is added; the reservoir sample becomes the next in-scattering point unless xi > wSum or ten iterations
+// were made.
+CRVPT_DEV bool rrtTraversalEnd(PathState& st, Walker& W, const SceneParams& P, const SupervoxelGrid& S,
+                               float4* out) {
+    constexpr int MODE = VPT_RESIDUAL_RATIO_TRACKING;
+    const uint4 e = pathBlock(P, st, kEventSpace | st.events++);
+    st.color += skyRadiance(P, W.w) * st.thr;
+    const uint32_t iteration = st.flags >> kBounceShift;
+    if (u01(e.x) > st.wSum || iteration >= 10u) {
+        writePath(out, st.unit, st.color, true);
+        return false;
+    }
+    st.flags += 1u << kBounceShift;
+    const V3 x = W.o + W.w * st.resDist;
+    const V3 nw = phaseDirection(P.phaseG, W.w, u01(e.y), u01(e.z));
+    st.thr = st.resT;
+    st.wSum = 0.0f;
+    st.resT = 0.0f;
+    st.resDist = 0.0f;
+    if (!walkerStart<MODE>(W, P, S, x, nw, kInf)) {
+        st.color += skyRadiance(P, nw) * st.thr;
+        writePath(out, st.unit, st.color, true);
+        return false;
+    }
+    return true;
+}
+
+// Tentative collision of a spectral tracker's main path at p (DeltaTracking.glsl:87-160,
+// NextEventTracking.glsl:89-215): collision probabilities from the path-history-weighted averages of the RGB
+// absorption, scattering and null coefficients; the weights carry the ratio of each channel's coefficient to the
+// chosen probability and are clamped to 100.
+template <int MODE>
+CRVPT_DEV bool spectralCollision(PathState& st, Walker& W, const SceneParams& P, const SupervoxelGrid& S, V3 p,
+                                 float rho, float xi, float4* out) {
+    const V3 ext = v3(P.extinctionRGB) * rho;
+    const V3 alb = v3(P.albedoRGB);
+    const V3 sa = (v3(1.0f) - alb) * ext;
+    const V3 ss = alb * ext;
+    const V3 sn = v3(W.mu) - ext;
+    float Pa = avg3(sa * st.weights), Ps = avg3(ss * st.weights), Pn = avg3(sn * st.weights);
+    const float C = Pa + Ps + Pn;
+    Pa /= C;
+    Ps /= C;
+    Pn /= C;
+    if (xi < Pa) {
+        writePath(out, st.unit, isNee<MODE>() ? st.color : v3(0.0f), true);
+        return false;
+    }
+    if (xi < Pa + Ps) {
+        if (P.maxBounces != 0u && (st.flags >> kBounceShift) >= P.maxBounces) {
+            writePath(out, st.unit, isNee<MODE>() ? st.color : v3(0.0f), true);
+            return false;
+        }
+        st.weights = st.weights * (ss / v3(W.mu * Ps));
+        // next-event estimation uses the weights before the clamp, as the reference's loop clamps at its end
+        if (!isNee<MODE>()) st.weights = vmin(st.weights, 100.0f);
+        const bool alive = scatterEvent<MODE>(st, W, P, S, p, 0.0f, out);
+        st.weights = vmin(st.weights, 100.0f);
+        return alive;
+    }
+    st.weights = vmin(st.weights * (sn / v3(W.mu * Pn)), 100.0f);
+    return true;
+}
+
+// One step of a live path. Returns false when the path finished (its radiance is written).
 template <int MODE, class Grid>
 CRVPT_DEV bool stepPath(PathState& st, Walker& W, const SceneParams& P, const SupervoxelGrid& S, const Grid& grid,
                         typename Grid::Cache& gc, float4* out) {
     const uint4 r = pathBlock(P, st, st.step++);
-    const int ev = walkAdvance(W, P, S, -__logf(uOpen(r.x)));
+    const int ev = walkAdvance<MODE>(W, P, S, -__logf(uOpen(r.x)));
     if (ev == WALK_MOVED) return true;
+    if (MODE == VPT_RESIDUAL_RATIO_TRACKING) {
+        st.thr *= __expf(-W.ctrl);  // control transmittance of the walked stretch
+        W.ctrl = 0.0f;
+        if (ev == WALK_END) return rrtTraversalEnd(st, W, P, S, out);
+        const float rho = grid.lookupWorldJ(P, W.o + W.w * W.t, jitterHi(r.y), jitterLo(r.y), jitterHi(r.z), gc);
+        st.taps++;
+        st.thr *= 1.0f - (P.extinction * rho - W.muC) * rcpf(W.mu);
+        st.wSum += st.thr;
+        if (u01(r.w) * st.wSum < st.thr) {
+            st.resT = st.thr;
+            st.resDist = W.t;
+        }
+        return true;
+    }
     if (ev == WALK_END) return endEvent<MODE>(st, W, P, S, out);
     const V3 p = W.o + W.w * W.t;
+    const float xi = u01(r.w);
+    if (MODE == VPT_DECOMPOSITION_TRACKING) {
+        // control collision (xi * mu < mu_c) needs no lookup; a residual one is real with prob (sigma rho - mu_c)/mu_r
+        const float xm = xi * W.mu;
+        if (!(xm < W.muC)) {
+            const float rho = grid.lookupWorldJ(P, p, jitterHi(r.y), jitterLo(r.y), jitterHi(r.z), gc);
+            st.taps++;
+            if (!(xm < P.extinction * rho)) return true;
+        }
+        if (P.maxBounces != 0u && (st.flags >> kBounceShift) >= P.maxBounces) {
+            writePath(out, st.unit, v3(0.0f), true);
+            return false;
+        }
+        return scatterEvent<MODE>(st, W, P, S, p, 0.0f, out);
+    }
     const float rho = grid.lookupWorldJ(P, p, jitterHi(r.y), jitterLo(r.y), jitterHi(r.z), gc);
     st.taps++;
     const float pReal = P.extinction * rho * rcpf(W.mu);
-    if (MODE == VPT_NEXT_EVENT_TRACKING && (st.flags & PF_SHADOW)) {
+    if (isSpectral<MODE>() && !(st.flags & PF_SHADOW)) return spectralCollision<MODE>(st, W, P, S, p, rho, xi, out);
+    if (isNee<MODE>() && (st.flags & PF_SHADOW)) {
         st.thr *= 1.0f - pReal;  // ratio tracking (NeeTransmittance.glsl:107-113)
         // Once the product is exactly 0 every later factor leaves it 0 and the shadow ray adds exactly 0.
         if (st.thr == 0.0f) return endEvent<MODE>(st, W, P, S, out);
         return true;
     }
-    const float xi = u01(r.w);
     if (!(xi < pReal)) return true;  // null collision
     const float pAbs = (1.0f - P.albedo) * pReal;
     const bool bounceLimit = P.maxBounces != 0u && (st.flags >> kBounceShift) >= P.maxBounces;
     if ((MODE != VPT_RATIO_TRACKING && xi < pAbs) || bounceLimit) {
         // absorption or a scattering event beyond the bounce limit: delta / ratio end with 0, next-event with its sum
-        writePath(out, st.unit, MODE == VPT_NEXT_EVENT_TRACKING ? st.color : v3(0.0f), true);
+        writePath(out, st.unit, isNee<MODE>() ? st.color : v3(0.0f), true);
         return false;
     }
     return scatterEvent<MODE>(st, W, P, S, p, pAbs / pReal, out);
@@ -368,9 +586,9 @@ CRVPT_DEV void flushCounters(unsigned long long* stats, uint32_t taps, uint32_t 
 }
 
 template <int MODE, class Grid>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CRVPT_PATHPOOL_WAVES))) void k_pathPool(SceneParams P, Grid grid, SupervoxelGrid S,
-                                                  float4* __restrict__ out, uint32_t* __restrict__ counter,
-                                                  int stepsPerRefill, unsigned long long* __restrict__ stats) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CRVPT_PATHPOOL_WAVES))) void k_pathPool(
+        SceneParams P, Grid grid, SupervoxelGrid S, float4* __restrict__ out, uint32_t* __restrict__ counter,
+        int stepsPerRefill, unsigned long long* __restrict__ stats) {
     const int lane = __lane_id();
     PathState st;
     st.flags = 0;
@@ -382,9 +600,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CRVPT_PATHP
     W.svReads = 0;
     typename Grid::Cache gc;
     while (true) {
-        const bool needs = !(st.flags & PF_ALIVE) && !exhausted;
-        const uint64_t needMask = __ballot(needs);
-        if (needMask) {
+        // Refill until every lane holds a live path or the queue is drained: a claimed path whose camera ray misses
+        // the volume finishes inside startPath, and its lane claims again before the wave steps.
+        while (true) {
+            const bool needs = !(st.flags & PF_ALIVE) && !exhausted;
+            const uint64_t needMask = __ballot(needs);
+            if (needMask == 0) break;
             const int leader = __ffsll((unsigned long long)needMask) - 1;
             uint32_t base = 0;
             if (lane == leader) base = atomicAdd(counter, uint32_t(__popcll(needMask)));
@@ -399,11 +620,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CRVPT_PATHP
                 }
             }
         }
-        const bool alive = (st.flags & PF_ALIVE) != 0;
-        if (__ballot(alive) == 0) {
-            if (__ballot(!exhausted) == 0) break;
-            continue;
-        }
+        if (__ballot((st.flags & PF_ALIVE) != 0) == 0) break;  // every lane is exhausted
         for (int k = 0; k < stepsPerRefill; k++) {
             if (st.flags & PF_ALIVE) {
                 if (!stepPath<MODE, Grid>(st, W, P, S, grid, gc, out)) st.flags = 0;
@@ -414,11 +631,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CRVPT_PATHP
 }
 
 // ------------------------------------------------------------------------------------------------------------
-// Decomposition tracking (DecompositionTracking.glsl:35-123) and residual ratio tracking
-// (ResidualRatioTracking.glsl:33-224) on the reference's supervoxel tables. One lane runs a whole path; lanes
-// claim paths from the same per-wave ballot queue.
+// Helpers of the batched residual-ratio and decomposition transmittance queries (reference traversal order).
 // ------------------------------------------------------------------------------------------------------------
-// Per-lane work counters of the whole-path and transmittance estimators (density taps, supervoxel table reads).
+// Per-lane work counters of the transmittance estimators (density taps, supervoxel table reads).
 struct WorkCount {
     uint32_t taps = 0;
     uint32_t svReads = 0;
@@ -436,77 +651,6 @@ CRVPT_DEV size_t refIndex(const SupervoxelGrid& S, int x, int y, int z) {
 CRVPT_DEV V3 gridDims(const SceneParams& P) {
     return v3(P.worldToTex[0] * (P.boxMax[0] - P.boxMin[0]), P.worldToTex[1] * (P.boxMax[1] - P.boxMin[1]),
               P.worldToTex[2] * (P.boxMax[2] - P.boxMin[2]));
-}
-
-template <class Grid>
-CRVPT_DEV V3 decompositionPath(const SceneParams& P, const SupervoxelGrid& S, const Grid& grid, Rng& rng, V3 x,
-                               V3 w, WorkCount& wc, typename Grid::Cache& gc) {
-    uint32_t bounces = 0;
-    const V3 eps = v3(1e-6f);
-    float tMinVal, tMaxVal;
-    if (rayBoxIntersect(v3(P.boxMin) + eps, v3(P.boxMax) - eps, x, w, tMinVal, tMaxVal)) {
-        const float majorant = P.extinction;
-        const float absorptionAlbedo = 1.0f - P.albedo;
-        const V3 dims = gridDims(P);
-        const V3 boxDelta = v3(P.boxMax) - v3(P.boxMin);
-        const V3 svSize = v3(float(S.refSize)) * boxDelta / dims;
-        x += w * tMinVal;
-        const V3 sp = (x - v3(P.boxMin)) / boxDelta * dims / v3(float(S.refSize));
-        int ix = int(floorf(sp.x)), iy = int(floorf(sp.y)), iz = int(floorf(sp.z));
-        while (refInside(S, ix, iy, iz)) {
-            const V3 minP = v3(P.boxMin) + v3(float(ix), float(iy), float(iz)) * svSize;
-            const V3 maxP = minP + svSize;
-            float a = 0.0f, b = 0.0f;
-            rayBoxIntersect(minP, maxP, x, w, a, b);
-            float dMax = b - a;
-            x += w * a;
-            const size_t si = refIndex(S, ix, iy, iz);
-            const float mmMin = S.decomp[2 * si], mmMax = S.decomp[2 * si + 1];
-            wc.svReads++;
-            if (mmMax < 1e-5f) {
-                x += w * dMax;
-            } else {
-                const float muC = fmaxf(0.0000000001f, majorant * mmMin);
-                const float muR = fmaxf(0.0000000001f, majorant * mmMax - muC);
-                float tc = rng.exponential() / muC;
-                float tr = 0.0f;
-                while (true) {
-                    tr += rng.exponential() / muR;
-                    if (tc >= dMax && tr >= dMax) {
-                        x = x + w * dMax;
-                        break;
-                    }
-                    const V3 xs = x + w * fminf(tc, tr);
-                    bool collide;
-                    if (tc <= tr) {
-                        collide = true;
-                    } else {
-                        const float rho = density(grid, P, rng, xs, gc);
-                        wc.taps++;
-                        collide = rng.uniform() * muR < P.extinction * rho - muC;
-                    }
-                    if (collide) {
-                        x = xs;
-                        if (rng.uniform() < absorptionAlbedo) return v3(0.0f);
-                        if (P.maxBounces != 0u && ++bounces > P.maxBounces) return v3(0.0f);
-                        w = samplePhase(rng, P.phaseG, w);
-                        tr = 0.0f;
-                        tc = rng.exponential() / muC;
-                        rayBoxIntersect(minP, maxP, x, w, a, b);
-                        dMax = b - a;
-                    }
-                }
-            }
-            const V3 cc = (minP + maxP) * 0.5f;
-            V3 mov = x + w * 0.00001f - cc;
-            const V3 sm = v3(signf(mov.x), signf(mov.y), signf(mov.z));
-            mov = mov * sm;
-            ix += ((mov.x >= mov.y && mov.x >= mov.z) ? 1 : 0) * int(sm.x);
-            iy += ((mov.y >= mov.x && mov.y >= mov.z) ? 1 : 0) * int(sm.y);
-            iz += ((mov.z >= mov.x && mov.z >= mov.y) ? 1 : 0) * int(sm.z);
-        }
-    }
-    return skyRadiance(P, w);
 }
 
 struct RefDda {
@@ -575,84 +719,6 @@ CRVPT_DEV float rrtEstimator(const SceneParams& P, const Grid& grid, Rng& rng, V
     return Tc * Tr;
 }
 
-template <class Grid>
-CRVPT_DEV V3 residualRatioPath(const SceneParams& P, const SupervoxelGrid& S, const Grid& grid, Rng& rng, V3 x,
-                               V3 w, WorkCount& wc, typename Grid::Cache& gc) {
-    float wSum = 0.0f, resT = 0.0f, resDist = 0.0f;
-    float T = 1.0f;
-    V3 acc = v3(0.0f);
-    const V3 eps = v3(1e-6f);
-    V3 oldX = x;
-    int iteration = 0;
-    while (true) {
-        float tMinVal, tMaxVal;
-        if (!rayBoxIntersect(v3(P.boxMin) + eps, v3(P.boxMax) - eps, x, w, tMinVal, tMaxVal)) break;
-        x += w * tMinVal;
-        oldX = x;
-        RefDda q;
-        if (!refDdaInit(P, S, x, w, tMaxVal - tMinVal, q)) break;
-        while (refInside(S, q.idx[0], q.idx[1], q.idx[2])) {
-            const size_t si = refIndex(S, q.idx[0], q.idx[1], q.idx[2]);
-            const float muC = S.rrt[2 * si], muR = S.rrt[2 * si + 1];
-            wc.svReads++;
-            V3 mn, mx;
-            refSvBounds(P, S, q.idx, mn, mx);
-            float a = 0.0f, b = 0.0f;
-            rayBoxIntersect(mn, mx, oldX, w, a, b);
-            x = oldX + w * a;
-            T *= rrtEstimator(P, grid, rng, x, w, a, b, T, wSum, resT, resDist, muC, muR, wc, gc);
-            refDdaAdvance(q);
-        }
-        const float xi = rng.uniform();
-        if (xi > wSum || iteration >= 10) break;
-        acc += skyRadiance(P, w) * T;
-        iteration++;
-        T = resT;
-        x = oldX + w * resDist;
-        w = samplePhase(rng, P.phaseG, w);
-        wSum = 0.0f;
-        resT = 0.0f;
-        resDist = 0.0f;
-    }
-    acc += skyRadiance(P, w) * T;
-    return acc;
-}
-
-template <int MODE, class Grid>
-__global__ __launch_bounds__(256) void k_wholePath(SceneParams P, Grid grid, SupervoxelGrid S,
-                                                   float4* __restrict__ out, uint32_t* __restrict__ counter,
-                                                   unsigned long long* __restrict__ stats) {
-    const int lane = __lane_id();
-    WorkCount wc;
-    typename Grid::Cache gc;
-    uint32_t paths = 0;
-    while (true) {
-        const uint64_t all = __ballot(true);
-        const int leader = __ffsll((unsigned long long)all) - 1;
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(counter, uint32_t(__popcll(all)));
-        base = __shfl(base, leader);
-        const uint32_t unit = base + uint32_t(__popcll(all & ((1ull << lane) - 1ull)));
-        if (__ballot(uint64_t(unit) < P.numUnits) == 0) break;
-        if (uint64_t(unit) < P.numUnits) {
-            PathState st;
-            st.unit = unit;
-            const uint32_t npix = uint32_t(P.width) * uint32_t(P.height);
-            st.pix = unit % npix;
-            st.frame = P.frameBase + unit / npix;
-            V3 x, w;
-            cameraRay(P, st, x, w);
-            Rng rng;
-            rng.init(st.pix, st.frame, P.seedOffset ^ 0x243F6A88u);
-            V3 c = MODE == VPT_DECOMPOSITION_TRACKING ? decompositionPath(P, S, grid, rng, x, w, wc, gc)
-                                                       : residualRatioPath(P, S, grid, rng, x, w, wc, gc);
-            out[unit] = make_float4(c.x, c.y, c.z, 1.0f);
-            paths++;
-        }
-    }
-    if (stats) flushCounters(stats, wc.taps, wc.svReads, paths, gc.descents());
-}
-
 // ------------------------------------------------------------------------------------------------------------
 // Accumulation: running mean over all frames since the last reset (Clouds.glsl:172-179), summed per pixel in
 // sample order, then written as [C, H, W] float32 (alpha = 1) when requested.
@@ -706,7 +772,7 @@ template <class Grid>
 CRVPT_DEV float transmittanceWalker(const SceneParams& P, const SupervoxelGrid& S, const Grid& grid, Rng& rng, V3 o,
                                     V3 d, float limit, bool binary) {
     Walker W;
-    if (!walkerStart(W, P, S, o, d, limit)) return 1.0f;
+    if (!walkerStart<VPT_DELTA_TRACKING>(W, P, S, o, d, limit)) return 1.0f;
     typename Grid::Cache gc;
     float T = 1.0f;
     while (true) {
@@ -1009,13 +1075,21 @@ static hipError_t launchRenderT(const SceneParams& P, const Grid& grid, const Su
             hipLaunchKernelGGL((k_pathPool<VPT_NEXT_EVENT_TRACKING, Grid>), gridDim, block, 0, stream, P, grid, S,
                                paths, counter, cfg.stepsPerRefill, stats);
             break;
+        case VPT_SPECTRAL_DELTA_TRACKING:
+            hipLaunchKernelGGL((k_pathPool<VPT_SPECTRAL_DELTA_TRACKING, Grid>), gridDim, block, 0, stream, P, grid,
+                               S, paths, counter, cfg.stepsPerRefill, stats);
+            break;
+        case VPT_NEXT_EVENT_TRACKING_SPECTRAL:
+            hipLaunchKernelGGL((k_pathPool<VPT_NEXT_EVENT_TRACKING_SPECTRAL, Grid>), gridDim, block, 0, stream, P,
+                               grid, S, paths, counter, cfg.stepsPerRefill, stats);
+            break;
         case VPT_DECOMPOSITION_TRACKING:
-            hipLaunchKernelGGL((k_wholePath<VPT_DECOMPOSITION_TRACKING, Grid>), gridDim, block, 0, stream, P, grid,
-                               S, paths, counter, stats);
+            hipLaunchKernelGGL((k_pathPool<VPT_DECOMPOSITION_TRACKING, Grid>), gridDim, block, 0, stream, P, grid,
+                               S, paths, counter, cfg.stepsPerRefill, stats);
             break;
         case VPT_RESIDUAL_RATIO_TRACKING:
-            hipLaunchKernelGGL((k_wholePath<VPT_RESIDUAL_RATIO_TRACKING, Grid>), gridDim, block, 0, stream, P,
-                               grid, S, paths, counter, stats);
+            hipLaunchKernelGGL((k_pathPool<VPT_RESIDUAL_RATIO_TRACKING, Grid>), gridDim, block, 0, stream, P,
+                               grid, S, paths, counter, cfg.stepsPerRefill, stats);
             break;
         default:
             hipLaunchKernelGGL((k_pathPool<VPT_DELTA_TRACKING, Grid>), gridDim, block, 0, stream, P, grid, S, paths,
@@ -1123,22 +1197,24 @@ hipError_t launchNanoLookupWorld(const NanoGrid& nano, int n, const float* xyz, 
     return hipGetLastError();
 }
 
+template <class Grid>
+static const void* pathPoolKernel(int mode) {
+    switch (mode) {
+        case VPT_RATIO_TRACKING: return (const void*)k_pathPool<VPT_RATIO_TRACKING, Grid>;
+        case VPT_NEXT_EVENT_TRACKING: return (const void*)k_pathPool<VPT_NEXT_EVENT_TRACKING, Grid>;
+        case VPT_SPECTRAL_DELTA_TRACKING: return (const void*)k_pathPool<VPT_SPECTRAL_DELTA_TRACKING, Grid>;
+        case VPT_NEXT_EVENT_TRACKING_SPECTRAL:
+            return (const void*)k_pathPool<VPT_NEXT_EVENT_TRACKING_SPECTRAL, Grid>;
+        case VPT_DECOMPOSITION_TRACKING: return (const void*)k_pathPool<VPT_DECOMPOSITION_TRACKING, Grid>;
+        case VPT_RESIDUAL_RATIO_TRACKING: return (const void*)k_pathPool<VPT_RESIDUAL_RATIO_TRACKING, Grid>;
+        default: return (const void*)k_pathPool<VPT_DELTA_TRACKING, Grid>;
+    }
+}
+
 int occupancyBlocksPerCU(int mode, bool sparse) {
     int blocks = 0;
-    hipError_t e;
-    if (sparse) {
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &blocks,
-            mode == VPT_NEXT_EVENT_TRACKING ? (const void*)k_pathPool<VPT_NEXT_EVENT_TRACKING, NanoAccess>
-                                            : (const void*)k_pathPool<VPT_DELTA_TRACKING, NanoAccess>,
-            256, 0);
-    } else {
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &blocks,
-            mode == VPT_NEXT_EVENT_TRACKING ? (const void*)k_pathPool<VPT_NEXT_EVENT_TRACKING, DenseAccess>
-                                            : (const void*)k_pathPool<VPT_DELTA_TRACKING, DenseAccess>,
-            256, 0);
-    }
+    const void* k = sparse ? pathPoolKernel<NanoAccess>(mode) : pathPoolKernel<DenseAccess>(mode);
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k, 256, 0);
     if (e != hipSuccess || blocks <= 0) return 4;
     return blocks;
 }

@@ -92,6 +92,9 @@ struct SceneParams {
     float gridMax[3];
     float extinction;       // extinction.x = majorant of the reference trackers
     float albedo;           // scatteringAlbedo.x
+    float extinctionRGB[3]; // extinction = extinctionBase * extinctionScale (spectral trackers)
+    float albedoRGB[3];     // scatteringAlbedo (spectral trackers)
+    float extinctionMax;    // maxComponent(extinction): majorant of the spectral trackers
     float phaseG;
     int32_t mode;
     int32_t interpolation;

@@ -31,7 +31,7 @@ __all__ = [
     "get_density_field", "get_nanovdb_buffer", "generate_fbm_cloud", "transmittance_batch",
     "set_use_majorant_grid", "set_launch_config", "get_last_render_stats", "get_num_lights",
     "set_collect_stats", "get_last_render_counters", "build_nanovdb_from_dense", "grid_lookup",
-    "grid_lookup_world", "get_supervoxel_tables", "set_max_bounces",
+    "grid_lookup_world", "get_supervoxel_tables", "set_max_bounces", "set_super_voxel_size",
     "VPT_MODE_NAMES", "CrvptError",
 ]
 
@@ -231,6 +231,12 @@ def set_headlight_intensity(_headlight_intensity: float) -> None:
 
 def set_seed_offset(offset: int) -> None:
     call("crvpt_set_seed_offset", int(offset))
+
+
+def set_super_voxel_size(super_voxel_size: int) -> None:
+    """Pass setting superVoxelSize (VolumetricPathTracingPass.hpp:254): edge of the residual-ratio and decomposition
+    supervoxel tables, 1-64, default 8."""
+    call("crvpt_set_super_voxel_size", int(super_voxel_size))
 
 
 def set_max_bounces(max_bounces: int) -> None:

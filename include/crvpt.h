@@ -81,6 +81,10 @@ int crvpt_set_extinction_scale(double extinction_scale);
 int crvpt_set_extinction_base(const double extinction_base[3]);
 int crvpt_set_phase_g(double phase_g);
 int crvpt_set_seed_offset(int64_t offset);
+/* Pass setting superVoxelSize (VolumetricPathTracingPass.hpp:254, GUI slider VolumetricPathTracingPass.cpp:2569,
+ * range 1-64): edge of the supervoxel tables of residual ratio and decomposition tracking (SuperVoxelGrid.cpp:40-60;
+ * halved until it divides grids smaller than the edge). */
+int crvpt_set_super_voxel_size(int32_t super_voxel_size);
 /* Scattering events per path before the path ends with zero further radiance (0 = unbounded). No counterpart in
  * the reference, whose trackers never cut a path (DeltaTracking.glsl:167-345 loops until the path leaves or is
  * absorbed); it bounds path length for benchmark workloads that state a maximum bounce count. */

@@ -18,7 +18,8 @@ import numpy as np
 ORACLE_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(ORACLE_DIR, "liboracle.so")
 
-MODES = {"delta": 0, "ratio": 2, "decomposition": 3, "residual_ratio": 4, "next_event": 5}
+MODES = {"delta": 0, "delta_spectral": 1, "ratio": 2, "decomposition": 3, "residual_ratio": 4, "next_event": 5,
+         "next_event_spectral": 6}
 INTERP = {"nearest": 0, "stochastic": 1, "trilinear": 2}
 ENVS = {"default": 0, "black": 1, "single_color": 2, "image": 3}
 T_ESTIMATORS = {"delta": 0, "ratio": 1, "residual_ratio": 2, "decomposition": 3}

@@ -11,6 +11,8 @@
  *   Data/Shaders/VPT/DeltaTracking.glsl:167-345     deltaTracking
  *   Data/Shaders/VPT/RatioTracking.glsl:25-96       ratioTracking (albedo-weighted analog variant)
  *   Data/Shaders/VPT/NextEventTracking.glsl:223-394 nextEventTracking
+ *   Data/Shaders/VPT/DeltaTracking.glsl:33-164      deltaTrackingSpectral (path-history average probabilities)
+ *   Data/Shaders/VPT/NextEventTracking.glsl:30-221  nextEventTrackingSpectral
  *   Data/Shaders/VPT/NeeTransmittance.glsl:4-119,263-382 calculateTransmittance(Distance) (ratio tracking)
  *   Data/Shaders/VPT/DecompositionTracking.glsl:35-123 analogDecompositionTracking (dense)
  *   Data/Shaders/VPT/ResidualRatioTracking.glsl:33-224 residualRatioTracking (dense)
@@ -515,6 +517,148 @@ vec3 nextEventTracking(const Ctx& c, Rng& rng, vec3 x, vec3 w) {  // NextEventTr
     return color;
 }
 
+/* ---------------- Spectral trackers (Kutz et al. 2017; path-history average probabilities) ---------------- */
+inline float maxComponent(vec3 v) { return std::max(std::max(v.x, v.y), v.z); }
+inline float avgComponent(vec3 v) { return (v.x + v.y + v.z) / 3.0f; }
+inline vec3 min3(vec3 a, float b) { return {std::min(a.x, b), std::min(a.y, b), std::min(a.z, b)}; }
+
+vec3 deltaTrackingSpectral(const Ctx& c, Rng& rng, vec3 x, vec3 w) {  // DeltaTracking.glsl:33-164
+    const OracleScene* s = c.sc;
+    const vec3 extinction(s->extinction), albedo(s->scattering_albedo);
+    float majorant = maxComponent(extinction);
+    vec3 weights(1.0f);
+    vec3 absorptionAlbedo = vec3(1.0f) - albedo;
+    vec3 scatteringAlbedo = albedo;
+    int bounces = 0;
+    float tMin, tMax;
+    if (rayBoxIntersect(c.boxMin, c.boxMax, x, w, tMin, tMax)) {
+        x += w * tMin;
+        float d = tMax - tMin;
+        while (true) {
+            float t = freePath(rng, majorant);
+            if (t > d) break;
+            vec3 xNew = x + w * t;
+            float density = sampleCloud(c, rng, xNew);
+            x = xNew;
+            vec3 sigma_a = absorptionAlbedo * extinction * density;
+            vec3 sigma_s = scatteringAlbedo * extinction * density;
+            vec3 sigma_n = vec3(majorant) - extinction * density;
+            float Pa = avgComponent(sigma_a * weights);
+            float Ps = avgComponent(sigma_s * weights);
+            float Pn = avgComponent(sigma_n * weights);
+            float C = Pa + Ps + Pn;
+            Pa /= C;
+            Ps /= C;
+            Pn /= C;
+            float xi = rng.random();
+            if (xi < Pa) return vec3(0.0f);
+            if (xi < Pa + Ps) {
+                if (s->max_bounces > 0 && ++bounces > s->max_bounces) return vec3(0.0f);
+                float pdf_w;
+                w = importanceSamplePhase(rng, s->phase_g, w, pdf_w);
+                if (rayBoxIntersect(c.boxMin, c.boxMax, x, w, tMin, tMax)) {
+                    x += w * tMin;
+                    d = tMax - tMin;
+                }
+                weights = weights * (sigma_s / (majorant * Ps));
+            } else {
+                d -= t;
+                weights = weights * (sigma_n / (majorant * Pn));
+            }
+            weights = min3(weights, 100.0f);
+        }
+    }
+    return min3(weights, 100000.0f) * (sampleSkybox(c, w) + sampleLight(c, w));
+}
+
+vec3 nextEventTrackingSpectral(const Ctx& c, Rng& rng, vec3 x, vec3 w) {  // NextEventTracking.glsl:30-221
+    const OracleScene* s = c.sc;
+    const vec3 extinction(s->extinction), albedo(s->scattering_albedo);
+    float majorant = maxComponent(extinction);
+    vec3 weights(1.0f);
+    vec3 absorptionAlbedo = vec3(1.0f) - albedo;
+    vec3 scatteringAlbedo = albedo;
+    vec3 color(0.0f);
+    bool hasValue = false;
+    int bounces = 0;
+    float tMin, tMax;
+    if (rayBoxIntersect(c.boxMin, c.boxMax, x, w, tMin, tMax)) {
+        x += w * tMin;
+        float d = tMax - tMin;
+        while (true) {
+            float t = freePath(rng, majorant);
+            if (t > d) break;
+            vec3 xNew = x + w * t;
+            float density = sampleCloud(c, rng, xNew);
+            x = xNew;
+            vec3 sigma_a = absorptionAlbedo * extinction * density;
+            vec3 sigma_s = scatteringAlbedo * extinction * density;
+            vec3 sigma_n = vec3(majorant) - extinction * density;
+            float Pa = avgComponent(sigma_a * weights);
+            float Ps = avgComponent(sigma_s * weights);
+            float Pn = avgComponent(sigma_n * weights);
+            float C = Pa + Ps + Pn;
+            Pa /= C;
+            Ps /= C;
+            Pn /= C;
+            float xi = rng.random();
+            if (xi < Pa) return color;
+            if (xi < Pa + Ps) {
+                if (s->max_bounces > 0 && ++bounces > s->max_bounces) return color;
+                float pdf_w;
+                vec3 next_w = importanceSamplePhase(rng, s->phase_g, w, pdf_w);
+                hasValue = true;
+                float pdfLightNee = 0.0f;
+                vec3 dirLightNee;
+                bool useLights = s->num_lights > 0;
+                bool isSamplingHeadlight = false;
+                float lightProbabilityFactor = 1.0f, lightDistance = 0.0f;
+                uint32_t lightIdx = 0;
+                if (useLights) {
+                    isSamplingHeadlight = (s->is_env_map_black != 0) ? true : (rng.random() > 0.5f);
+                    lightProbabilityFactor = s->is_env_map_black != 0 ? 1.0f : 2.0f;
+                    if (isSamplingHeadlight) {
+                        dirLightNee = getHeadlightDirection(c, rng, x, lightIdx, lightProbabilityFactor, lightDistance);
+                    } else {
+                        dirLightNee = importanceSampleSkybox(rng, pdfLightNee);
+                    }
+                } else {
+                    dirLightNee = importanceSampleSkybox(rng, pdfLightNee);
+                }
+                float pdf_nee_phase = evaluatePhase(s->phase_g, w, dirLightNee);
+                w = next_w;
+                weights = weights * (sigma_s / (majorant * Ps));
+                if (useLights) {
+                    vec3 commonFactor = (lightProbabilityFactor * pdf_nee_phase) * min3(weights, 100000.0f);
+                    if (isSamplingHeadlight) {
+                        color += commonFactor *
+                                 calculateTransmittanceDistance(c, rng, x, dirLightNee, lightDistance, true) *
+                                 sampleHeadlight(c, x, lightIdx);
+                    } else {
+                        color += commonFactor / pdfLightNee *
+                                 calculateTransmittanceDistance(c, rng, x, dirLightNee, 0.0f, false) *
+                                 (sampleSkybox(c, dirLightNee) + sampleLight(c, dirLightNee));
+                    }
+                } else {
+                    color += (pdf_nee_phase / pdfLightNee *
+                              calculateTransmittanceDistance(c, rng, x, dirLightNee, 0.0f, false)) *
+                             min3(weights, 100000.0f) * (sampleSkybox(c, dirLightNee) + sampleLight(c, dirLightNee));
+                }
+                if (rayBoxIntersect(c.boxMin, c.boxMax, x, w, tMin, tMax)) {
+                    x += w * tMin;
+                    d = tMax - tMin;
+                }
+            } else {
+                d -= t;
+                weights = weights * (sigma_n / (majorant * Pn));
+            }
+            weights = min3(weights, 100.0f);
+        }
+    }
+    if (!hasValue) color += min3(weights, 100000.0f) * (sampleSkybox(c, w) + sampleLight(c, w));
+    return color;
+}
+
 inline int svIndex(const Ctx& c, int x, int y, int z) { return x + (y + z * c.svGrid[1]) * c.svGrid[0]; }
 inline bool svInside(const Ctx& c, int x, int y, int z) {
     return x >= 0 && y >= 0 && z >= 0 && x < c.svGrid[0] && y < c.svGrid[1] && z < c.svGrid[2];
@@ -798,6 +942,8 @@ vec3 tracePath(const Ctx& c, Rng& rng, vec3 x, vec3 w) {
         case OR_DECOMPOSITION: return analogDecompositionTracking(c, rng, x, w);
         case OR_RESIDUAL_RATIO: return residualRatioTracking(c, rng, x, w);
         case OR_NEXT_EVENT: return nextEventTracking(c, rng, x, w);
+        case OR_DELTA_SPECTRAL: return deltaTrackingSpectral(c, rng, x, w);
+        case OR_NEXT_EVENT_SPECTRAL: return nextEventTrackingSpectral(c, rng, x, w);
         default: return deltaTracking(c, rng, x, w);
     }
 }

@@ -21,7 +21,8 @@ extern "C" {
 #endif
 
 /* VptMode (RenderSettings.hpp:32-36) subset on the hot path. */
-enum { OR_DELTA = 0, OR_RATIO = 2, OR_DECOMPOSITION = 3, OR_RESIDUAL_RATIO = 4, OR_NEXT_EVENT = 5 };
+enum { OR_DELTA = 0, OR_DELTA_SPECTRAL = 1, OR_RATIO = 2, OR_DECOMPOSITION = 3, OR_RESIDUAL_RATIO = 4, OR_NEXT_EVENT = 5,
+       OR_NEXT_EVENT_SPECTRAL = 6 };
 /* GridInterpolationType (RenderSettings.hpp:52-56). */
 enum { OR_NEAREST = 0, OR_STOCHASTIC = 1, OR_TRILINEAR = 2 };
 /* Environment kinds: BuiltinEnvMap DEFAULT/BLACK/SINGLE_COLOR (RenderSettings.hpp:118-121) + equirect image. */

@@ -136,6 +136,49 @@ def test_render_vs_oracle(vpt, oracle, name, mode, interp):
     _check_equal(m_gpu, s_gpu, m_ref, s_ref, f"{name}/{interp}")
 
 
+@pytest.mark.parametrize("name,mode", [("delta_spectral", 1), ("next_event_spectral", 6)])
+@pytest.mark.parametrize("majorant_grid", [True, False])
+def test_spectral_vs_oracle(vpt, oracle, name, mode, majorant_grid):
+    """Spectral delta / next-event tracking with chromatic extinction and albedo. The oracle uses the reference's
+    global majorant; the GPU walks the majorant grid (scaled by the largest extinction component) or, with the grid
+    off, the same global majorant."""
+    print(f"case: spectral {name} majorant_grid={majorant_grid}", flush=True)
+    d = oracle.fbm_cloud(24, 20, 16, seed=6)
+    _set_volume(vpt, d)
+    _common(vpt, mode, ext=96.0, g=0.5)
+    vpt.set_extinction_base([1.0, 0.7, 0.4])
+    vpt.set_scattering_albedo([0.95, 0.8, 0.6])
+    vpt.set_use_majorant_grid(majorant_grid)
+    try:
+        m_gpu, s_gpu, _ = _render_stats(vpt, SPP)
+    finally:
+        vpt.set_use_majorant_grid(True)
+        vpt.set_extinction_base([1.0, 1.0, 1.0])
+    sc = oracle.Scene(d, mode=name, extinction=(96.0, 96.0 * 0.7, 96.0 * 0.4), albedo=(0.95, 0.8, 0.6), g=0.5,
+                      width=RES, height=RES)
+    m_ref, s_ref = _oracle_stats(sc, SPP)
+    _check_equal(m_gpu, s_gpu, m_ref, s_ref, f"{name}/grid {majorant_grid}")
+
+
+@pytest.mark.parametrize("name,mode", [("decomposition", 3), ("residual_ratio", 4)])
+@pytest.mark.parametrize("sv", [4, 8])
+def test_supervoxel_estimators_vs_oracle(vpt, oracle, name, mode, sv):
+    """Decomposition and residual ratio tracking on a volume spanning many supervoxels (denser medium, two table
+    edges), against the oracle's restatement of the reference's supervoxel traversal."""
+    print(f"case: supervoxel estimator {name} edge={sv}", flush=True)
+    d = oracle.fbm_cloud(40, 36, 32, seed=17)
+    _set_volume(vpt, d)
+    _common(vpt, mode, ext=250.0, g=0.85)
+    vpt.set_super_voxel_size(sv)
+    try:
+        m_gpu, s_gpu, _ = _render_stats(vpt, SPP)
+    finally:
+        vpt.set_super_voxel_size(8)
+    sc = oracle.Scene(d, mode=name, extinction=250.0, g=0.85, width=RES, height=RES, super_voxel_size=sv)
+    m_ref, s_ref = _oracle_stats(sc, SPP)
+    _check_equal(m_gpu, s_gpu, m_ref, s_ref, f"{name}/edge {sv}")
+
+
 @pytest.mark.parametrize("name,mode", [("delta", 0), ("ratio", 2), ("next_event", 5), ("decomposition", 3)])
 def test_max_bounces_vs_oracle(vpt, oracle, name, mode):
     print(f"case: max bounces 1 mode={name}", flush=True)

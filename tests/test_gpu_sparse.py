@@ -53,10 +53,12 @@ def test_coherent_lookups_through_the_memo(vpt, oracle):
 
 @pytest.mark.parametrize("interp", ["nearest", "stochastic"])
 @pytest.mark.parametrize("sparse", [False, True])
-def test_supervoxel_tables_bit_exact(vpt, oracle, interp, sparse):
-    print(f"case: supervoxel tables interp={interp} sparse={sparse}", flush=True)
+@pytest.mark.parametrize("sv", [8, 16])
+def test_supervoxel_tables_bit_exact(vpt, oracle, interp, sparse, sv):
+    print(f"case: supervoxel tables interp={interp} sparse={sparse} edge={sv}", flush=True)
     d = oracle.fbm_cloud(44, 36, 28, seed=12)
     vpt.set_density_field(d)
+    vpt.set_super_voxel_size(sv)
     vpt.set_grid_interpolation_type(interp)
     vpt.set_extinction_scale(300.0)
     vpt.set_extinction_base([1.0, 1.0, 1.0])
@@ -67,11 +69,12 @@ def test_supervoxel_tables_bit_exact(vpt, oracle, interp, sparse):
     if sparse:
         nz_, ny_, nx_ = np.nonzero(norm != 0)
         crop = norm[nz_.min():nz_.max() + 1, ny_.min():ny_.max() + 1, nx_.min():nx_.max() + 1]
-        sc = oracle.Scene(crop, interpolation=interp, extinction=300.0, voxel_range=(0.0, 1.0))
+        sc = oracle.Scene(crop, interpolation=interp, extinction=300.0, voxel_range=(0.0, 1.0), super_voxel_size=sv)
     else:
-        sc = oracle.Scene(d, interpolation=interp, extinction=300.0)
+        sc = oracle.Scene(d, interpolation=interp, extinction=300.0, super_voxel_size=sv)
     osv, og, orrt, odec, _ = sc.supervoxel_tables()
     vpt.set_use_sparse_grid(False)
+    vpt.set_super_voxel_size(8)
     assert (sv, g) == (osv, og)
     np.testing.assert_array_equal(dec.view(np.uint32), odec.view(np.uint32))
     np.testing.assert_array_equal(rrt.view(np.uint32), orrt.view(np.uint32))

@@ -293,3 +293,32 @@ def test_max_bounces_truncates_paths(oracle, mode):
         # running means in FP32: allow the last-ulp difference of the 1/(frame+1) mix
         assert np.all(one <= two * (1 + 1e-6) + 1e-7)
         assert np.all(two <= free * (1 + 1e-6) + 1e-7)
+
+
+@pytest.mark.parametrize("spectral,scalar", [("delta_spectral", "delta"), ("next_event_spectral", "next_event")])
+def test_spectral_with_grey_medium_is_the_scalar_tracker(oracle, spectral, scalar):
+    """With grey extinction and albedo every spectral weight update is sigma / (majorant * P) = 1 and the collision
+    probabilities are the scalar tracker's, so both consume the same random numbers and agree pixel for pixel up to
+    FP32 rounding of the normalised probabilities."""
+    d = oracle.fbm_cloud(16, 16, 16, seed=23)
+    kw = dict(extinction=80.0, albedo=0.8, g=0.5, width=24, height=24)
+    a, _ = oracle.Scene(d, mode=spectral, **kw).render(16)
+    b, _ = oracle.Scene(d, mode=scalar, **kw).render(16)
+    close = np.abs(a - b) <= 1e-4 * np.maximum(1.0, np.abs(b))
+    assert close.mean() > 0.99, close.mean()
+    np.testing.assert_allclose(a.mean(), b.mean(), rtol=2e-3)
+
+
+def test_spectral_white_furnace_per_channel(oracle):
+    """Albedo 1 with a constant sky: each channel of spectral tracking is an unbiased estimate of the sky radiance,
+    whatever the chromatic extinction (Kutz et al. 2017)."""
+    d = oracle.fbm_cloud(16, 16, 16, seed=24)
+    sc = oracle.Scene(d, mode="delta_spectral", extinction=(120.0, 80.0, 40.0), albedo=1.0, g=0.3, env="single_color",
+                      width=32, height=32)
+    img, sq = sc.render(64)
+    n = 32 * 32 * 64
+    mean = img.reshape(-1, 3).mean(0).astype(np.float64)
+    var = np.maximum(sq.reshape(-1, 3).sum(0) / n - mean ** 2, 1e-12)
+    sky = np.array([2.6, 2.6 * 0.961538462, 2.6 * 0.884615385])
+    z = (mean - sky) / np.sqrt(var / n)
+    assert np.all(np.abs(z) < 4.0), (mean, sky, z)
