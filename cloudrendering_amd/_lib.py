@@ -49,6 +49,10 @@ SIGNATURES = {
     "crvpt_set_seed_offset": (_c_int, [_c_i64]),
     "crvpt_set_max_bounces": (_c_int, [_c_int]),
     "crvpt_set_super_voxel_size": (_c_int, [_c_int]),
+    "crvpt_set_use_feature_maps": (_c_int, [ctypes.c_void_p, _c_int]),
+    "crvpt_set_feature_map_type": (_c_int, [_c_int]),
+    "crvpt_get_feature_map": (_c_int, [_c_int, ctypes.c_void_p, _c_int, _c_int, _c_int, _c_int, ctypes.c_void_p]),
+    "crvpt_set_view_projection_matrix_as_previous": (_c_int, []),
     "crvpt_get_camera_position": (_c_int, [_pd]),
     "crvpt_get_camera_view_matrix": (_c_int, [_pd]),
     "crvpt_get_camera_fovy": (_c_int, [_pd]),

@@ -182,6 +182,12 @@ struct Renderer {
     double maxGridExtent = 0.25;
     bool useSparse = false;
     int denoiser = 0;  // 0 none, 1 EAW, 2 SVGF
+    // feature maps (VolumetricPathTracingPass::setUseFeatureMaps / setFeatureMapType)
+    std::vector<int> featureMaps;
+    int featureMapType = 0;          // map render_frame returns (0: Result)
+    bool prevViewProjSet = false;    // setViewProjectionMatrixAsPrevious was called
+    float prevViewProj[16] = {};
+    int featW = 0, featH = 0;        // resolution of the feature state (0: none)
     int eawIterations = 0;
     float eawPhiColor = 5.0f, eawPhiPosition = 0.1f, eawPhiNormal = 0.1f;
     int eawUseColorDistance = 0;
@@ -207,7 +213,7 @@ struct Renderer {
     // ---- device state ----
     hipStream_t stream = nullptr;
     DevBuf dRaw, dNorm, dNvdb, dMajorant, dRrt, dDecomp, dEnv, dAcc, dPaths, dCounter, dStage, dRays;
-    DevBuf dSvMin, dSvMax, dSvAvg, dDen0, dDen1, dStats;
+    DevBuf dSvMin, dSvMax, dSvAvg, dDen0, dDen1, dStats, dFeat, dFeatAcc;
     bool sceneDirty = true;     // grid contents changed
     bool tablesDirty = true;    // supervoxel tables must be rebuilt
     bool envDirty = true;
@@ -507,6 +513,21 @@ int prepareScene(Renderer& r) {
 
 Mat4 projection(const Renderer& r, int w, int h) { return perspective(r.fovy, double(w) / double(h), r.znear, r.zfar); }
 
+// Feature maps are traced whenever a map is requested, returned by render_frame, or read by the EAW denoiser
+// (COLOR + POSITION + NORMAL, EAWDenoiser.cpp:67-74).
+bool featuresActive(const Renderer& r) {
+    return !r.featureMaps.empty() || r.featureMapType != 0 || (r.denoiser == 1 && r.eawIterations > 0);
+}
+
+FeatureAcc featureAcc(Renderer& r, uint32_t npix) {
+    float4* b = r.dFeatAcc.as<float4>();
+    FeatureAcc a;
+    float4** slots[kFeatureAccArrays] = {&a.firstX, &a.firstW, &a.normal, &a.cloudOnly, &a.background,
+                                         &a.albedo, &a.depth, &a.dens, &a.flow, &a.flowRev};
+    for (int i = 0; i < kFeatureAccArrays; i++) *slots[i] = b + size_t(i) * npix;
+    return a;
+}
+
 void fillSceneParams(Renderer& r, int w, int h, SceneParams& P, SupervoxelGrid& S) {
     std::memset(&P, 0, sizeof(P));
     const Mat4 vp = mul(projection(r, w, h), r.view);
@@ -515,7 +536,10 @@ void fillSceneParams(Renderer& r, int w, int h, SceneParams& P, SupervoxelGrid& 
     for (int i = 0; i < 16; i++) {
         P.invViewProj[i] = float(ivp.m[i]);
         P.invView[i] = float(iv.m[i]);
+        // no previous matrix set: the current one (VolumetricPathTracingPass.cpp:1985-1989)
+        P.prevViewProj[i] = r.prevViewProjSet ? r.prevViewProj[i] : float(vp.m[i]);
     }
+    P.farDistance = float(r.zfar);
     const bool sparse = sparseActive(r);
     for (int a = 0; a < 3; a++) {
         P.boxMin[a] = float(sparse ? r.boxMinSparse[a] : r.boxMinDense[a]);
@@ -672,7 +696,7 @@ int crvpt_cleanup(void) {
         (void)hipStreamSynchronize(r->stream);
         for (DevBuf* b : {&r->dRaw, &r->dNorm, &r->dNvdb, &r->dMajorant, &r->dRrt, &r->dDecomp, &r->dEnv, &r->dAcc,
                           &r->dPaths, &r->dCounter, &r->dStage, &r->dRays, &r->dSvMin, &r->dSvMax, &r->dSvAvg,
-                          &r->dDen0, &r->dDen1, &r->dStats})
+                          &r->dDen0, &r->dDen1, &r->dStats, &r->dFeat, &r->dFeatAcc})
             b->release();
         (void)hipEventDestroy(r->ev0);
         (void)hipEventDestroy(r->ev1);
@@ -729,10 +753,22 @@ int crvpt_render_frame(float* out, int channels, int height, int width, int fram
     cfg.numBlocks = uint32_t(launchBlocks(r));
     cfg.stepsPerRefill = r.stepsPerRefill;
     const bool denoise = r.denoiser == 1 && r.eawIterations > 0;
-    float* directOut = (out_on_device && !denoise) ? out : nullptr;
+    const bool features = featuresActive(r);
+    float4* feat = nullptr;
+    FeatureAcc facc{};
+    if (features) {
+        HIP_TRY(r.dFeat.ensure(size_t(npix) * framesPerLaunch * 3 * sizeof(float4)));
+        HIP_TRY(r.dFeatAcc.ensure(size_t(npix) * kFeatureAccArrays * sizeof(float4)));
+        feat = r.dFeat.as<float4>();
+        facc = featureAcc(r, npix);
+        r.featW = width;
+        r.featH = height;
+    }
+    const bool lateOut = denoise || r.featureMapType != 0;  // the output is written after the sample loop
+    float* directOut = (out_on_device && !lateOut) ? out : nullptr;
     if (!out_on_device) {
         HIP_TRY(r.dStage.ensure(size_t(npix) * channels * sizeof(float)));
-        if (!denoise) directOut = r.dStage.as<float>();
+        if (!lateOut) directOut = r.dStage.as<float>();
     }
     int launches = 0;
     HIP_TRY(hipEventRecord(r.evT0, s));
@@ -744,16 +780,18 @@ int crvpt_render_frame(float* out, int channels, int height, int width, int fram
         P.numFrames = f;
         P.numUnits = uint64_t(npix) * f;
         HIP_TRY(launchRender(P, sp ? nullptr : &dg, sp ? &ng : nullptr, S, r.dPaths.as<float4>(),
-                             r.dCounter.as<uint32_t>(), cfg, stats, s));
+                             r.dCounter.as<uint32_t>(), cfg, stats, feat, s));
         const bool last = done + f == uint32_t(frame_count);
         if (last) HIP_TRY(hipEventRecord(r.ev1, s));
         HIP_TRY(launchAccumulate(r.dPaths.as<float4>(), npix, f, r.dAcc.as<float4>(), r.frameCount + done,
                                  last ? directOut : nullptr, channels, s));
+        if (features)
+            HIP_TRY(launchAccumulateFeatures(P, r.dPaths.as<float4>(), feat, npix, f, r.frameCount + done, facc, s));
         done += f;
         launches++;
     }
     if (frame_count == 0) HIP_TRY(hipEventRecord(r.ev1, s));
-    if (frame_count == 0 || denoise) {
+    if (frame_count == 0 || lateOut) {
         float* target = out_on_device ? out : r.dStage.as<float>();
         if (frame_count == 0 && r.frameCount == 0) {
             HIP_TRY(hipMemsetAsync(r.dAcc.ptr, 0, size_t(npix) * sizeof(float4), s));
@@ -764,9 +802,11 @@ int crvpt_render_frame(float* out, int channels, int height, int width, int fram
             EawParams ep{width, height, r.eawIterations, r.eawPhiColor, r.eawPhiPosition, r.eawPhiNormal,
                          r.eawUseColorDistance};
             float4* res = nullptr;
-            HIP_TRY(launchEaw(r.dAcc.as<float4>(), nullptr, nullptr, r.dDen0.as<float4>(), r.dDen1.as<float4>(), ep,
-                              &res, s));
+            HIP_TRY(launchEaw(r.dAcc.as<float4>(), facc.firstX, facc.normal, r.dDen0.as<float4>(),
+                              r.dDen1.as<float4>(), ep, &res, s));
             HIP_TRY(launchWriteCHW(res, npix, target, channels, s));
+        } else if (r.featureMapType != 0) {
+            HIP_TRY(launchFeatureOut(r.featureMapType, facc, r.dAcc.as<float4>(), npix, target, channels, s));
         } else {
             HIP_TRY(launchWriteCHW(r.dAcc.as<float4>(), npix, target, channels, s));
         }
@@ -780,6 +820,79 @@ int crvpt_render_frame(float* out, int channels, int height, int width, int fram
         HIP_TRY(hipStreamSynchronize(s));
         HIP_TRY(hipMemcpy(out, r.dStage.ptr, size_t(npix) * channels * sizeof(float), hipMemcpyDeviceToHost));
     }
+    return CRVPT_OK;
+}
+
+namespace {
+// FeatureMapTypeVpt values the gfx950 path produces (VolumetricPathTracingPass.hpp:67-73); Depth Blended, Depth
+// Nearest Opaque, the transmittance volume and the absorption moments belong to renderers outside this path.
+bool featureMapSupported(int m) { return (m >= 0 && m <= 14) || m == 17; }
+}  // namespace
+
+int crvpt_set_use_feature_maps(const int32_t* maps, int count) {
+    if (int rc = requireInit()) return rc;
+    if (count < 0 || (count > 0 && !maps)) return fail(CRVPT_ERR_INVALID_ARGUMENT, "set_use_feature_maps: bad list");
+    std::vector<int> v;
+    for (int i = 0; i < count; i++) {
+        if (!featureMapSupported(maps[i]))
+            return fail(CRVPT_ERR_UNSUPPORTED, "set_use_feature_maps: feature map " + std::to_string(maps[i]) +
+                                                   " is not produced by the gfx950 path tracer");
+        v.push_back(maps[i]);
+    }
+    g_r->featureMaps = v;
+    g_r->resetAccumulation();
+    return CRVPT_OK;
+}
+
+int crvpt_set_feature_map_type(int32_t type) {
+    if (int rc = requireInit()) return rc;
+    if (!featureMapSupported(type))
+        return fail(CRVPT_ERR_UNSUPPORTED, "set_feature_map_type: feature map " + std::to_string(type) +
+                                               " is not produced by the gfx950 path tracer");
+    if (type != g_r->featureMapType) g_r->resetAccumulation();
+    g_r->featureMapType = type;
+    return CRVPT_OK;
+}
+
+int crvpt_get_feature_map(int32_t map, float* out, int channels, int height, int width, int out_on_device,
+                          void* stream) {
+    if (int rc = requireInit()) return rc;
+    if (channels != 3 && channels != 4)
+        return fail(CRVPT_ERR_INVALID_ARGUMENT,
+                    "Error in renderFrame: The number of image channels is not equal to 3 or 4.");
+    if (!out || height <= 0 || width <= 0) return fail(CRVPT_ERR_INVALID_ARGUMENT, "get_feature_map: bad output");
+    if (!featureMapSupported(map))
+        return fail(CRVPT_ERR_UNSUPPORTED, "get_feature_map: feature map " + std::to_string(map) +
+                                               " is not produced by the gfx950 path tracer");
+    Renderer& r = *g_r;
+    if (width != r.accW || height != r.accH)
+        return fail(CRVPT_ERR_INVALID_ARGUMENT, "get_feature_map: no frame of this size was rendered");
+    if (map != 0 && (r.featW != width || r.featH != height || !r.dFeatAcc.ptr))
+        return fail(CRVPT_ERR_INVALID_ARGUMENT,
+                    "get_feature_map: enable the map with set_use_feature_maps before rendering");
+    const uint32_t npix = uint32_t(width) * uint32_t(height);
+    hipStream_t s = out_on_device ? reinterpret_cast<hipStream_t>(stream) : r.stream;
+    float* target = out;
+    if (!out_on_device) {
+        HIP_TRY(r.dStage.ensure(size_t(npix) * channels * sizeof(float)));
+        target = r.dStage.as<float>();
+    }
+    const FeatureAcc acc = map != 0 ? featureAcc(r, npix) : FeatureAcc{};
+    HIP_TRY(launchFeatureOut(map, acc, r.dAcc.as<float4>(), npix, target, channels, s));
+    if (!out_on_device) {
+        HIP_TRY(hipStreamSynchronize(s));
+        HIP_TRY(hipMemcpy(out, r.dStage.ptr, size_t(npix) * channels * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    return CRVPT_OK;
+}
+
+int crvpt_set_view_projection_matrix_as_previous(void) {
+    if (int rc = requireInit()) return rc;
+    Renderer& r = *g_r;
+    const int w = r.accW > 0 ? r.accW : 1, h = r.accH > 0 ? r.accH : 1;
+    const Mat4 vp = mul(projection(r, w, h), r.view);
+    for (int i = 0; i < 16; i++) r.prevViewProj[i] = float(vp.m[i]);
+    r.prevViewProjSet = true;
     return CRVPT_OK;
 }
 

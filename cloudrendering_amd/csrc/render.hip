@@ -269,7 +269,8 @@ CRVPT_DEV float density(const Grid& g, const SceneParams& P, Rng& rng, V3 p, typ
 constexpr uint32_t kEventSpace = 0x80000000u;
 constexpr uint32_t kCameraBlock = 0xFFFFFFFFu;
 
-enum : uint32_t { PF_ALIVE = 1u, PF_EXHAUSTED = 2u, PF_HAS_VALUE = 4u, PF_SHADOW = 8u };
+enum : uint32_t { PF_ALIVE = 1u, PF_EXHAUSTED = 2u, PF_HAS_VALUE = 4u, PF_SHADOW = 8u, PF_FIRST = 16u };
+constexpr uint32_t kFeatureBlock = 0xFFFFFFFEu;  // Philox block of the first event's finite-difference jitter
 // flags bits 8..31 count the scattering events of the main path (in-scattering iterations for residual ratio
 // tracking)
 constexpr uint32_t kBounceShift = 8;
@@ -334,7 +335,8 @@ CRVPT_DEV void cameraRay(const SceneParams& P, const PathState& st, V3& x, V3& w
 }
 
 template <int MODE>
-CRVPT_DEV bool startPath(PathState& st, Walker& W, const SceneParams& P, const SupervoxelGrid& S, float4* out) {
+CRVPT_DEV bool startPath(PathState& st, Walker& W, const SceneParams& P, const SupervoxelGrid& S, float4* out,
+                         float4* feat) {
     const uint32_t npix = uint32_t(P.width) * uint32_t(P.height);
     const uint32_t s = st.unit / npix;
     st.pix = st.unit - s * npix;
@@ -348,6 +350,7 @@ CRVPT_DEV bool startPath(PathState& st, Walker& W, const SceneParams& P, const S
     st.resT = 0.0f;
     st.resDist = 0.0f;
     st.weights = v3(1.0f);
+    if (feat) feat[3 * size_t(st.unit)] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);  // no first event yet
     V3 x, w;
     cameraRay(P, st, x, w);
     if (!walkerStart<MODE>(W, P, S, x, w, kInf)) {
@@ -358,13 +361,59 @@ CRVPT_DEV bool startPath(PathState& st, Walker& W, const SceneParams& P, const S
     return true;
 }
 
+// getCloudFiniteDifference (VptUtils.glsl:726-748) at p for the normal feature map: central differences of the
+// normalised grid one texel apart around the (stochastically jittered) texel, scaled by 100 / dim, normalised; the
+// reference returns 0 for NanoVDB grids.
+CRVPT_DEV V3 cloudFiniteDifference(const NanoAccess&, const SceneParams&, const PathState&, V3) { return v3(0.0f); }
+CRVPT_DEV V3 cloudFiniteDifference(const DenseAccess& grid, const SceneParams& P, const PathState& st, V3 p) {
+    V3 u = grid.toTex(P, p);
+    if (P.interpolation == INTERP_STOCHASTIC) {
+        const uint4 b = pathBlock(P, st, kFeatureBlock);
+        u = u + v3(u01(b.x) - 0.5f, u01(b.y) - 0.5f, u01(b.z) - 0.5f);
+    }
+    const V3 dim = v3(P.worldToTex[0] * (P.boxMax[0] - P.boxMin[0]), P.worldToTex[1] * (P.boxMax[1] - P.boxMin[1]),
+                      P.worldToTex[2] * (P.boxMax[2] - P.boxMin[2]));
+    const float gx = grid.lookupTex(P, u - v3(1.0f, 0.0f, 0.0f), 0.0f, 0.0f, 0.0f) -
+                     grid.lookupTex(P, u + v3(1.0f, 0.0f, 0.0f), 0.0f, 0.0f, 0.0f);
+    const float gy = grid.lookupTex(P, u - v3(0.0f, 1.0f, 0.0f), 0.0f, 0.0f, 0.0f) -
+                     grid.lookupTex(P, u + v3(0.0f, 1.0f, 0.0f), 0.0f, 0.0f, 0.0f);
+    const float gz = grid.lookupTex(P, u - v3(0.0f, 0.0f, 1.0f), 0.0f, 0.0f, 0.0f) -
+                     grid.lookupTex(P, u + v3(0.0f, 0.0f, 1.0f), 0.0f, 0.0f, 0.0f);
+    V3 d = v3(gx / dim.x * 100.0f, gy / dim.y * 100.0f, gz / dim.z * 100.0f);
+    const float l = length(d);
+    if (l > 1e-5f) d = d * (1.0f / l);
+    return d;
+}
+
+// First scattering or absorption event of the path (the ScatterEvent the trackers fill, Clouds.glsl:91-96): its
+// position, depth (distance along the camera ray), outgoing direction and phase pdf, extinction-scaled density and
+// the finite-difference normal, written once per path for the feature maps.
+template <int MODE, class Grid>
+CRVPT_DEV void recordFirst(PathState& st, const SceneParams& P, const Grid& grid, float4* feat, V3 p, V3 w,
+                           float pdfW, float dens, float depth) {
+    if (!feat || (st.flags & PF_FIRST)) return;
+    st.flags |= PF_FIRST;
+    const V3 n = cloudFiniteDifference(grid, P, st, p);
+    const size_t k = 3 * size_t(st.unit);
+    feat[k] = make_float4(p.x, p.y, p.z, depth);
+    feat[k + 1] = make_float4(w.x, w.y, w.z, pdfW);
+    feat[k + 2] = make_float4(n.x, n.y, n.z, dens);
+}
+
 // Real scattering collision at p of the main path (rare compared with tentative collisions).
-template <int MODE>
+template <int MODE, class Grid>
 CRVPT_DEV bool scatterEvent(PathState& st, Walker& W, const SceneParams& P, const SupervoxelGrid& S, V3 p,
-                            float pAbsOverReal, float4* out) {
+                            float pAbsOverReal, float rho, const Grid& grid, float4* feat, float4* out) {
     const uint4 e = pathBlock(P, st, kEventSpace | st.events++);
     st.flags += 1u << kBounceShift;
     const V3 nw = phaseDirection(P.phaseG, W.w, u01(e.x), u01(e.y));
+    if (feat && MODE != VPT_DECOMPOSITION_TRACKING) {
+        // ratio tracking records neither depth nor density (RatioTracking.glsl:75-81)
+        const bool ratio = MODE == VPT_RATIO_TRACKING;
+        const float ext = (isNee<MODE>() || isSpectral<MODE>()) ? P.extinctionMax : P.extinction;
+        recordFirst<MODE>(st, P, grid, feat, p, nw, evalPhase(P.phaseG, W.w, nw), ratio ? 0.0f : rho * ext,
+                          ratio ? 0.0f : W.t);
+    }
     if (MODE == VPT_RATIO_TRACKING) st.thr *= 1.0f - pAbsOverReal;
     st.flags |= PF_HAS_VALUE;
     if (isNee<MODE>()) {
@@ -473,9 +522,9 @@ CRVPT_DEV bool rrtTraversalEnd(PathState& st, Walker& W, const SceneParams& P, c
 // NextEventTracking.glsl:89-215): collision probabilities from the path-history-weighted averages of the RGB
 // absorption, scattering and null coefficients; the weights carry the ratio of each channel's coefficient to the
 // chosen probability and are clamped to 100.
-template <int MODE>
+template <int MODE, class Grid>
 CRVPT_DEV bool spectralCollision(PathState& st, Walker& W, const SceneParams& P, const SupervoxelGrid& S, V3 p,
-                                 float rho, float xi, float4* out) {
+                                 float rho, float xi, const Grid& grid, float4* feat, float4* out) {
     const V3 ext = v3(P.extinctionRGB) * rho;
     const V3 alb = v3(P.albedoRGB);
     const V3 sa = (v3(1.0f) - alb) * ext;
@@ -487,6 +536,7 @@ CRVPT_DEV bool spectralCollision(PathState& st, Walker& W, const SceneParams& P,
     Ps /= C;
     Pn /= C;
     if (xi < Pa) {
+        recordFirst<MODE>(st, P, grid, feat, p, v3(0.0f), 0.0f, rho * P.extinctionMax, W.t);
         writePath(out, st.unit, isNee<MODE>() ? st.color : v3(0.0f), true);
         return false;
     }
@@ -498,7 +548,7 @@ CRVPT_DEV bool spectralCollision(PathState& st, Walker& W, const SceneParams& P,
         st.weights = st.weights * (ss / v3(W.mu * Ps));
         // next-event estimation uses the weights before the clamp, as the reference's loop clamps at its end
         if (!isNee<MODE>()) st.weights = vmin(st.weights, 100.0f);
-        const bool alive = scatterEvent<MODE>(st, W, P, S, p, 0.0f, out);
+        const bool alive = scatterEvent<MODE>(st, W, P, S, p, 0.0f, rho, grid, feat, out);
         st.weights = vmin(st.weights, 100.0f);
         return alive;
     }
@@ -509,7 +559,7 @@ CRVPT_DEV bool spectralCollision(PathState& st, Walker& W, const SceneParams& P,
 // One step of a live path. Returns false when the path finished (its radiance is written).
 template <int MODE, class Grid>
 CRVPT_DEV bool stepPath(PathState& st, Walker& W, const SceneParams& P, const SupervoxelGrid& S, const Grid& grid,
-                        typename Grid::Cache& gc, float4* out) {
+                        typename Grid::Cache& gc, float4* out, float4* feat) {
     const uint4 r = pathBlock(P, st, st.step++);
     const int ev = walkAdvance<MODE>(W, P, S, -__logf(uOpen(r.x)));
     if (ev == WALK_MOVED) return true;
@@ -542,12 +592,13 @@ CRVPT_DEV bool stepPath(PathState& st, Walker& W, const SceneParams& P, const Su
             writePath(out, st.unit, v3(0.0f), true);
             return false;
         }
-        return scatterEvent<MODE>(st, W, P, S, p, 0.0f, out);
+        return scatterEvent<MODE>(st, W, P, S, p, 0.0f, 0.0f, grid, feat, out);
     }
     const float rho = grid.lookupWorldJ(P, p, jitterHi(r.y), jitterLo(r.y), jitterHi(r.z), gc);
     st.taps++;
     const float pReal = P.extinction * rho * rcpf(W.mu);
-    if (isSpectral<MODE>() && !(st.flags & PF_SHADOW)) return spectralCollision<MODE>(st, W, P, S, p, rho, xi, out);
+    if (isSpectral<MODE>() && !(st.flags & PF_SHADOW))
+        return spectralCollision<MODE>(st, W, P, S, p, rho, xi, grid, feat, out);
     if (isNee<MODE>() && (st.flags & PF_SHADOW)) {
         st.thr *= 1.0f - pReal;  // ratio tracking (NeeTransmittance.glsl:107-113)
         // Once the product is exactly 0 every later factor leaves it 0 and the shadow ray adds exactly 0.
@@ -559,10 +610,15 @@ CRVPT_DEV bool stepPath(PathState& st, Walker& W, const SceneParams& P, const Su
     const bool bounceLimit = P.maxBounces != 0u && (st.flags >> kBounceShift) >= P.maxBounces;
     if ((MODE != VPT_RATIO_TRACKING && xi < pAbs) || bounceLimit) {
         // absorption or a scattering event beyond the bounce limit: delta / ratio end with 0, next-event with its sum
+        if (MODE != VPT_RATIO_TRACKING && xi < pAbs) {
+            // delta tracking records w = 0 (DeltaTracking.glsl:291-300), next-event the incoming direction
+            recordFirst<MODE>(st, P, grid, feat, p, isNee<MODE>() ? W.w : v3(0.0f), 0.0f,
+                              rho * (isNee<MODE>() ? P.extinctionMax : P.extinction), W.t);
+        }
         writePath(out, st.unit, isNee<MODE>() ? st.color : v3(0.0f), true);
         return false;
     }
-    return scatterEvent<MODE>(st, W, P, S, p, pAbs / pReal, out);
+    return scatterEvent<MODE>(st, W, P, S, p, pAbs / pReal, rho, grid, feat, out);
 }
 
 // Wave-reduced 64-bit work counters: [0] density taps, [1] supervoxel majorant / table reads, [2] paths started,
@@ -588,7 +644,7 @@ CRVPT_DEV void flushCounters(unsigned long long* stats, uint32_t taps, uint32_t 
 template <int MODE, class Grid>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CRVPT_PATHPOOL_WAVES))) void k_pathPool(
         SceneParams P, Grid grid, SupervoxelGrid S, float4* __restrict__ out, uint32_t* __restrict__ counter,
-        int stepsPerRefill, unsigned long long* __restrict__ stats) {
+        int stepsPerRefill, unsigned long long* __restrict__ stats, float4* __restrict__ feat) {
     const int lane = __lane_id();
     PathState st;
     st.flags = 0;
@@ -616,14 +672,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CRVPT_PATHP
                     exhausted = true;
                 } else {
                     paths++;
-                    startPath<MODE>(st, W, P, S, out);
+                    startPath<MODE>(st, W, P, S, out, feat);
                 }
             }
         }
         if (__ballot((st.flags & PF_ALIVE) != 0) == 0) break;  // every lane is exhausted
         for (int k = 0; k < stepsPerRefill; k++) {
             if (st.flags & PF_ALIVE) {
-                if (!stepPath<MODE, Grid>(st, W, P, S, grid, gc, out)) st.flags = 0;
+                if (!stepPath<MODE, Grid>(st, W, P, S, grid, gc, out, feat)) st.flags = 0;
             }
         }
     }
@@ -758,6 +814,150 @@ __global__ void k_writeCHW(const float4* __restrict__ img, uint32_t npix, float*
     outCHW[npix + p] = a.y;
     outCHW[2 * size_t(npix) + p] = a.z;
     if (channels == 4) outCHW[3 * size_t(npix) + p] = a.w;
+}
+
+// Feature maps (Clouds.glsl:146-383): per pixel, the chunk's samples in order, each mixed into the running state
+// with weight 1 / (frame + 1) as the reference's imageLoad / mix / imageStore sequence does.
+__global__ void k_accumulateFeatures(SceneParams P, const float4* __restrict__ paths, const float4* __restrict__ feat,
+                                     uint32_t npix, uint32_t frames, uint32_t prevFrames, FeatureAcc A) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npix) return;
+    const float px = float(p % uint32_t(P.width)), py = float(p / uint32_t(P.width));
+    float4 firstX = A.firstX[p], firstW = A.firstW[p], normal = A.normal[p], cloudOnly = A.cloudOnly[p];
+    float4 background = A.background[p], albedo = A.albedo[p], depth = A.depth[p], dens = A.dens[p];
+    float4 flow = A.flow[p], flowRev = A.flowRev[p];
+    const float* M = P.prevViewProj;
+    for (uint32_t f = 0; f < frames; f++) {
+        const uint32_t frame = prevFrames + f;
+        const float a = 1.0f / float(frame + 1);
+        const float b = 1.0f - a;
+        const size_t u = size_t(f) * npix + p;
+        const float4 res = paths[u];
+        const float4 f0 = feat[3 * u], f1 = feat[3 * u + 1], f2 = feat[3 * u + 2];
+        const bool has = f0.w >= 0.0f;
+        if (frame == 0) {
+            const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            firstX = normal = cloudOnly = background = albedo = depth = dens = z;
+            flow = make_float4(0.0f, 0.0f, -1.0f, -1.0f);
+        }
+        PathState st;
+        st.pix = p;
+        st.frame = P.frameBase + f;
+        V3 xo, w;
+        cameraRay(P, st, xo, w);
+        auto mix4 = [&](float4 o, float4 v) {
+            return make_float4(o.x * b + v.x * a, o.y * b + v.y * a, o.z * b + v.z * a, o.w * b + v.w * a);
+        };
+        cloudOnly = mix4(cloudOnly, has ? make_float4(res.x, res.y, res.z, 1.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+        const V3 sky = sampleSky(P, w);
+        background = mix4(background, has ? make_float4(sky.x, sky.y, sky.z, 1.0f) : make_float4(res.x, res.y, res.z, 1.0f));
+        firstX = mix4(firstX, has ? make_float4(f0.x, f0.y, f0.z, 1.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+        const float d = has ? f0.w : 0.0f;
+        depth.x = depth.x * b + d * a;
+        depth.y = depth.y * b + d * d * a;
+        depth.z = depth.z * b + (has ? f0.w : P.farDistance) * a;
+        const float dn = has ? f2.w * 0.001f : 0.0f;
+        dens.x = dens.x * b + dn * a;
+        dens.y = dens.y * b + dn * dn * a;
+        albedo = mix4(albedo, has ? make_float4(P.albedoRGB[0], P.albedoRGB[1], P.albedoRGB[2], 1.0f)
+                                  : make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+        float nx = 0.0f, ny = 0.0f;
+        float2 fl = make_float2(0.0f, 0.0f);
+        float2 uv = make_float2(flow.z, flow.w);
+        if (has) {
+            normal = make_float4(normal.x * b + f2.x * a, normal.y * b + f2.y * a, normal.z * b + f2.z * a, 1.0f);
+            // inverseTransposedViewMatrix * (n, 0): rows of the inverse view's rotation columns
+            const float* iv = P.invView;
+            const float Acos = iv[0] * normal.x + iv[1] * normal.y + iv[2] * normal.z;
+            const float Bcos = iv[4] * normal.x + iv[5] * normal.y + iv[6] * normal.z;
+            nx = Acos / sqrtf(1.0f - Acos * Acos);
+            ny = Bcos / sqrtf(1.0f - Bcos * Bcos);
+            firstW = f1;
+            const float cx = M[0] * f0.x + M[4] * f0.y + M[8] * f0.z + M[12];
+            const float cy = M[1] * f0.x + M[5] * f0.y + M[9] * f0.z + M[13];
+            const float cw = M[3] * f0.x + M[7] * f0.y + M[11] * f0.z + M[15];
+            uv = make_float2(cx / cw * 0.5f + 0.5f, cy / cw * 0.5f + 0.5f);
+            fl = make_float2(px - ((0.5f * (cx / cw) + 0.5f) * float(P.width) - 0.5f),
+                             py - ((0.5f * (cy / cw) + 0.5f) * float(P.height) - 0.5f));
+        } else {
+            const float k = float(frame) / (float(frame) + 1.0f);
+            normal = make_float4(normal.x * k, normal.y * k, normal.z * k, 1.0f);
+            firstW = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+        flow = make_float4(fl.x, fl.y, uv.x, uv.y);
+        {
+            const float wx = has ? f0.x : w.x, wy = has ? f0.y : w.y, wz = has ? f0.z : w.z, ww = has ? 1.0f : 0.0f;
+            const float cx = M[0] * wx + M[4] * wy + M[8] * wz + M[12] * ww;
+            const float cy = M[1] * wx + M[5] * wy + M[9] * wz + M[13] * ww;
+            const float cw = M[3] * wx + M[7] * wy + M[11] * wz + M[15] * ww;
+            flowRev = make_float4((0.5f * (cx / cw) + 0.5f) * float(P.width) - 0.5f - px,
+                                  (0.5f * (cy / cw) + 0.5f) * float(P.height) - 0.5f - py, 0.0f, 0.0f);
+        }
+        if (frame != 0) {
+            nx = dens.z * b + nx * a;
+            ny = dens.w * b + ny * a;
+        }
+        dens.z = nx;
+        dens.w = ny;
+        float fw = fabsf(nx) + fabsf(ny);
+        if (frame != 0) fw = depth.w * b + fw * a;
+        depth.w = fw;
+    }
+    A.firstX[p] = firstX;
+    A.firstW[p] = firstW;
+    A.normal[p] = normal;
+    A.cloudOnly[p] = cloudOnly;
+    A.background[p] = background;
+    A.albedo[p] = albedo;
+    A.depth[p] = depth;
+    A.dens[p] = dens;
+    A.flow[p] = flow;
+    A.flowRev[p] = flowRev;
+}
+
+// FeatureMapTypeVpt (VolumetricPathTracingPass.hpp:67-73) blitted to RGBA as the reference's readback does: RG and R
+// formats fill the missing channels with 0 and alpha with 1.
+__global__ void k_featureOut(int map, FeatureAcc A, const float4* __restrict__ result, uint32_t npix,
+                             float* __restrict__ outCHW, int channels) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npix) return;
+    float4 v;
+    switch (map) {
+        case 1: v = A.firstX[p]; break;
+        case 2: v = A.firstW[p]; break;
+        case 3: v = A.normal[p]; break;
+        case 4: {
+            const float4 n = A.normal[p];
+            const float l = sqrtf(n.x * n.x + n.y * n.y + n.z * n.z);
+            const float k = l > 0.0f ? 1.0f / l : 0.0f;
+            v = make_float4(n.x * k, n.y * k, n.z * k, 1.0f);
+            break;
+        }
+        case 5: v = A.cloudOnly[p]; break;
+        case 6: {
+            const float4 d = A.depth[p];
+            v = make_float4(d.x, sqrtf(fmaxf(0.0f, d.y - d.x * d.x)), 0.0f, 1.0f);
+            break;
+        }
+        case 7: v = make_float4(A.depth[p].z, 0.0f, 0.0f, 1.0f); break;
+        case 8: v = make_float4(A.flow[p].x, A.flow[p].y, 0.0f, 1.0f); break;
+        case 9: v = make_float4(A.flowRev[p].x, A.flowRev[p].y, 0.0f, 1.0f); break;
+        case 10: v = make_float4(A.dens[p].z, A.dens[p].w, 0.0f, 1.0f); break;
+        case 11: v = make_float4(A.depth[p].w, 0.0f, 0.0f, 1.0f); break;
+        case 12: {
+            const float4 d = A.dens[p];
+            v = make_float4(d.x, sqrtf(fmaxf(0.0f, d.y - d.x * d.x)), 0.0f, 1.0f);
+            break;
+        }
+        case 13: v = A.background[p]; break;
+        case 14: v = make_float4(A.flow[p].z, A.flow[p].w, 0.0f, 1.0f); break;
+        case 17: v = A.albedo[p]; break;
+        default: v = make_float4(result[p].x, result[p].y, result[p].z, 1.0f); break;
+    }
+    outCHW[p] = v.x;
+    outCHW[npix + p] = v.y;
+    outCHW[2 * size_t(npix) + p] = v.z;
+    if (channels == 4) outCHW[3 * size_t(npix) + p] = v.w;
 }
 
 __global__ void k_zeroU32(uint32_t* p, int n) {
@@ -1061,7 +1261,7 @@ __global__ void k_nanoLookupWorld(NanoAccess grid, int n, const float* __restric
 // ------------------------------------------------------------------------------------------------------------
 template <class Grid>
 static hipError_t launchRenderT(const SceneParams& P, const Grid& grid, const SupervoxelGrid& S, float4* paths,
-                                uint32_t* counter, const LaunchConfig& cfg, unsigned long long* stats,
+                                uint32_t* counter, const LaunchConfig& cfg, unsigned long long* stats, float4* feat,
                                 hipStream_t stream) {
     hipLaunchKernelGGL(k_zeroU32, dim3(1), dim3(64), 0, stream, counter, 1);
     const dim3 block(256);
@@ -1069,31 +1269,31 @@ static hipError_t launchRenderT(const SceneParams& P, const Grid& grid, const Su
     switch (P.mode) {
         case VPT_RATIO_TRACKING:
             hipLaunchKernelGGL((k_pathPool<VPT_RATIO_TRACKING, Grid>), gridDim, block, 0, stream, P, grid, S, paths,
-                               counter, cfg.stepsPerRefill, stats);
+                               counter, cfg.stepsPerRefill, stats, feat);
             break;
         case VPT_NEXT_EVENT_TRACKING:
             hipLaunchKernelGGL((k_pathPool<VPT_NEXT_EVENT_TRACKING, Grid>), gridDim, block, 0, stream, P, grid, S,
-                               paths, counter, cfg.stepsPerRefill, stats);
+                               paths, counter, cfg.stepsPerRefill, stats, feat);
             break;
         case VPT_SPECTRAL_DELTA_TRACKING:
             hipLaunchKernelGGL((k_pathPool<VPT_SPECTRAL_DELTA_TRACKING, Grid>), gridDim, block, 0, stream, P, grid,
-                               S, paths, counter, cfg.stepsPerRefill, stats);
+                               S, paths, counter, cfg.stepsPerRefill, stats, feat);
             break;
         case VPT_NEXT_EVENT_TRACKING_SPECTRAL:
             hipLaunchKernelGGL((k_pathPool<VPT_NEXT_EVENT_TRACKING_SPECTRAL, Grid>), gridDim, block, 0, stream, P,
-                               grid, S, paths, counter, cfg.stepsPerRefill, stats);
+                               grid, S, paths, counter, cfg.stepsPerRefill, stats, feat);
             break;
         case VPT_DECOMPOSITION_TRACKING:
             hipLaunchKernelGGL((k_pathPool<VPT_DECOMPOSITION_TRACKING, Grid>), gridDim, block, 0, stream, P, grid,
-                               S, paths, counter, cfg.stepsPerRefill, stats);
+                               S, paths, counter, cfg.stepsPerRefill, stats, feat);
             break;
         case VPT_RESIDUAL_RATIO_TRACKING:
             hipLaunchKernelGGL((k_pathPool<VPT_RESIDUAL_RATIO_TRACKING, Grid>), gridDim, block, 0, stream, P,
-                               grid, S, paths, counter, cfg.stepsPerRefill, stats);
+                               grid, S, paths, counter, cfg.stepsPerRefill, stats, feat);
             break;
         default:
             hipLaunchKernelGGL((k_pathPool<VPT_DELTA_TRACKING, Grid>), gridDim, block, 0, stream, P, grid, S, paths,
-                               counter, cfg.stepsPerRefill, stats);
+                               counter, cfg.stepsPerRefill, stats, feat);
             break;
     }
     return hipGetLastError();
@@ -1101,15 +1301,29 @@ static hipError_t launchRenderT(const SceneParams& P, const Grid& grid, const Su
 
 hipError_t launchRender(const SceneParams& P, const DenseGrid* dense, const NanoGrid* nano, const SupervoxelGrid& S,
                         float4* paths, uint32_t* counter, const LaunchConfig& cfg, unsigned long long* stats,
-                        hipStream_t stream) {
-    if (nano) return launchRenderT(P, NanoAccess{*nano}, S, paths, counter, cfg, stats, stream);
-    return launchRenderT(P, DenseAccess{*dense}, S, paths, counter, cfg, stats, stream);
+                        float4* feat, hipStream_t stream) {
+    if (nano) return launchRenderT(P, NanoAccess{*nano}, S, paths, counter, cfg, stats, feat, stream);
+    return launchRenderT(P, DenseAccess{*dense}, S, paths, counter, cfg, stats, feat, stream);
 }
 
 hipError_t launchAccumulate(const float4* paths, uint32_t npix, uint32_t frames, float4* acc, uint32_t prevFrames,
                             float* outCHW, int channels, hipStream_t stream) {
     hipLaunchKernelGGL(k_accumulate, dim3((npix + 255) / 256), dim3(256), 0, stream, paths, npix, frames, acc,
                        prevFrames, outCHW, channels);
+    return hipGetLastError();
+}
+
+hipError_t launchAccumulateFeatures(const SceneParams& P, const float4* paths, const float4* feat, uint32_t npix,
+                                    uint32_t frames, uint32_t prevFrames, const FeatureAcc& acc, hipStream_t stream) {
+    hipLaunchKernelGGL(k_accumulateFeatures, dim3((npix + 255) / 256), dim3(256), 0, stream, P, paths, feat, npix,
+                       frames, prevFrames, acc);
+    return hipGetLastError();
+}
+
+hipError_t launchFeatureOut(int map, const FeatureAcc& acc, const float4* result, uint32_t npix, float* outCHW,
+                            int channels, hipStream_t stream) {
+    hipLaunchKernelGGL(k_featureOut, dim3((npix + 255) / 256), dim3(256), 0, stream, map, acc, result, npix, outCHW,
+                       channels);
     return hipGetLastError();
 }
 

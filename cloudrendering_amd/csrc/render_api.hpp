@@ -9,6 +9,21 @@
 
 namespace crvpt {
 
+// Per-pixel running state of the feature maps (Clouds.glsl:146-383), float4 per pixel each.
+struct FeatureAcc {
+    float4* firstX;      // First X: (x, y, z, coverage)
+    float4* firstW;      // First W: (w, pdf_w) of the last sample
+    float4* normal;      // Normal: running mean of the finite-difference normal
+    float4* cloudOnly;   // Cloud Only
+    float4* background;  // Background
+    float4* albedo;      // Albedo
+    float4* depth;       // (mean depth, mean squared depth, Depth Far mean, Depth (fwidth))
+    float4* dens;        // (mean density * 1e-3, its mean square, Depth (nabla) x, y)
+    float4* flow;        // (Flow x, y of the last sample, Reprojected UV)
+    float4* flowRev;     // (Flow Reverse x, y of the last sample, 0, 0)
+};
+constexpr int kFeatureAccArrays = 10;
+
 struct LaunchConfig {
     uint32_t numBlocks;      // persistent workgroups of 256 threads
     int32_t stepsPerRefill;  // walker events per lane between two refill ballots (the compaction interval)
@@ -30,10 +45,15 @@ struct FbmParams {
 // stats (nullable): 64-bit device counters [density taps, supervoxel majorant reads, paths], accumulated.
 hipError_t launchRender(const SceneParams& P, const DenseGrid* dense, const NanoGrid* nano, const SupervoxelGrid& S,
                         float4* paths, uint32_t* counter, const LaunchConfig& cfg, unsigned long long* stats,
-                        hipStream_t stream);
+                        float4* feat, hipStream_t stream);
 hipError_t launchAccumulate(const float4* paths, uint32_t npix, uint32_t frames, float4* acc, uint32_t prevFrames,
                             float* outCHW, int channels, hipStream_t stream);
 hipError_t launchWriteCHW(const float4* img, uint32_t npix, float* outCHW, int channels, hipStream_t stream);
+hipError_t launchAccumulateFeatures(const SceneParams& P, const float4* paths, const float4* feat, uint32_t npix,
+                                    uint32_t frames, uint32_t prevFrames, const FeatureAcc& acc, hipStream_t stream);
+// Writes feature map `map` (FeatureMapTypeVpt order) as C planes of H*W floats; result = the image accumulator.
+hipError_t launchFeatureOut(int map, const FeatureAcc& acc, const float4* result, uint32_t npix, float* outCHW,
+                            int channels, hipStream_t stream);
 hipError_t launchTransmittance(const SceneParams& P, const DenseGrid* dense, const NanoGrid* nano,
                                const SupervoxelGrid& S, int estimator, const float* o, const float* d,
                                const float* tmax, int nRays, int nSamples, uint32_t seed, float* mean, float* m2,

@@ -85,6 +85,8 @@ struct SupervoxelGrid {
 struct SceneParams {
     float invViewProj[16];  // column-major
     float invView[16];      // column-major
+    float prevViewProj[16]; // previous frame's view-projection (feature maps: flow, reprojected UV), column-major
+    float farDistance;      // camera far plane (the "Depth Far" feature map of rays without an event)
     float boxMin[3];
     float boxMax[3];
     float worldToTex[3];    // grid dim / (boxMax - boxMin): texel coordinates of a world position

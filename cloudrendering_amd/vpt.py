@@ -32,6 +32,8 @@ __all__ = [
     "set_use_majorant_grid", "set_launch_config", "get_last_render_stats", "get_num_lights",
     "set_collect_stats", "get_last_render_counters", "build_nanovdb_from_dense", "grid_lookup",
     "grid_lookup_world", "get_supervoxel_tables", "set_max_bounces", "set_super_voxel_size",
+    "set_use_feature_maps", "get_feature_map_from_string", "get_feature_map", "set_feature_map_type",
+    "set_view_projection_matrix_as_previous", "VPT_FEATURE_MAP_NAMES",
     "VPT_MODE_NAMES", "CrvptError",
 ]
 
@@ -79,6 +81,61 @@ def render_frame(input_tensor, frame_count: int):
     else:
         raise CrvptError("Unsupported PyTorch device type.")
     return out
+
+
+# VPT_FEATURE_MAP_NAMES (VolumetricPathTracingPass.hpp:74-80), indexed by FeatureMapTypeVpt
+VPT_FEATURE_MAP_NAMES = [
+    "Result", "First X", "First W", "Normal", "Normal (Length 1)", "Cloud Only", "Depth", "Depth Far",
+    "Flow", "Flow Reverse", "Depth (nabla)", "Depth (fwidth)", "Density", "Background", "Reprojected UV",
+    "Depth Blended", "Depth Nearest Opaque", "Albedo",
+    "Transmittance Volume",
+    "Primary Ray Absorption Moments", "Scatter Ray Absorption Moments",
+]
+
+
+def set_use_feature_maps(feature_map_names: Sequence[str]) -> None:
+    """Module.cpp:319-330: names not in VPT_FEATURE_MAP_NAMES are ignored, as in the reference."""
+    idx = [VPT_FEATURE_MAP_NAMES.index(n) for n in feature_map_names if n in VPT_FEATURE_MAP_NAMES]
+    arr = (ctypes.c_int32 * max(1, len(idx)))(*idx)
+    call("crvpt_set_use_feature_maps", ctypes.cast(arr, ctypes.c_void_p), len(idx))
+
+
+def set_feature_map_type(type: int) -> None:  # noqa: A002 - the reference's argument name
+    call("crvpt_set_feature_map_type", int(type))
+
+
+def get_feature_map(input_tensor, feature_map: int):
+    """Module.cpp:343-400: the feature map of the last render as a new [C, H, W] float32 tensor on the input
+    tensor's device (shape, device and dtype of input_tensor are used, not its contents)."""
+    torch = _torch()
+    if input_tensor.dim() != 3:
+        raise CrvptError("Error in renderFrame: inputTensor.sizes().size() != 3.")
+    c, h, w = (int(s) for s in input_tensor.shape)
+    if c != 3 and c != 4:
+        raise CrvptError("Error in renderFrame: The number of image channels is not equal to 3 or 4.")
+    if input_tensor.dtype != torch.float32:
+        raise CrvptError("Error in renderFrame: The only data type currently supported is 32-bit float.")
+    dev = input_tensor.device
+    out = torch.empty((c, h, w), dtype=torch.float32, device=dev)
+    if dev.type == "cpu":
+        call("crvpt_get_feature_map", int(feature_map), ctypes.c_void_p(out.data_ptr()), c, h, w, 0, None)
+    elif dev.type == "cuda":
+        stream = torch.cuda.current_stream(dev).cuda_stream
+        call("crvpt_get_feature_map", int(feature_map), ctypes.c_void_p(out.data_ptr()), c, h, w, 1,
+             ctypes.c_void_p(stream))
+    else:
+        raise CrvptError("Unsupported PyTorch device type.")
+    return out
+
+
+def get_feature_map_from_string(input_tensor, feature_map: str):
+    """Module.cpp:332-341: unknown names select map 0 (Result), as in the reference."""
+    idx = VPT_FEATURE_MAP_NAMES.index(feature_map) if feature_map in VPT_FEATURE_MAP_NAMES else 0
+    return get_feature_map(input_tensor, idx)
+
+
+def set_view_projection_matrix_as_previous() -> None:
+    call("crvpt_set_view_projection_matrix_as_previous")
 
 
 def load_cloud_file(filename: str) -> None:

@@ -156,6 +156,23 @@ int crvpt_get_supervoxel_tables(int* supervoxel_size, int grid_size[3], float* d
 int crvpt_set_use_majorant_grid(int use_majorant_grid);
 int crvpt_set_launch_config(int supervoxel_size, int blocks, int steps_per_refill);
 
+/* Module.cpp:319-330 set_use_feature_maps(names) with the names mapped to FeatureMapTypeVpt values
+ * (VolumetricPathTracingPass.hpp:67-80): the listed maps are accumulated by the following renders. Produced here:
+ * 0 Result, 1 First X, 2 First W, 3 Normal, 4 Normal (Length 1), 5 Cloud Only, 6 Depth, 7 Depth Far, 8 Flow,
+ * 9 Flow Reverse, 10 Depth (nabla), 11 Depth (fwidth), 12 Density, 13 Background, 14 Reprojected UV, 17 Albedo;
+ * the others report CRVPT_ERR_UNSUPPORTED. */
+int crvpt_set_use_feature_maps(const int32_t* maps, int count);
+/* Module.cpp:1078-1081 set_feature_map_type(type): the map render_frame returns instead of the result. */
+int crvpt_set_feature_map_type(int32_t type);
+/* Module.cpp:343-400 get_feature_map(input_tensor, map): the map of the last render as C planes of H*W floats
+ * (the reference's RGBA blit: RG and R maps have 0 in the missing channels and alpha 1). Device / host output as in
+ * crvpt_render_frame. */
+int crvpt_get_feature_map(int32_t map, float* out, int channels, int height, int width, int out_on_device,
+                          void* stream);
+/* Module.cpp:1339-1341 set_view_projection_matrix_as_previous(): the current view-projection becomes the
+ * previous-frame matrix of the Flow / Flow Reverse / Reprojected UV maps (until then the current one is used). */
+int crvpt_set_view_projection_matrix_as_previous(void);
+
 /* No reference counterpart: statistics of the last crvpt_render_frame: kernel milliseconds of the path kernels
  * (device events on the render stream), number of paths, number of path-kernel launches. */
 int crvpt_get_last_render_stats(double* path_kernel_ms, double* total_ms, uint64_t* num_paths, int* launches);

@@ -85,6 +85,9 @@ def load():
     lib.oracle_supervoxel_tables.argtypes = [P(OracleScene), ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p,
                                              ctypes.c_void_p, ctypes.c_void_p]
     lib.oracle_supervoxel_tables.restype = ctypes.c_int
+    lib.oracle_render_features.argtypes = [P(OracleScene), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p, ctypes.c_int]
+    lib.oracle_render_features.restype = ctypes.c_int
     lib.oracle_hybrid_taus.argtypes = [ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p]
     lib.oracle_camera_matrices.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_double] * 4 + [ctypes.c_void_p] * 3
     _lib = lib
@@ -114,6 +117,16 @@ def box_for_grid(shape_zyx, max_grid_extent=0.25):
     s = np.array([nx, ny, nz], np.float32)
     bmax = s * np.float32(max_grid_extent) / np.float32(s.max())
     return -bmax, bmax
+
+
+# Feature maps and their channel counts in the reference's image formats (VolumetricPathTracingPass.cpp:295-445).
+FEATURE_MAPS = {"result": 4, "first_x": 4, "first_w": 4, "normal": 4, "cloud_only": 4, "background": 4, "depth": 2,
+                "depth_far": 1, "density": 2, "reproj_uv": 2, "albedo": 4, "flow": 2, "flow_reverse": 2,
+                "depth_nabla": 2, "depth_fwidth": 1}
+
+
+class OracleFeatureMaps(ctypes.Structure):
+    _fields_ = [(name, ctypes.POINTER(ctypes.c_float)) for name in FEATURE_MAPS]
 
 
 class Scene:
@@ -178,6 +191,7 @@ class Scene:
         if camera is None:
             camera = camera_matrices(aspect=width / height)
         ivp, iv = camera[0], camera[1]
+        self.view_proj = camera[2] if len(camera) > 2 else None
         sc.inverse_view_proj[:] = [float(v) for v in ivp]
         sc.inverse_view[:] = [float(v) for v in iv]
         sc.mode = MODES[mode] if isinstance(mode, str) else int(mode)
@@ -199,6 +213,28 @@ class Scene:
                                threads)
         assert rc == 0
         return acc, sq
+
+    def render_features(self, frames, names=FEATURE_MAPS, frame_start=0, state=None, prev_view_proj=None,
+                        far_distance=80.0, threads=0):
+        """Feature maps of Clouds.glsl after frames [frame_start, frame_start + frames): a dict name -> [H, W, C]
+        float32 in the reference image format's channel count; `state` (a previous result) continues it."""
+        lib = load()
+        h, w = self.height, self.width
+        maps = OracleFeatureMaps()
+        out = {}
+        for name in names:
+            ch = FEATURE_MAPS[name]
+            a = state[name] if state is not None else np.zeros((h, w, ch), np.float32)
+            a = np.ascontiguousarray(a, np.float32)
+            out[name] = a
+            setattr(maps, name, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+        if prev_view_proj is None:
+            prev_view_proj = np.asarray(self.view_proj, np.float32)
+        pvp = np.ascontiguousarray(np.asarray(prev_view_proj, np.float32).reshape(16))
+        rc = lib.oracle_render_features(ctypes.byref(self.sc), w, h, frame_start, frames, pvp.ctypes.data,
+                                        float(far_distance), ctypes.byref(maps), threads)
+        assert rc == 0
+        return out
 
     def transmittance(self, estimator, origins, directions, t_max=None, n_samples=1, seed=1, threads=0):
         lib = load()

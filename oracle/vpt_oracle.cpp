@@ -357,8 +357,27 @@ float calculateTransmittanceDistance(const Ctx& c, Rng& rng, vec3 x, vec3 w, flo
     return transmittance;
 }
 
+/* First scattering / absorption event of a path (Clouds.glsl:91-96 ScatterEvent; feature maps). */
+struct ScatterEvent {
+    bool hasValue = false;
+    vec3 x;
+    vec3 w;
+    float pdf_w = 0.0f;
+    float depth = 0.0f;
+    float density = 0.0f;
+};
+inline void recordFirst(ScatterEvent* fe, vec3 x, vec3 w, float pdf_w, float density, float depth) {
+    if (!fe || fe->hasValue) return;
+    fe->hasValue = true;
+    fe->x = x;
+    fe->w = w;
+    fe->pdf_w = pdf_w;
+    fe->density = density;
+    fe->depth = depth;
+}
+
 /* ---------------- Trackers ---------------- */
-vec3 deltaTracking(const Ctx& c, Rng& rng, vec3 x, vec3 w) {  // DeltaTracking.glsl:167-345
+vec3 deltaTracking(const Ctx& c, Rng& rng, vec3 x, vec3 w, ScatterEvent* fe = nullptr) {  // DeltaTracking.glsl:167-345
     const OracleScene* s = c.sc;
     float majorant = s->extinction[0];
     float absorptionAlbedo = 1.0f - s->scattering_albedo[0];
@@ -381,11 +400,15 @@ vec3 deltaTracking(const Ctx& c, Rng& rng, vec3 x, vec3 w) {  // DeltaTracking.g
             float Pa = sigma_a / majorant;
             float Pn = sigma_n / majorant;
             float xi = rng.random();
-            if (xi < Pa) return vec3(0.0f);
+            if (xi < Pa) {
+                recordFirst(fe, x, vec3(0.0f), 0.0f, density * s->extinction[0], tMax - d + t);
+                return vec3(0.0f);
+            }
             if (xi < 1 - Pn) {
                 if (s->max_bounces > 0 && ++bounces > s->max_bounces) return vec3(0.0f);
                 float pdf_w;
                 w = importanceSamplePhase(rng, s->phase_g, w, pdf_w);
+                recordFirst(fe, x, w, pdf_w, density * s->extinction[0], tMax - d + t);
                 if (rayBoxIntersect(c.boxMin, c.boxMax, x, w, tMin, tMax)) {
                     x += w * tMin;
                     d = tMax - tMin;
@@ -399,7 +422,7 @@ vec3 deltaTracking(const Ctx& c, Rng& rng, vec3 x, vec3 w) {  // DeltaTracking.g
     return sampleSkybox(c, w) + sampleLight(c, w);
 }
 
-vec3 ratioTracking(const Ctx& c, Rng& rng, vec3 x, vec3 w) {  // RatioTracking.glsl:25-96
+vec3 ratioTracking(const Ctx& c, Rng& rng, vec3 x, vec3 w, ScatterEvent* fe = nullptr) {  // RatioTracking.glsl:25-96
     const OracleScene* s = c.sc;
     float majorant = s->extinction[0];
     float PA = (1.0f - s->scattering_albedo[0]) * s->extinction[0];
@@ -427,6 +450,7 @@ vec3 ratioTracking(const Ctx& c, Rng& rng, vec3 x, vec3 w) {  // RatioTracking.g
                 transmittance *= 1.f - Pa / (Pa + Ps);
                 float pdf_w;
                 w = importanceSamplePhase(rng, s->phase_g, w, pdf_w);
+                recordFirst(fe, x, w, pdf_w, 0.0f, 0.0f);  // ratio tracking sets neither depth nor density
                 if (rayBoxIntersect(c.boxMin, c.boxMax, x, w, tMin, tMax)) {
                     x += w * tMin;
                     d = tMax - tMin;
@@ -439,7 +463,7 @@ vec3 ratioTracking(const Ctx& c, Rng& rng, vec3 x, vec3 w) {  // RatioTracking.g
     return transmittance * (sampleSkybox(c, w) + sampleLight(c, w));
 }
 
-vec3 nextEventTracking(const Ctx& c, Rng& rng, vec3 x, vec3 w) {  // NextEventTracking.glsl:223-394
+vec3 nextEventTracking(const Ctx& c, Rng& rng, vec3 x, vec3 w, ScatterEvent* fe = nullptr) {  // NextEventTracking.glsl:223-394
     const OracleScene* s = c.sc;
     float majorant = s->extinction[0];
     float PA = (1.0f - s->scattering_albedo[0]) * s->extinction[0];
@@ -461,11 +485,16 @@ vec3 nextEventTracking(const Ctx& c, Rng& rng, vec3 x, vec3 w) {  // NextEventTr
             float Pa = sigma_a / majorant;
             float Pn = sigma_n / majorant;
             float xi = rng.random();
-            if (xi < Pa) return color;
+            const float extMax = std::max(std::max(s->extinction[0], s->extinction[1]), s->extinction[2]);
+            if (xi < Pa) {
+                recordFirst(fe, x, w, 0.0f, density * extMax, tMax - d + t);
+                return color;
+            }
             if (xi < 1 - Pn) {
                 if (s->max_bounces > 0 && ++bounces > s->max_bounces) return color;
                 float pdf_w;
                 vec3 next_w = importanceSamplePhase(rng, s->phase_g, w, pdf_w);
+                recordFirst(fe, x, next_w, pdf_w, density * extMax, tMax - d + t);
                 hasValue = true;
                 float pdfLightNee = 0.0f;
                 vec3 dirLightNee;
@@ -522,7 +551,7 @@ inline float maxComponent(vec3 v) { return std::max(std::max(v.x, v.y), v.z); }
 inline float avgComponent(vec3 v) { return (v.x + v.y + v.z) / 3.0f; }
 inline vec3 min3(vec3 a, float b) { return {std::min(a.x, b), std::min(a.y, b), std::min(a.z, b)}; }
 
-vec3 deltaTrackingSpectral(const Ctx& c, Rng& rng, vec3 x, vec3 w) {  // DeltaTracking.glsl:33-164
+vec3 deltaTrackingSpectral(const Ctx& c, Rng& rng, vec3 x, vec3 w, ScatterEvent* fe = nullptr) {  // DeltaTracking.glsl:33-164
     const OracleScene* s = c.sc;
     const vec3 extinction(s->extinction), albedo(s->scattering_albedo);
     float majorant = maxComponent(extinction);
@@ -551,11 +580,15 @@ vec3 deltaTrackingSpectral(const Ctx& c, Rng& rng, vec3 x, vec3 w) {  // DeltaTr
             Ps /= C;
             Pn /= C;
             float xi = rng.random();
-            if (xi < Pa) return vec3(0.0f);
+            if (xi < Pa) {
+                recordFirst(fe, x, vec3(0.0f), 0.0f, density * majorant, tMax - d + t);
+                return vec3(0.0f);
+            }
             if (xi < Pa + Ps) {
                 if (s->max_bounces > 0 && ++bounces > s->max_bounces) return vec3(0.0f);
                 float pdf_w;
                 w = importanceSamplePhase(rng, s->phase_g, w, pdf_w);
+                recordFirst(fe, x, w, pdf_w, density * majorant, tMax - d + t);
                 if (rayBoxIntersect(c.boxMin, c.boxMax, x, w, tMin, tMax)) {
                     x += w * tMin;
                     d = tMax - tMin;
@@ -571,7 +604,7 @@ vec3 deltaTrackingSpectral(const Ctx& c, Rng& rng, vec3 x, vec3 w) {  // DeltaTr
     return min3(weights, 100000.0f) * (sampleSkybox(c, w) + sampleLight(c, w));
 }
 
-vec3 nextEventTrackingSpectral(const Ctx& c, Rng& rng, vec3 x, vec3 w) {  // NextEventTracking.glsl:30-221
+vec3 nextEventTrackingSpectral(const Ctx& c, Rng& rng, vec3 x, vec3 w, ScatterEvent* fe = nullptr) {  // NextEventTracking.glsl:30-221
     const OracleScene* s = c.sc;
     const vec3 extinction(s->extinction), albedo(s->scattering_albedo);
     float majorant = maxComponent(extinction);
@@ -602,11 +635,15 @@ vec3 nextEventTrackingSpectral(const Ctx& c, Rng& rng, vec3 x, vec3 w) {  // Nex
             Ps /= C;
             Pn /= C;
             float xi = rng.random();
-            if (xi < Pa) return color;
+            if (xi < Pa) {
+                recordFirst(fe, x, vec3(0.0f), 0.0f, density * majorant, tMax - d + t);
+                return color;
+            }
             if (xi < Pa + Ps) {
                 if (s->max_bounces > 0 && ++bounces > s->max_bounces) return color;
                 float pdf_w;
                 vec3 next_w = importanceSamplePhase(rng, s->phase_g, w, pdf_w);
+                recordFirst(fe, x, next_w, pdf_w, density * majorant, tMax - d + t);
                 hasValue = true;
                 float pdfLightNee = 0.0f;
                 vec3 dirLightNee;
@@ -936,15 +973,15 @@ void initCtx(Ctx& c, const OracleScene* s) {
     if (s->mode == OR_RESIDUAL_RATIO || s->mode == OR_DECOMPOSITION) buildSupervoxels(c, s->extinction[0]);
 }
 
-vec3 tracePath(const Ctx& c, Rng& rng, vec3 x, vec3 w) {
+vec3 tracePath(const Ctx& c, Rng& rng, vec3 x, vec3 w, ScatterEvent* fe = nullptr) {
     switch (c.sc->mode) {
-        case OR_RATIO: return ratioTracking(c, rng, x, w);
-        case OR_DECOMPOSITION: return analogDecompositionTracking(c, rng, x, w);
-        case OR_RESIDUAL_RATIO: return residualRatioTracking(c, rng, x, w);
-        case OR_NEXT_EVENT: return nextEventTracking(c, rng, x, w);
-        case OR_DELTA_SPECTRAL: return deltaTrackingSpectral(c, rng, x, w);
-        case OR_NEXT_EVENT_SPECTRAL: return nextEventTrackingSpectral(c, rng, x, w);
-        default: return deltaTracking(c, rng, x, w);
+        case OR_RATIO: return ratioTracking(c, rng, x, w, fe);
+        case OR_DECOMPOSITION: return analogDecompositionTracking(c, rng, x, w);  // records no first event
+        case OR_RESIDUAL_RATIO: return residualRatioTracking(c, rng, x, w);     // records no first event
+        case OR_NEXT_EVENT: return nextEventTracking(c, rng, x, w, fe);
+        case OR_DELTA_SPECTRAL: return deltaTrackingSpectral(c, rng, x, w, fe);
+        case OR_NEXT_EVENT_SPECTRAL: return nextEventTrackingSpectral(c, rng, x, w, fe);
+        default: return deltaTracking(c, rng, x, w, fe);
     }
 }
 
@@ -1268,6 +1305,201 @@ extern "C" int oracle_render(const OracleScene* scene, int width, int height, in
                 acc[0] = result.x;
                 acc[1] = result.y;
                 acc[2] = result.z;
+            }
+        }
+    }
+    return 0;
+}
+
+namespace {
+/* getCloudFiniteDifference, dense branch (VptUtils.glsl:726-748): central differences of the normalised grid one
+ * texel apart around the (stochastically jittered) texture coordinate, scaled by 100 / dim and normalised. */
+vec3 cloudFiniteDifference(const Ctx& c, Rng& rng, vec3 pos) {
+    vec3 coord = (pos - c.boxMin) / (c.boxMax - c.boxMin);
+    const vec3 dim(float(c.dim[0]), float(c.dim[1]), float(c.dim[2]));
+    if (c.sc->interpolation == OR_STOCHASTIC) {
+        float r0 = rng.random();
+        float r1 = rng.random();
+        float r2 = rng.random();
+        coord += vec3(r0 - 0.5f, r1 - 0.5f, r2 - 0.5f) / dim;
+    }
+    auto S = [&](vec3 q) { return (textureFetch(c, q) - c.vmin) / (c.vmax - c.vmin); };
+    vec3 d = vec3(S(coord - vec3(1, 0, 0) / dim) - S(coord + vec3(1, 0, 0) / dim),
+                  S(coord - vec3(0, 1, 0) / dim) - S(coord + vec3(0, 1, 0) / dim),
+                  S(coord - vec3(0, 0, 1) / dim) - S(coord + vec3(0, 0, 1) / dim)) / dim * 100.0f;
+    const float l = length(d);
+    if (l > 1e-5f) d = d / l;
+    return d;
+}
+inline void store(float* img, size_t px, int ch, const float* v) {
+    if (img)
+        for (int i = 0; i < ch; i++) img[px * ch + i] = v[i];
+}
+}  // namespace
+
+extern "C" int oracle_render_features(const OracleScene* scene, int width, int height, int frame_start,
+                                      int num_frames, const float prev_view_proj[16], float far_distance,
+                                      OracleFeatureMaps* m, int num_threads) {
+    if (!scene || !m || width <= 0 || height <= 0 || num_frames < 0) return 1;
+    Ctx c;
+    initCtx(c, scene);
+    const vec3 albedo(scene->scattering_albedo);
+#ifdef _OPENMP
+    if (num_threads > 0) omp_set_num_threads(num_threads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+    for (int y = 0; y < height; y++) {
+        for (int x = 0; x < width; x++) {
+            const size_t px = size_t(y) * width + x;
+            for (int f = frame_start; f < frame_start + num_frames; f++) {
+                // Clouds.glsl:73-383 for one sample with the enabled feature maps.
+                const uint32_t frame = uint32_t(f);
+                uint32_t seed = frame * uint32_t(width) * uint32_t(height) + uint32_t(x) + uint32_t(y) * uint32_t(width);
+                seed += scene->custom_seed_offset;
+                Rng rng;
+                rng.init(seed);
+                float r0 = rng.random();
+                float r1 = rng.random();
+                float sx = 2.0f * (float(x) + r0) / float(width) - 1;
+                float sy = 2.0f * (float(y) + r1) / float(height) - 1;
+                vec3 xo, w;
+                createCameraRay(c, sx, sy, xo, w);
+                ScatterEvent fe;
+                vec3 result = tracePath(c, rng, xo, w, &fe);
+                const float a = 1.0f / float(frame + 1);
+                auto mix4 = [&](float* img, const float* v) {
+                    float o[4] = {0, 0, 0, 0};
+                    if (frame != 0)
+                        for (int i = 0; i < 4; i++) o[i] = img[px * 4 + i];
+                    for (int i = 0; i < 4; i++) o[i] = mixf(o[i], v[i], a);
+                    store(img, px, 4, o);
+                };
+                if (m->result) {
+                    const float v[4] = {result.x, result.y, result.z, 1.0f};
+                    mix4(m->result, v);
+                }
+                if (m->cloud_only) {
+                    const float v[4] = {fe.hasValue ? result.x : 0.0f, fe.hasValue ? result.y : 0.0f,
+                                        fe.hasValue ? result.z : 0.0f, fe.hasValue ? 1.0f : 0.0f};
+                    mix4(m->cloud_only, v);
+                }
+                if (m->background) {
+                    const vec3 b = fe.hasValue ? sampleSkybox(c, w) : result;
+                    const float v[4] = {b.x, b.y, b.z, 1.0f};
+                    mix4(m->background, v);
+                }
+                if (m->first_x) {
+                    const float v[4] = {fe.hasValue ? fe.x.x : 0.0f, fe.hasValue ? fe.x.y : 0.0f,
+                                        fe.hasValue ? fe.x.z : 0.0f, fe.hasValue ? 1.0f : 0.0f};
+                    mix4(m->first_x, v);
+                }
+                if (m->depth) {  // (mean, std) with the second moment rebuilt from the stored std
+                    float o[2] = {0, 0};
+                    if (frame != 0) {
+                        o[0] = m->depth[px * 2];
+                        o[1] = m->depth[px * 2 + 1] * m->depth[px * 2 + 1] + o[0] * o[0];
+                    }
+                    const float v0 = fe.hasValue ? fe.depth : 0.0f, v1 = fe.hasValue ? fe.depth * fe.depth : 0.0f;
+                    const float n0 = mixf(o[0], v0, a), n1 = mixf(o[1], v1, a);
+                    const float v[2] = {n0, std::sqrt(std::max(0.0f, n1 - n0 * n0))};
+                    store(m->depth, px, 2, v);
+                }
+                if (m->depth_far) {
+                    const float o = frame == 0 ? 0.0f : m->depth_far[px];
+                    const float v = mixf(o, fe.hasValue ? fe.depth : far_distance, a);
+                    store(m->depth_far, px, 1, &v);
+                }
+                if (m->density) {
+                    float o[2] = {0, 0};
+                    if (frame != 0) {
+                        o[0] = m->density[px * 2];
+                        o[1] = m->density[px * 2 + 1] * m->density[px * 2 + 1] + o[0] * o[0];
+                    }
+                    const float v0 = fe.hasValue ? fe.density * 0.001f : 0.0f;
+                    const float v1 = fe.hasValue ? fe.density * fe.density * 0.001f * 0.001f : 0.0f;
+                    const float n0 = mixf(o[0], v0, a), n1 = mixf(o[1], v1, a);
+                    const float v[2] = {n0, std::sqrt(std::max(0.0f, n1 - n0 * n0))};
+                    store(m->density, px, 2, v);
+                }
+                if (m->reproj_uv) {
+                    float v[2] = {-1.0f, -1.0f};
+                    if (frame != 0) {
+                        v[0] = m->reproj_uv[px * 2];
+                        v[1] = m->reproj_uv[px * 2 + 1];
+                    }
+                    if (fe.hasValue) {
+                        const vec4 pc = mul(prev_view_proj, vec4{fe.x.x, fe.x.y, fe.x.z, 1.0f});
+                        v[0] = pc.x / pc.w * 0.5f + 0.5f;
+                        v[1] = pc.y / pc.w * 0.5f + 0.5f;
+                    }
+                    store(m->reproj_uv, px, 2, v);
+                }
+                if (m->albedo) {
+                    const float v[4] = {fe.hasValue ? albedo.x : 0.0f, fe.hasValue ? albedo.y : 0.0f,
+                                        fe.hasValue ? albedo.z : 0.0f, fe.hasValue ? 1.0f : 0.0f};
+                    mix4(m->albedo, v);
+                }
+                float nabla[2] = {0.0f, 0.0f};
+                const bool wantNormal = m->normal || m->depth_nabla || m->depth_fwidth;
+                if (fe.hasValue) {
+                    if (wantNormal) {
+                        vec3 diff = cloudFiniteDifference(c, rng, fe.x);
+                        vec3 old(0.0f);
+                        if (frame != 0 && m->normal) old = vec3(m->normal[px * 4], m->normal[px * 4 + 1], m->normal[px * 4 + 2]);
+                        diff = mix3(old, diff, 1.0f / (float(frame) + 1.0f));
+                        const float v[4] = {diff.x, diff.y, diff.z, 1.0f};
+                        store(m->normal, px, 4, v);
+                        // inverseTransposedViewMatrix * (diff, 0) = transpose(inverseView) * (diff, 0)
+                        const float* iv = scene->inverse_view;
+                        const float A = iv[0] * diff.x + iv[1] * diff.y + iv[2] * diff.z;
+                        const float B = iv[4] * diff.x + iv[5] * diff.y + iv[6] * diff.z;
+                        nabla[0] = A / std::sqrt(1.0f - A * A);
+                        nabla[1] = B / std::sqrt(1.0f - B * B);
+                    }
+                    const float v[4] = {fe.w.x, fe.w.y, fe.w.z, fe.pdf_w};
+                    store(m->first_w, px, 4, v);
+                } else {
+                    if (m->normal) {
+                        const float k = float(frame) / (float(frame) + 1.0f);
+                        const float v[4] = {frame == 0 ? 0.0f : m->normal[px * 4] * k,
+                                            frame == 0 ? 0.0f : m->normal[px * 4 + 1] * k,
+                                            frame == 0 ? 0.0f : m->normal[px * 4 + 2] * k, 1.0f};
+                        store(m->normal, px, 4, v);
+                    }
+                    const float z4[4] = {0, 0, 0, 0};
+                    store(m->first_w, px, 4, z4);
+                }
+                if (m->flow) {
+                    float v[2] = {0.0f, 0.0f};
+                    if (fe.hasValue) {
+                        vec4 q = mul(prev_view_proj, vec4{fe.x.x, fe.x.y, fe.x.z, 1.0f});
+                        const float lx = (0.5f * (q.x / q.w) + 0.5f) * float(width) - 0.5f;
+                        const float ly = (0.5f * (q.y / q.w) + 0.5f) * float(height) - 0.5f;
+                        v[0] = float(x) - lx;
+                        v[1] = float(y) - ly;
+                    }
+                    store(m->flow, px, 2, v);
+                }
+                if (m->flow_reverse) {
+                    const vec4 wp = fe.hasValue ? vec4{fe.x.x, fe.x.y, fe.x.z, 1.0f} : vec4{w.x, w.y, w.z, 0.0f};
+                    vec4 q = mul(prev_view_proj, wp);
+                    const float lx = (0.5f * (q.x / q.w) + 0.5f) * float(width) - 0.5f;
+                    const float ly = (0.5f * (q.y / q.w) + 0.5f) * float(height) - 0.5f;
+                    const float v[2] = {lx - float(x), ly - float(y)};
+                    store(m->flow_reverse, px, 2, v);
+                }
+                if (m->depth_nabla) {
+                    if (frame != 0) {
+                        nabla[0] = mixf(m->depth_nabla[px * 2], nabla[0], a);
+                        nabla[1] = mixf(m->depth_nabla[px * 2 + 1], nabla[1], a);
+                    }
+                    store(m->depth_nabla, px, 2, nabla);
+                }
+                if (m->depth_fwidth) {
+                    float v = std::fabs(nabla[0]) + std::fabs(nabla[1]);
+                    if (frame != 0) v = mixf(m->depth_fwidth[px], v, a);
+                    store(m->depth_fwidth, px, 1, &v);
+                }
             }
         }
     }

@@ -89,6 +89,34 @@ int oracle_render(const OracleScene* scene, int width, int height, int frame_sta
                   float* acc_rgb, double* sum_sq, int num_threads);
 
 /*
+ * Feature maps of Clouds.glsl:146-383 (WRITE_*_MAP): every pointer that is not null receives its map as height x
+ * width pixels of the reference image format's channels (RGBA: result, first_x, first_w, normal, cloud_only,
+ * background, albedo; RG: depth (mean, std), density (mean, std), reproj_uv, flow, flow_reverse, depth_nabla;
+ * R: depth_far, depth_fwidth) and is read back as the running state when frame_start > 0. prev_view_proj is the
+ * previous frame's view-projection matrix (column-major). Returns 0 on success.
+ */
+typedef struct OracleFeatureMaps {
+    float* result;
+    float* first_x;
+    float* first_w;
+    float* normal;
+    float* cloud_only;
+    float* background;
+    float* depth;
+    float* depth_far;
+    float* density;
+    float* reproj_uv;
+    float* albedo;
+    float* flow;
+    float* flow_reverse;
+    float* depth_nabla;
+    float* depth_fwidth;
+} OracleFeatureMaps;
+int oracle_render_features(const OracleScene* scene, int width, int height, int frame_start, int num_frames,
+                           const float prev_view_proj[16], float far_distance, OracleFeatureMaps* maps,
+                           int num_threads);
+
+/*
  * Transmittance queries: for ray i, origin o[3i..], unit direction d[3i..], segment [0, t_max[i]] clipped to the box.
  * Each estimator is evaluated n_samples times with hybrid-Taus streams seeded from seed + i*n_samples + s; the
  * mean and the second moment per ray are written to mean[i], m2[i].

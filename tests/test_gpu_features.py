@@ -1,0 +1,119 @@
+"""GPU checks of the feature maps (Clouds.glsl:146-383) against the oracle's restatement: exact relations between the
+maps of one first event, an opaque cube whose answers are geometric, and image means of every map within 3 sigma of
+the oracle's (four seed-offset batches on each side)."""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+RES = 24
+NAMES = ["Result", "First X", "First W", "Normal", "Cloud Only", "Depth", "Depth Far", "Density", "Background",
+         "Reprojected UV", "Albedo", "Flow", "Flow Reverse", "Depth (nabla)", "Depth (fwidth)"]
+ORACLE = {"First X": "first_x", "Normal": "normal", "Cloud Only": "cloud_only", "Depth": "depth",
+          "Depth Far": "depth_far", "Density": "density", "Background": "background", "Albedo": "albedo",
+          "Reprojected UV": "reproj_uv", "Flow Reverse": "flow_reverse", "Depth (fwidth)": "depth_fwidth"}
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def _maps(vpt, frames, names=NAMES, res=RES):
+    torch = _torch()
+    t = torch.zeros((4, res, res), dtype=torch.float32, device="cuda")
+    vpt.render_frame(t, frames)
+    return {n: vpt.get_feature_map_from_string(t, n).cpu().numpy() for n in names}
+
+
+def _setup(vpt, d, mode, interp="stochastic", ext=60.0, albedo=0.9):
+    vpt.set_density_field(d)
+    vpt.set_vpt_mode(mode)
+    vpt.set_grid_interpolation_type(interp)
+    vpt.set_extinction_scale(ext)
+    vpt.set_extinction_base([1.0, 1.0, 1.0])
+    vpt.set_scattering_albedo([albedo] * 3)
+    vpt.set_phase_g(0.5)
+    vpt.set_use_builtin_environment_map("Default")
+    vpt.set_use_lights(False)
+    vpt.set_seed_offset(0)
+    vpt.set_use_feature_maps(NAMES)
+
+
+def test_maps_share_one_first_event(vpt, oracle):
+    print("case: feature map relations delta 20^3", flush=True)
+    _setup(vpt, oracle.fbm_cloud(20, 20, 20, seed=31), 0)
+    m = _maps(vpt, 12)
+    cov = m["First X"][3]
+    assert 0.05 < cov.mean() < 0.95
+    np.testing.assert_allclose(m["Albedo"][3], cov, atol=1e-6)
+    np.testing.assert_allclose(m["Cloud Only"][3], cov, atol=1e-6)
+    np.testing.assert_allclose(m["Albedo"][0], 0.9 * cov, atol=1e-6)
+    np.testing.assert_allclose(m["Depth Far"][0], m["Depth"][0] + (1.0 - cov) * 80.0, rtol=1e-5, atol=1e-3)
+    torch = _torch()
+    vpt.set_seed_offset(0)
+    img = vpt.render_frame(torch.zeros((3, RES, RES), dtype=torch.float32, device="cuda"), 12).cpu().numpy()
+    res = vpt.get_feature_map_from_string(torch.zeros((3, RES, RES), device="cuda"), "Result").cpu().numpy()
+    np.testing.assert_array_equal(img, res)
+    vpt.set_feature_map_type(6)  # render_frame returns "Depth"
+    vpt.set_seed_offset(0)
+    dep = vpt.render_frame(torch.zeros((4, RES, RES), dtype=torch.float32, device="cuda"), 12).cpu().numpy()
+    vpt.set_feature_map_type(0)
+    np.testing.assert_allclose(dep[:2], m["Depth"][:2], rtol=1e-6, atol=1e-6)
+    assert np.all(dep[2] == 0.0) and np.all(dep[3] == 1.0)
+
+
+def test_opaque_cube(vpt, oracle):
+    print("case: feature maps opaque cube 16^3", flush=True)
+    n = 16
+    cube = np.ones((n, n, n), np.float32)
+    cube[0, 0, 0] = 0.0
+    _setup(vpt, cube, 0, interp="nearest", ext=2.0e4, albedo=0.0)
+    m = _maps(vpt, 8, res=16)
+    hit = m["First X"][3] == 1.0
+    assert hit.sum() > 20
+    fx = np.stack([m["First X"][i][hit] for i in range(3)], 1)
+    np.testing.assert_allclose(fx[:, 2], 0.25, atol=2e-3)
+    r = np.linalg.norm(fx - np.array([0.0, 0.0, 0.8]), axis=1)
+    cos = (0.8 - fx[:, 2]) / r
+    np.testing.assert_allclose(m["Depth"][0][hit], r - (1.0 / 512.0) / cos, atol=2e-3)
+    inner = hit & (np.abs(m["First X"][0]) < 0.2) & (np.abs(m["First X"][1]) < 0.2)
+    assert inner.sum() > 10
+    nrm = np.stack([m["Normal"][i][inner] for i in range(3)], 1)
+    np.testing.assert_allclose(nrm, np.broadcast_to([0.0, 0.0, 1.0], nrm.shape), atol=1e-5)
+    assert np.all(np.abs(m["Flow"][:2][:, hit]) <= 0.5 + 1e-4)
+    ys, xs = np.nonzero(hit)
+    u = m["Reprojected UV"][0][ys, xs] * 16
+    assert np.all(u >= xs - 1e-3) and np.all(u <= xs + 1 + 1e-3)
+
+
+@pytest.mark.parametrize("name,mode", [("delta", 0), ("next_event", 5)])
+def test_map_means_vs_oracle(vpt, oracle, name, mode):
+    print(f"case: feature map means vs oracle {name}", flush=True)
+    d = oracle.fbm_cloud(24, 20, 16, seed=33)
+    _setup(vpt, d, mode)
+    gpu = {k: [] for k in ORACLE}
+    for b in range(4):
+        vpt.set_seed_offset(1000003 * (b + 1))
+        m = _maps(vpt, 32, names=list(ORACLE))
+        for k in ORACLE:
+            gpu[k].append(m[k].reshape(m[k].shape[0], -1).mean(1))
+    vpt.set_seed_offset(0)
+    ref = {k: [] for k in ORACLE}
+    for b in range(4):
+        sc = oracle.Scene(d, mode=name, extinction=60.0, g=0.5, width=RES, height=RES, seed_offset=7919 * (b + 1))
+        o = sc.render_features(32, names=list(ORACLE.values()))
+        for k, on in ORACLE.items():
+            ref[k].append(o[on].reshape(-1, o[on].shape[-1]).mean(0))
+    for k in ORACLE:
+        g = np.array(gpu[k], np.float64)
+        r = np.array(ref[k], np.float64)
+        c = r.shape[1]
+        g = g[:, :c]
+        mg, sg = g.mean(0), g.std(0, ddof=1) / 2
+        mr, sr = r.mean(0), r.std(0, ddof=1) / 2
+        tol = 3.5 * np.sqrt(sg ** 2 + sr ** 2) + 1e-4 * (1 + np.abs(mr))
+        assert np.all(np.abs(mg - mr) <= tol), f"{name}/{k}: gpu {mg} oracle {mr} tol {tol}"

@@ -322,3 +322,54 @@ def test_spectral_white_furnace_per_channel(oracle):
     sky = np.array([2.6, 2.6 * 0.961538462, 2.6 * 0.884615385])
     z = (mean - sky) / np.sqrt(var / n)
     assert np.all(np.abs(z) < 4.0), (mean, sky, z)
+
+
+def _feature_scene(oracle, density, **kw):
+    args = dict(mode="delta", interpolation="nearest", extinction=2.0e4, albedo=0.9, g=0.5, width=24, height=24)
+    args.update(kw)
+    return oracle.Scene(density, **args)
+
+
+def test_feature_maps_share_one_first_event(oracle):
+    """Clouds.glsl writes every map from the same first event: the coverage channels of first X, albedo and cloud
+    only agree, albedo = scattering albedo x coverage, and depth far = depth + (1 - coverage) x far distance."""
+    d = oracle.fbm_cloud(20, 20, 20, seed=31)
+    sc = _feature_scene(oracle, d, extinction=60.0)
+    m = sc.render_features(12)
+    cov = m["first_x"][..., 3]
+    assert 0.05 < cov.mean() < 0.95
+    np.testing.assert_allclose(m["albedo"][..., 3], cov, atol=1e-6)
+    np.testing.assert_allclose(m["cloud_only"][..., 3], cov, atol=1e-6)
+    np.testing.assert_allclose(m["albedo"][..., 0], 0.9 * cov, atol=1e-6)
+    np.testing.assert_allclose(m["depth_far"][..., 0], m["depth"][..., 0] + (1.0 - cov) * 80.0, rtol=1e-5, atol=1e-4)
+    # results equal the plain render's running mean
+    img, _ = sc.render(12)
+    np.testing.assert_allclose(m["result"][..., :3], img, rtol=1e-6, atol=1e-7)
+
+
+def test_feature_maps_opaque_medium(oracle):
+    """A dense absorbing cube stops every ray at the face it enters (z = 0.25 for the camera at z = 0.8 looking down
+    -z): first X lies on that face, depth is the distance from the ray's near-plane origin, the normal is the
+    central difference of getCloudFiniteDifference across the face, (0, 0, 1) away from the cube's side edges (the
+    border outside the grid is 0), and with the previous view-projection equal to the current one the flow of a
+    pixel is its sub-pixel jitter offset (|flow| <= 0.5) and the reprojected UV lies in the pixel's footprint."""
+    n = 16
+    cube = np.ones((n, n, n), np.float32)
+    cube[0, 0, 0] = 0.0  # a non-constant field (normalisation range [0, 1]) that leaves the +z face untouched
+    sc = _feature_scene(oracle, cube, albedo=0.0, width=16, height=16)
+    m = sc.render_features(8)
+    hit = m["first_x"][..., 3] == 1.0
+    assert hit.sum() > 20
+    fx = m["first_x"][hit][:, :3]
+    np.testing.assert_allclose(fx[:, 2], 0.25, atol=2e-3)
+    r = np.linalg.norm(fx - np.array([0.0, 0.0, 0.8]), axis=1)
+    cos = (0.8 - fx[:, 2]) / r
+    np.testing.assert_allclose(m["depth"][..., 0][hit], r - (1.0 / 512.0) / cos, atol=2e-3)
+    inner = hit & (np.abs(m["first_x"][..., 0]) < 0.2) & (np.abs(m["first_x"][..., 1]) < 0.2)
+    assert inner.sum() > 10
+    nrm = m["normal"][inner][:, :3]
+    np.testing.assert_allclose(nrm, np.broadcast_to([0.0, 0.0, 1.0], nrm.shape), atol=1e-5)
+    assert np.all(np.abs(m["flow"][hit]) <= 0.5 + 1e-4)
+    ys, xs = np.nonzero(hit)
+    u = m["reproj_uv"][ys, xs, 0] * 16
+    assert np.all(u >= xs - 1e-3) and np.all(u <= xs + 1 + 1e-3)
