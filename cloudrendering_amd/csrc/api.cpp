@@ -188,6 +188,8 @@ struct Renderer {
     bool prevViewProjSet = false;    // setViewProjectionMatrixAsPrevious was called
     float prevViewProj[16] = {};
     int featW = 0, featH = 0;        // resolution of the feature state (0: none)
+    int svgfW = 0, svgfH = 0;        // resolution of the SVGF history (0: none yet)
+    int svgfIterations = 5;
     int eawIterations = 0;
     float eawPhiColor = 5.0f, eawPhiPosition = 0.1f, eawPhiNormal = 0.1f;
     int eawUseColorDistance = 0;
@@ -213,7 +215,7 @@ struct Renderer {
     // ---- device state ----
     hipStream_t stream = nullptr;
     DevBuf dRaw, dNorm, dNvdb, dMajorant, dRrt, dDecomp, dEnv, dAcc, dPaths, dCounter, dStage, dRays;
-    DevBuf dSvMin, dSvMax, dSvAvg, dDen0, dDen1, dStats, dFeat, dFeatAcc;
+    DevBuf dSvMin, dSvMax, dSvAvg, dDen0, dDen1, dStats, dFeat, dFeatAcc, dSvgf;
     bool sceneDirty = true;     // grid contents changed
     bool tablesDirty = true;    // supervoxel tables must be rebuilt
     bool envDirty = true;
@@ -516,7 +518,8 @@ Mat4 projection(const Renderer& r, int w, int h) { return perspective(r.fovy, do
 // Feature maps are traced whenever a map is requested, returned by render_frame, or read by the EAW denoiser
 // (COLOR + POSITION + NORMAL, EAWDenoiser.cpp:67-74).
 bool featuresActive(const Renderer& r) {
-    return !r.featureMaps.empty() || r.featureMapType != 0 || (r.denoiser == 1 && r.eawIterations > 0);
+    return !r.featureMaps.empty() || r.featureMapType != 0 || (r.denoiser == 1 && r.eawIterations > 0) ||
+           r.denoiser == 2;
 }
 
 FeatureAcc featureAcc(Renderer& r, uint32_t npix) {
@@ -696,7 +699,7 @@ int crvpt_cleanup(void) {
         (void)hipStreamSynchronize(r->stream);
         for (DevBuf* b : {&r->dRaw, &r->dNorm, &r->dNvdb, &r->dMajorant, &r->dRrt, &r->dDecomp, &r->dEnv, &r->dAcc,
                           &r->dPaths, &r->dCounter, &r->dStage, &r->dRays, &r->dSvMin, &r->dSvMax, &r->dSvAvg,
-                          &r->dDen0, &r->dDen1, &r->dStats, &r->dFeat, &r->dFeatAcc})
+                          &r->dDen0, &r->dDen1, &r->dStats, &r->dFeat, &r->dFeatAcc, &r->dSvgf})
             b->release();
         (void)hipEventDestroy(r->ev0);
         (void)hipEventDestroy(r->ev1);
@@ -746,8 +749,14 @@ int crvpt_render_frame(float* out, int channels, int height, int width, int fram
         HIP_TRY(hipMemsetAsync(r.dStats.ptr, 0, 4 * sizeof(unsigned long long), s));
         stats = r.dStats.as<unsigned long long>();
     }
+    // SVGF wants non-accumulated input and the global frame number (SVGF.hpp:81-82, DISABLE_ACCUMULATION): every
+    // frame overwrites the previous one, so only the call's last frame is traced.
+    const bool svgf = r.denoiser == 2;
+    const int traceFrames = svgf ? std::min(frame_count, 1) : frame_count;
+    const uint32_t skipFrames = uint32_t(frame_count - traceFrames);
+    if (svgf) r.frameCount = 0;
     const uint32_t maxPaths = 1u << 24;
-    const uint32_t framesPerLaunch = std::max(1u, std::min(uint32_t(std::max(frame_count, 1)), maxPaths / npix));
+    const uint32_t framesPerLaunch = std::max(1u, std::min(uint32_t(std::max(traceFrames, 1)), maxPaths / npix));
     HIP_TRY(r.dPaths.ensure(size_t(npix) * framesPerLaunch * sizeof(float4)));
     LaunchConfig cfg;
     cfg.numBlocks = uint32_t(launchBlocks(r));
@@ -764,7 +773,7 @@ int crvpt_render_frame(float* out, int channels, int height, int width, int fram
         r.featW = width;
         r.featH = height;
     }
-    const bool lateOut = denoise || r.featureMapType != 0;  // the output is written after the sample loop
+    const bool lateOut = denoise || svgf || r.featureMapType != 0;  // the output is written after the sample loop
     float* directOut = (out_on_device && !lateOut) ? out : nullptr;
     if (!out_on_device) {
         HIP_TRY(r.dStage.ensure(size_t(npix) * channels * sizeof(float)));
@@ -774,23 +783,24 @@ int crvpt_render_frame(float* out, int channels, int height, int width, int fram
     HIP_TRY(hipEventRecord(r.evT0, s));
     HIP_TRY(hipEventRecord(r.ev0, s));
     uint32_t done = 0;
-    while (done < uint32_t(frame_count)) {
-        const uint32_t f = std::min(framesPerLaunch, uint32_t(frame_count) - done);
-        P.frameBase = r.globalFrameNumber + done;
+    while (done < uint32_t(traceFrames)) {
+        const uint32_t f = std::min(framesPerLaunch, uint32_t(traceFrames) - done);
+        P.frameBase = r.globalFrameNumber + skipFrames + done;
         P.numFrames = f;
         P.numUnits = uint64_t(npix) * f;
         HIP_TRY(launchRender(P, sp ? nullptr : &dg, sp ? &ng : nullptr, S, r.dPaths.as<float4>(),
                              r.dCounter.as<uint32_t>(), cfg, stats, feat, s));
-        const bool last = done + f == uint32_t(frame_count);
+        const bool last = done + f == uint32_t(traceFrames);
         if (last) HIP_TRY(hipEventRecord(r.ev1, s));
         HIP_TRY(launchAccumulate(r.dPaths.as<float4>(), npix, f, r.dAcc.as<float4>(), r.frameCount + done,
                                  last ? directOut : nullptr, channels, s));
         if (features)
-            HIP_TRY(launchAccumulateFeatures(P, r.dPaths.as<float4>(), feat, npix, f, r.frameCount + done, facc, s));
+            HIP_TRY(launchAccumulateFeatures(P, r.dPaths.as<float4>(), feat, npix, f, r.frameCount + done, svgf,
+                                             facc, s));
         done += f;
         launches++;
     }
-    if (frame_count == 0) HIP_TRY(hipEventRecord(r.ev1, s));
+    if (traceFrames == 0) HIP_TRY(hipEventRecord(r.ev1, s));
     if (frame_count == 0 || lateOut) {
         float* target = out_on_device ? out : r.dStage.as<float>();
         if (frame_count == 0 && r.frameCount == 0) {
@@ -807,12 +817,27 @@ int crvpt_render_frame(float* out, int channels, int height, int width, int fram
             HIP_TRY(launchWriteCHW(res, npix, target, channels, s));
         } else if (r.featureMapType != 0) {
             HIP_TRY(launchFeatureOut(r.featureMapType, facc, r.dAcc.as<float4>(), npix, target, channels, s));
+        } else if (svgf && frame_count > 0) {
+            HIP_TRY(r.dSvgf.ensure(size_t(npix) * kSvgfBuffers * sizeof(float4)));
+            float4* b = r.dSvgf.as<float4>();
+            if (r.svgfW != width || r.svgfH != height) {
+                // new history: all zero (SVGF.cpp:244-268)
+                HIP_TRY(hipMemsetAsync(b, 0, size_t(npix) * 4 * sizeof(float4), s));
+                r.svgfW = width;
+                r.svgfH = height;
+            }
+            const SvgfState st{b, b + npix, b + 2 * size_t(npix), b + 3 * size_t(npix), b + 4 * size_t(npix),
+                               b + 5 * size_t(npix), b + 6 * size_t(npix), b + 7 * size_t(npix)};
+            const SvgfInputs in{r.dAcc.as<float4>(), facc.flow, facc.normal, facc.depth, width, height};
+            float4* res = nullptr;
+            HIP_TRY(launchSvgf(in, st, r.svgfIterations, &res, s));
+            HIP_TRY(launchWriteCHW(res, npix, target, channels, s));
         } else {
             HIP_TRY(launchWriteCHW(r.dAcc.as<float4>(), npix, target, channels, s));
         }
     }
     HIP_TRY(hipEventRecord(r.evT1, s));
-    r.frameCount += uint32_t(frame_count);
+    r.frameCount = svgf ? uint32_t(traceFrames) : r.frameCount + uint32_t(frame_count);
     r.globalFrameNumber += uint32_t(frame_count);
     r.lastPaths = uint64_t(npix) * uint64_t(frame_count);
     r.lastLaunches = launches;
@@ -1366,9 +1391,16 @@ int crvpt_set_denoiser(const char* name) {
     if (int rc = requireInit()) return rc;
     const std::string n = name ? name : "";
     Renderer& r = *g_r;
+    // DENOISER_NAMES (Denoiser.hpp:63-79) plus the short names "Default", "EAW"
     if (n == "None") r.denoiser = 0;
-    else if (n == "Default" || n == "EAW" || n == "Edge-Avoiding À-Trous Wavelet") r.denoiser = 1;
+    else if (n == "Default" || n == "EAW" || n == "Edge-Avoiding À-Trous Wavelet Transform" ||
+             n == "Edge-Avoiding À-Trous Wavelet")
+        r.denoiser = 1;
+    else if (n == "SVGF") r.denoiser = 2;
     else return fail(CRVPT_ERR_INVALID_ARGUMENT, "Error in setDenoiser: Invalid denoiser name \"" + n + "\".");
+    r.svgfW = r.svgfH = 0;  // a new denoiser starts without history (SVGF.cpp:244-268)
+    r.svgfIterations = 5;   // SVGF_ATrous_Pass::maxNumIterations (SVGF.hpp:160)
+    r.resetAccumulation();
     // VolumetricPathTracingModuleRenderer::setDenoiserType clears the settings (EAW defaults, EAWDenoiser.hpp:107-119)
     r.eawIterations = 0;
     r.eawPhiColor = 5.0f;
@@ -1382,7 +1414,10 @@ int crvpt_set_denoiser_property(const char* key_c, const char* value_c) {
     Renderer& r = *g_r;
     const std::string key = key_c ? key_c : "", value = value_c ? value_c : "";
     try {
-        if (key == "maxNumIterations" || key == "numIterations") r.eawIterations = std::stoi(value);
+        if (key == "maxNumIterations" || key == "numIterations") {
+            r.eawIterations = std::stoi(value);
+            r.svgfIterations = r.eawIterations;
+        }
         else if (key == "phiColor") r.eawPhiColor = std::stof(value);
         else if (key == "phiPosition") r.eawPhiPosition = std::stof(value);
         else if (key == "phiNormal") r.eawPhiNormal = std::stof(value);

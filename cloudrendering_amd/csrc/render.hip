@@ -819,7 +819,8 @@ __global__ void k_writeCHW(const float4* __restrict__ img, uint32_t npix, float*
 // Feature maps (Clouds.glsl:146-383): per pixel, the chunk's samples in order, each mixed into the running state
 // with weight 1 / (frame + 1) as the reference's imageLoad / mix / imageStore sequence does.
 __global__ void k_accumulateFeatures(SceneParams P, const float4* __restrict__ paths, const float4* __restrict__ feat,
-                                     uint32_t npix, uint32_t frames, uint32_t prevFrames, FeatureAcc A) {
+                                     uint32_t npix, uint32_t frames, uint32_t prevFrames, int noAccumulation,
+                                     FeatureAcc A) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npix) return;
     const float px = float(p % uint32_t(P.width)), py = float(p / uint32_t(P.width));
@@ -852,9 +853,9 @@ __global__ void k_accumulateFeatures(SceneParams P, const float4* __restrict__ p
         const V3 sky = sampleSky(P, w);
         background = mix4(background, has ? make_float4(sky.x, sky.y, sky.z, 1.0f) : make_float4(res.x, res.y, res.z, 1.0f));
         firstX = mix4(firstX, has ? make_float4(f0.x, f0.y, f0.z, 1.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f));
-        const float d = has ? f0.w : 0.0f;
+        const float d = has ? f0.w : (noAccumulation ? P.farDistance : 0.0f);
         depth.x = depth.x * b + d * a;
-        depth.y = depth.y * b + d * d * a;
+        depth.y = depth.y * b + (has || !noAccumulation ? d * d : 0.0f) * a;
         depth.z = depth.z * b + (has ? f0.w : P.farDistance) * a;
         const float dn = has ? f2.w * 0.001f : 0.0f;
         dens.x = dens.x * b + dn * a;
@@ -926,11 +927,10 @@ __global__ void k_featureOut(int map, FeatureAcc A, const float4* __restrict__ r
         case 1: v = A.firstX[p]; break;
         case 2: v = A.firstW[p]; break;
         case 3: v = A.normal[p]; break;
-        case 4: {
+        case 4: {  // NormalizeNormals.glsl: unit length where the length exceeds 1e-5
             const float4 n = A.normal[p];
             const float l = sqrtf(n.x * n.x + n.y * n.y + n.z * n.z);
-            const float k = l > 0.0f ? 1.0f / l : 0.0f;
-            v = make_float4(n.x * k, n.y * k, n.z * k, 1.0f);
+            v = l > 1e-5f ? make_float4(n.x / l, n.y / l, n.z / l, n.w) : n;
             break;
         }
         case 5: v = A.cloudOnly[p]; break;
@@ -1314,9 +1314,10 @@ hipError_t launchAccumulate(const float4* paths, uint32_t npix, uint32_t frames,
 }
 
 hipError_t launchAccumulateFeatures(const SceneParams& P, const float4* paths, const float4* feat, uint32_t npix,
-                                    uint32_t frames, uint32_t prevFrames, const FeatureAcc& acc, hipStream_t stream) {
+                                    uint32_t frames, uint32_t prevFrames, bool noAccumulation, const FeatureAcc& acc,
+                                    hipStream_t stream) {
     hipLaunchKernelGGL(k_accumulateFeatures, dim3((npix + 255) / 256), dim3(256), 0, stream, P, paths, feat, npix,
-                       frames, prevFrames, acc);
+                       frames, prevFrames, int(noAccumulation), acc);
     return hipGetLastError();
 }
 

@@ -49,8 +49,10 @@ hipError_t launchRender(const SceneParams& P, const DenseGrid* dense, const Nano
 hipError_t launchAccumulate(const float4* paths, uint32_t npix, uint32_t frames, float4* acc, uint32_t prevFrames,
                             float* outCHW, int channels, hipStream_t stream);
 hipError_t launchWriteCHW(const float4* img, uint32_t npix, float* outCHW, int channels, hipStream_t stream);
+// noAccumulation: the reference's DISABLE_ACCUMULATION maps (the depth of a ray without an event is the far plane).
 hipError_t launchAccumulateFeatures(const SceneParams& P, const float4* paths, const float4* feat, uint32_t npix,
-                                    uint32_t frames, uint32_t prevFrames, const FeatureAcc& acc, hipStream_t stream);
+                                    uint32_t frames, uint32_t prevFrames, bool noAccumulation, const FeatureAcc& acc,
+                                    hipStream_t stream);
 // Writes feature map `map` (FeatureMapTypeVpt order) as C planes of H*W floats; result = the image accumulator.
 hipError_t launchFeatureOut(int map, const FeatureAcc& acc, const float4* result, uint32_t npix, float* outCHW,
                             int channels, hipStream_t stream);
@@ -80,5 +82,26 @@ struct EawParams {
 };
 hipError_t launchEaw(const float4* color, const float4* position, const float4* normal, float4* ping, float4* pong,
                      const EawParams& p, float4** result, hipStream_t stream);
+
+// SVGF: this frame's maps and the history carried between frames (all W*H float4).
+struct SvgfInputs {
+    const float4* noisy;   // non-accumulated colour
+    const float4* flow;    // motion in .xy
+    const float4* normal;  // .xyz
+    const float4* depth;   // depth in .x, depth fwidth in .w
+    int width, height;
+};
+struct SvgfState {
+    float4* prevNormal;
+    float4* prevDepth;
+    float4* colorHistory;
+    float4* momentsHistory;
+    float4* accumMoments;
+    float4* tempA;
+    float4* tempB;
+    float4* pong;
+};
+constexpr int kSvgfBuffers = 8;
+hipError_t launchSvgf(const SvgfInputs& in, const SvgfState& st, int iterations, float4** result, hipStream_t stream);
 
 }  // namespace crvpt
