@@ -215,7 +215,7 @@ struct Renderer {
     // ---- device state ----
     hipStream_t stream = nullptr;
     DevBuf dRaw, dNorm, dNvdb, dMajorant, dRrt, dDecomp, dEnv, dAcc, dPaths, dCounter, dStage, dRays;
-    DevBuf dSvMin, dSvMax, dSvAvg, dDen0, dDen1, dStats, dFeat, dFeatAcc, dSvgf;
+    DevBuf dSvMin, dSvMax, dSvAvg, dDen0, dDen1, dStats, dFeat, dFeatAcc, dSvgf, dKpn;
     bool sceneDirty = true;     // grid contents changed
     bool tablesDirty = true;    // supervoxel tables must be rebuilt
     bool envDirty = true;
@@ -699,7 +699,7 @@ int crvpt_cleanup(void) {
         (void)hipStreamSynchronize(r->stream);
         for (DevBuf* b : {&r->dRaw, &r->dNorm, &r->dNvdb, &r->dMajorant, &r->dRrt, &r->dDecomp, &r->dEnv, &r->dAcc,
                           &r->dPaths, &r->dCounter, &r->dStage, &r->dRays, &r->dSvMin, &r->dSvMax, &r->dSvAvg,
-                          &r->dDen0, &r->dDen1, &r->dStats, &r->dFeat, &r->dFeatAcc, &r->dSvgf})
+                          &r->dDen0, &r->dDen1, &r->dStats, &r->dFeat, &r->dFeatAcc, &r->dSvgf, &r->dKpn})
             b->release();
         (void)hipEventDestroy(r->ev0);
         (void)hipEventDestroy(r->ev1);
@@ -918,6 +918,41 @@ int crvpt_set_view_projection_matrix_as_previous(void) {
     const Mat4 vp = mul(projection(r, w, h), r.view);
     for (int i = 0; i < 16; i++) r.prevViewProj[i] = float(vp.m[i]);
     r.prevViewProjSet = true;
+    return CRVPT_OK;
+}
+
+int crvpt_kpn_per_pixel_kernel(const float* image, const float* weights, float* out, int n, int c, int img_h,
+                                int img_w, int weight_h, int weight_w, int kernel_size, void* stream) {
+    if (int rc = requireInit()) return rc;
+    if (!image || !weights || !out || n <= 0 || c <= 0 || img_h <= 0 || img_w <= 0 || weight_h <= 0 ||
+        weight_w <= 0 || kernel_size <= 0 || kernel_size % 2 == 0)
+        return fail(CRVPT_ERR_INVALID_ARGUMENT, "kpn_per_pixel_kernel: invalid arguments");
+    if (c > 4) return fail(CRVPT_ERR_INVALID_ARGUMENT, "Error, must have <= 4 channels");
+    HIP_TRY(launchPerPixelKernel(image, weights, out, n, c, img_h, img_w, weight_h, weight_w, kernel_size,
+                                 reinterpret_cast<hipStream_t>(stream)));
+    return CRVPT_OK;
+}
+
+int crvpt_kpn_inv_iter_kernel(const float* x, const float* low_res_pred, const float* low_res_x, const float* prev,
+                              const float* fused_weights, float* out, int n, int c, int img_h, int img_w, int lo_h,
+                              int lo_w, int weight_c, int weight_h, int weight_w, int kernel_size, int use_previous,
+                              void* stream) {
+    if (int rc = requireInit()) return rc;
+    const int kk = kernel_size * kernel_size;
+    if (!x || !low_res_pred || !low_res_x || !fused_weights || !out || (use_previous && !prev) || n <= 0 || c <= 0 ||
+        kernel_size <= 0 || kernel_size % 2 == 0 || weight_c < kk + 3)
+        return fail(CRVPT_ERR_INVALID_ARGUMENT, "kpn_inv_iter_kernel: invalid arguments");
+    if (c > 4) return fail(CRVPT_ERR_INVALID_ARGUMENT, "Error, must have <= 4 channels");
+    Renderer& r = *g_r;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const int h = std::min(img_h, weight_h), w = std::min(img_w, weight_w);
+    HIP_TRY(r.dKpn.ensure((size_t(n) * c * h * w + size_t(n) * kk * weight_h * weight_w) * sizeof(float)));
+    float* base = r.dKpn.as<float>();
+    float* soft = base + size_t(n) * c * h * w;
+    HIP_TRY(launchInvIterBase(x, low_res_pred, low_res_x, prev, fused_weights, base, use_previous != 0, n, c, img_h,
+                              img_w, lo_h, lo_w, weight_h, weight_w, kernel_size, s));
+    HIP_TRY(launchSoftmaxChannels(fused_weights, soft, n, weight_c, kk, weight_h, weight_w, s));
+    HIP_TRY(launchPerPixelKernel(base, soft, out, n, c, h, w, weight_h, weight_w, kernel_size, s));
     return CRVPT_OK;
 }
 

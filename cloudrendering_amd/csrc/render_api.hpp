@@ -102,6 +102,14 @@ struct SvgfState {
     float4* pong;
 };
 constexpr int kSvgfBuffers = 8;
+
+// Kernel-prediction denoiser application (kpn.hip).
+hipError_t launchPerPixelKernel(const float* image, const float* weights, float* out, int N, int C, int imgH,
+                                int imgW, int wH, int wW, int K, hipStream_t stream);
+hipError_t launchInvIterBase(const float* hiX, const float* loPred, const float* loX, const float* prev,
+                             const float* weights, float* out, bool usePrevious, int N, int C, int imgH, int imgW,
+                             int loH, int loW, int wH, int wW, int K, hipStream_t stream);
+hipError_t launchSoftmaxChannels(const float* in, float* out, int N, int Ct, int kk, int H, int W, hipStream_t stream);
 hipError_t launchSvgf(const SvgfInputs& in, const SvgfState& st, int iterations, float4** result, hipStream_t stream);
 
 }  // namespace crvpt

@@ -33,7 +33,7 @@ __all__ = [
     "set_collect_stats", "get_last_render_counters", "build_nanovdb_from_dense", "grid_lookup",
     "grid_lookup_world", "get_supervoxel_tables", "set_max_bounces", "set_super_voxel_size",
     "set_use_feature_maps", "get_feature_map_from_string", "get_feature_map", "set_feature_map_type",
-    "set_view_projection_matrix_as_previous", "VPT_FEATURE_MAP_NAMES",
+    "set_view_projection_matrix_as_previous", "VPT_FEATURE_MAP_NAMES", "per_pixel_kernel_forward", "inv_iter_kernel",
     "VPT_MODE_NAMES", "CrvptError",
 ]
 
@@ -136,6 +136,51 @@ def get_feature_map_from_string(input_tensor, feature_map: str):
 
 def set_view_projection_matrix_as_previous() -> None:
     call("crvpt_set_view_projection_matrix_as_previous")
+
+
+def _dev4(t, name):
+    torch = _torch()
+    if t.dim() != 4:
+        raise CrvptError(f"Error in perPixelKernelCuda: {name}.sizes().size() != 4.")
+    if t.device.type != "cuda":
+        raise CrvptError("Error in perPixelKernelForward: Unsupported device.")
+    return t.to(torch.float32).contiguous()
+
+
+def per_pixel_kernel_forward(image, weights, kernel_size: int):
+    """modules/kpn_module per_pixel_kernel::forward: image [N, C<=4, H, W], weights [N, K*K, Hw, Ww] on the GPU ->
+    [N, C, min(H, Hw), min(W, Ww)]."""
+    torch = _torch()
+    image = _dev4(image, "image")
+    weights = _dev4(weights, "weights")
+    n, c, h, w = (int(v) for v in image.shape)
+    wh, ww = int(weights.shape[2]), int(weights.shape[3])
+    out = torch.zeros((n, c, min(h, wh), min(w, ww)), dtype=torch.float32, device=image.device)
+    stream = torch.cuda.current_stream(image.device).cuda_stream
+    call("crvpt_kpn_per_pixel_kernel", ctypes.c_void_p(image.data_ptr()), ctypes.c_void_p(weights.data_ptr()),
+         ctypes.c_void_p(out.data_ptr()), n, c, h, w, wh, ww, int(kernel_size), ctypes.c_void_p(stream))
+    return out
+
+
+def inv_iter_kernel(x, low_res_pred, low_res_x, prev, fused_weights, kernel_size: int, use_previous: bool):
+    """PyTorchDenoiser.cpp:174-179: blend base (getInvIterBase) then the per-pixel kernel with softmax over the
+    first K*K channels of fused_weights."""
+    torch = _torch()
+    x = _dev4(x, "image")
+    low_res_pred = _dev4(low_res_pred, "image")
+    low_res_x = _dev4(low_res_x, "image")
+    fused_weights = _dev4(fused_weights, "weights")
+    prev = _dev4(prev, "image") if prev is not None else x
+    n, c, h, w = (int(v) for v in x.shape)
+    lh, lw = int(low_res_x.shape[2]), int(low_res_x.shape[3])
+    wc, wh, ww = (int(v) for v in fused_weights.shape[1:])
+    out = torch.zeros((n, c, min(h, wh), min(w, ww)), dtype=torch.float32, device=x.device)
+    stream = torch.cuda.current_stream(x.device).cuda_stream
+    call("crvpt_kpn_inv_iter_kernel", ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(low_res_pred.data_ptr()),
+         ctypes.c_void_p(low_res_x.data_ptr()), ctypes.c_void_p(prev.data_ptr()),
+         ctypes.c_void_p(fused_weights.data_ptr()), ctypes.c_void_p(out.data_ptr()), n, c, h, w, lh, lw, wc, wh, ww,
+         int(kernel_size), int(bool(use_previous)), ctypes.c_void_p(stream))
+    return out
 
 
 def load_cloud_file(filename: str) -> None:

@@ -173,6 +173,21 @@ int crvpt_get_feature_map(int32_t map, float* out, int channels, int height, int
  * previous-frame matrix of the Flow / Flow Reverse / Reprojected UV maps (until then the current one is used). */
 int crvpt_set_view_projection_matrix_as_previous(void);
 
+/* modules/kpn_module per_pixel_kernel::forward (perPixelKernel.cpp:31-40, Kernel.cu:38-131) on device buffers:
+ * image [n, c, img_h, img_w], weights [n, k*k, weight_h, weight_w] (already normalised) -> out
+ * [n, c, min(img_h, weight_h), min(img_w, weight_w)]; c <= 4; stream is a hipStream_t. Device tensors only, as in
+ * the reference (which rejects CPU tensors). */
+int crvpt_kpn_per_pixel_kernel(const float* image, const float* weights, float* out, int n, int c, int img_h,
+                               int img_w, int weight_h, int weight_w, int kernel_size, void* stream);
+/* PyTorchDenoiser.cpp:174-179 inv_iter_kernel: getInvIterBase (Kernel.cu:137-186; sigmoid-gated blend of the
+ * low-resolution prediction, the previous frame and the detail, with the reference's weight batch stride) followed by
+ * the per-pixel kernel with softmax over the first k*k channels of fused_weights [n, weight_c >= k*k + 3, weight_h,
+ * weight_w]. x, prev: [n, c, img_h, img_w]; low_res_pred, low_res_x: [n, c, lo_h, lo_w]. */
+int crvpt_kpn_inv_iter_kernel(const float* x, const float* low_res_pred, const float* low_res_x, const float* prev,
+                              const float* fused_weights, float* out, int n, int c, int img_h, int img_w, int lo_h,
+                              int lo_w, int weight_c, int weight_h, int weight_w, int kernel_size, int use_previous,
+                              void* stream);
+
 /* No reference counterpart: statistics of the last crvpt_render_frame: kernel milliseconds of the path kernels
  * (device events on the render stream), number of paths, number of path-kernel launches. */
 int crvpt_get_last_render_stats(double* path_kernel_ms, double* total_ms, uint64_t* num_paths, int* launches);
