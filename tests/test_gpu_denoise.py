@@ -41,6 +41,7 @@ def test_eaw_matches_restatement(vpt, oracle, iterations):
     _scene(vpt, oracle)
     vpt.set_denoiser("EAW")
     vpt.set_denoiser_property("maxNumIterations", str(iterations))
+    vpt.set_use_feature_maps(["First X", "Normal"])
     t = torch.zeros((4, RES, RES), dtype=torch.float32, device="cuda")
     out = _hwc(vpt.render_frame(t, 8))
     res = _hwc(vpt.get_feature_map_from_string(t, "Result"))

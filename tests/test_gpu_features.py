@@ -1,6 +1,6 @@
 """GPU checks of the feature maps (Clouds.glsl:146-383) against the oracle's restatement: exact relations between the
-maps of one first event, an opaque cube whose answers are geometric, and image means of every map within 3 sigma of
-the oracle's (four seed-offset batches on each side)."""
+maps of one first event, an opaque cube whose answers are geometric, and image means of every map within 4 sigma of
+the oracle's (eight seed-offset batches on each side)."""
 import math
 
 import numpy as np
@@ -13,7 +13,9 @@ NAMES = ["Result", "First X", "First W", "Normal", "Cloud Only", "Depth", "Depth
          "Reprojected UV", "Albedo", "Flow", "Flow Reverse", "Depth (nabla)", "Depth (fwidth)"]
 ORACLE = {"First X": "first_x", "Normal": "normal", "Cloud Only": "cloud_only", "Depth": "depth",
           "Depth Far": "depth_far", "Density": "density", "Background": "background", "Albedo": "albedo",
-          "Reprojected UV": "reproj_uv", "Flow Reverse": "flow_reverse", "Depth (fwidth)": "depth_fwidth"}
+          "Reprojected UV": "reproj_uv", "Flow Reverse": "flow_reverse"}
+# "Depth (nabla)" / "Depth (fwidth)" divide by sqrt(1 - A^2) of the view-space normal and are infinite wherever a
+# normal lies along the view x or y axis, on the GPU as in the reference; they are checked on the opaque cube.
 
 
 def _torch():
@@ -85,6 +87,9 @@ def test_opaque_cube(vpt, oracle):
     nrm = np.stack([m["Normal"][i][inner] for i in range(3)], 1)
     np.testing.assert_allclose(nrm, np.broadcast_to([0.0, 0.0, 1.0], nrm.shape), atol=1e-5)
     assert np.all(np.abs(m["Flow"][:2][:, hit]) <= 0.5 + 1e-4)
+    # the +z face normal is the view axis: A = B = 0, so nabla and fwidth vanish
+    np.testing.assert_allclose(m["Depth (nabla)"][:2][:, inner], 0.0, atol=1e-5)
+    np.testing.assert_allclose(m["Depth (fwidth)"][0][inner], 0.0, atol=1e-5)
     ys, xs = np.nonzero(hit)
     u = m["Reprojected UV"][0][ys, xs] * 16
     assert np.all(u >= xs - 1e-3) and np.all(u <= xs + 1 + 1e-3)
@@ -96,14 +101,15 @@ def test_map_means_vs_oracle(vpt, oracle, name, mode):
     d = oracle.fbm_cloud(24, 20, 16, seed=33)
     _setup(vpt, d, mode)
     gpu = {k: [] for k in ORACLE}
-    for b in range(4):
+    nb = 8
+    for b in range(nb):
         vpt.set_seed_offset(1000003 * (b + 1))
         m = _maps(vpt, 32, names=list(ORACLE))
         for k in ORACLE:
             gpu[k].append(m[k].reshape(m[k].shape[0], -1).mean(1))
     vpt.set_seed_offset(0)
     ref = {k: [] for k in ORACLE}
-    for b in range(4):
+    for b in range(nb):
         sc = oracle.Scene(d, mode=name, extinction=60.0, g=0.5, width=RES, height=RES, seed_offset=7919 * (b + 1))
         o = sc.render_features(32, names=list(ORACLE.values()))
         for k, on in ORACLE.items():
@@ -113,7 +119,7 @@ def test_map_means_vs_oracle(vpt, oracle, name, mode):
         r = np.array(ref[k], np.float64)
         c = r.shape[1]
         g = g[:, :c]
-        mg, sg = g.mean(0), g.std(0, ddof=1) / 2
-        mr, sr = r.mean(0), r.std(0, ddof=1) / 2
-        tol = 3.5 * np.sqrt(sg ** 2 + sr ** 2) + 1e-4 * (1 + np.abs(mr))
+        mg, sg = g.mean(0), g.std(0, ddof=1) / math.sqrt(nb)
+        mr, sr = r.mean(0), r.std(0, ddof=1) / math.sqrt(nb)
+        tol = 4.0 * np.sqrt(sg ** 2 + sr ** 2) + 1e-4 * (1 + np.abs(mr))
         assert np.all(np.abs(mg - mr) <= tol), f"{name}/{k}: gpu {mg} oracle {mr} tol {tol}"
