@@ -269,7 +269,15 @@ CRVPT_DEV float density(const Grid& g, const SceneParams& P, Rng& rng, V3 p, typ
 constexpr uint32_t kEventSpace = 0x80000000u;
 constexpr uint32_t kCameraBlock = 0xFFFFFFFFu;
 
-enum : uint32_t { PF_ALIVE = 1u, PF_EXHAUSTED = 2u, PF_HAS_VALUE = 4u, PF_SHADOW = 8u, PF_FIRST = 16u };
+enum : uint32_t {
+    PF_ALIVE = 1u,
+    PF_EXHAUSTED = 2u,
+    PF_HAS_VALUE = 4u,
+    PF_SHADOW = 8u,
+    PF_FIRST = 16u,
+    PF_PENDING = 32u,   // the lane waits for the wave to resolve its segment end or real collision together
+    PF_PEND_END = 64u,  // the pending event is a segment end (else a real scattering collision)
+};
 constexpr uint32_t kFeatureBlock = 0xFFFFFFFEu;  // Philox block of the first event's finite-difference jitter
 // flags bits 8..31 count the scattering events of the main path (in-scattering iterations for residual ratio
 // tracking)
@@ -289,6 +297,7 @@ struct PathState {
     V3 mainX, mainW;   // where the main path resumes after the shadow ray
     float wSum, resT, resDist;  // residual ratio tracking reservoir
     V3 weights;        // spectral tracking path weights
+    float pendRho;     // density at the pending real collision
 };
 
 CRVPT_DEV uint4 pathBlock(const SceneParams& P, const PathState& st, uint32_t c0) {
@@ -566,7 +575,10 @@ CRVPT_DEV bool stepPath(PathState& st, Walker& W, const SceneParams& P, const Su
     if (MODE == VPT_RESIDUAL_RATIO_TRACKING) {
         st.thr *= __expf(-W.ctrl);  // control transmittance of the walked stretch
         W.ctrl = 0.0f;
-        if (ev == WALK_END) return rrtTraversalEnd(st, W, P, S, out);
+        if (ev == WALK_END) {
+            st.flags |= PF_PENDING | PF_PEND_END;
+            return true;
+        }
         const float rho = grid.lookupWorldJ(P, W.o + W.w * W.t, jitterHi(r.y), jitterLo(r.y), jitterHi(r.z), gc);
         st.taps++;
         st.thr *= 1.0f - (P.extinction * rho - W.muC) * rcpf(W.mu);
@@ -577,7 +589,11 @@ CRVPT_DEV bool stepPath(PathState& st, Walker& W, const SceneParams& P, const Su
         }
         return true;
     }
-    if (ev == WALK_END) return endEvent<MODE>(st, W, P, S, out);
+    if (ev == WALK_END) {
+        if (isSpectral<MODE>()) return endEvent<MODE>(st, W, P, S, out);
+        st.flags |= PF_PENDING | PF_PEND_END;
+        return true;
+    }
     const V3 p = W.o + W.w * W.t;
     const float xi = u01(r.w);
     if (MODE == VPT_DECOMPOSITION_TRACKING) {
@@ -592,7 +608,9 @@ CRVPT_DEV bool stepPath(PathState& st, Walker& W, const SceneParams& P, const Su
             writePath(out, st.unit, v3(0.0f), true);
             return false;
         }
-        return scatterEvent<MODE>(st, W, P, S, p, 0.0f, 0.0f, grid, feat, out);
+        st.pendRho = 0.0f;
+        st.flags |= PF_PENDING;
+        return true;
     }
     const float rho = grid.lookupWorldJ(P, p, jitterHi(r.y), jitterLo(r.y), jitterHi(r.z), gc);
     st.taps++;
@@ -602,7 +620,7 @@ CRVPT_DEV bool stepPath(PathState& st, Walker& W, const SceneParams& P, const Su
     if (isNee<MODE>() && (st.flags & PF_SHADOW)) {
         st.thr *= 1.0f - pReal;  // ratio tracking (NeeTransmittance.glsl:107-113)
         // Once the product is exactly 0 every later factor leaves it 0 and the shadow ray adds exactly 0.
-        if (st.thr == 0.0f) return endEvent<MODE>(st, W, P, S, out);
+        if (st.thr == 0.0f) st.flags |= PF_PENDING | PF_PEND_END;
         return true;
     }
     if (!(xi < pReal)) return true;  // null collision
@@ -618,7 +636,27 @@ CRVPT_DEV bool stepPath(PathState& st, Walker& W, const SceneParams& P, const Su
         writePath(out, st.unit, isNee<MODE>() ? st.color : v3(0.0f), true);
         return false;
     }
-    return scatterEvent<MODE>(st, W, P, S, p, pAbs / pReal, rho, grid, feat, out);
+    st.pendRho = rho;
+    st.flags |= PF_PENDING;
+    return true;
+}
+
+// Resolves a lane's pending segment end or real scattering collision (the walker still stands at the event).
+// Deferring these rarer, longer branches lets a wave run them once for many lanes instead of once per step for
+// whichever lane happens to need them; the path's random numbers are counter-based, so the result is the same.
+template <int MODE, class Grid>
+CRVPT_DEV bool resolvePending(PathState& st, Walker& W, const SceneParams& P, const SupervoxelGrid& S,
+                              const Grid& grid, float4* out, float4* feat) {
+    const bool end = (st.flags & PF_PEND_END) != 0;
+    st.flags &= ~(PF_PENDING | PF_PEND_END);
+    if (end) {
+        if (MODE == VPT_RESIDUAL_RATIO_TRACKING) return rrtTraversalEnd(st, W, P, S, out);
+        return endEvent<MODE>(st, W, P, S, out);
+    }
+    const V3 p = W.o + W.w * W.t;
+    // analog ratio tracking weights a real collision by its scattering probability, the albedo
+    return scatterEvent<MODE>(st, W, P, S, p, MODE == VPT_RATIO_TRACKING ? 1.0f - P.albedo : 0.0f, st.pendRho,
+                              grid, feat, out);
 }
 
 // Wave-reduced 64-bit work counters: [0] density taps, [1] supervoxel majorant / table reads, [2] paths started,
@@ -640,6 +678,11 @@ CRVPT_DEV void flushCounters(unsigned long long* stats, uint32_t taps, uint32_t 
         atomicAdd(&stats[3], d);
     }
 }
+
+#ifndef CRVPT_RESOLVE_LANES
+#define CRVPT_RESOLVE_LANES 16
+#endif
+constexpr int kResolveLanes = CRVPT_RESOLVE_LANES;  // pending lanes that trigger a resolve pass of the wave
 
 template <int MODE, class Grid>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CRVPT_PATHPOOL_WAVES))) void k_pathPool(
@@ -678,8 +721,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CRVPT_PATHP
         }
         if (__ballot((st.flags & PF_ALIVE) != 0) == 0) break;  // every lane is exhausted
         for (int k = 0; k < stepsPerRefill; k++) {
-            if (st.flags & PF_ALIVE) {
+            if ((st.flags & (PF_ALIVE | PF_PENDING)) == PF_ALIVE) {
                 if (!stepPath<MODE, Grid>(st, W, P, S, grid, gc, out, feat)) st.flags = 0;
+            }
+            const uint64_t pend = __ballot((st.flags & PF_PENDING) != 0);
+            if (pend != 0ull && (__popcll(pend) >= kResolveLanes || k == stepsPerRefill - 1)) {
+                if (st.flags & PF_PENDING) {
+                    if (!resolvePending<MODE, Grid>(st, W, P, S, grid, out, feat)) st.flags = 0;
+                }
             }
         }
     }
