@@ -308,8 +308,14 @@ CRVPT_DEV float uOpen(uint32_t x) { return float((x >> 8) + 1u) * (1.0f / 167772
 CRVPT_DEV float jitterHi(uint32_t x) { return (float(x >> 16) + 0.5f) * (1.0f / 65536.0f) - 0.5f; }
 CRVPT_DEV float jitterLo(uint32_t x) { return (float(x & 0xFFFFu) + 0.5f) * (1.0f / 65536.0f) - 0.5f; }
 
-CRVPT_DEV void writePath(float4* out, uint32_t unit, V3 c, bool hasValue) {
-    out[unit] = make_float4(c.x, c.y, c.z, hasValue ? 1.0f : 0.0f);
+// A finished path leaves its radiance in st.color and whether it met the medium in PF_HAS_VALUE; the kernel that
+// runs it stores the result (one record per path, or a sum over the samples of a batch unit).
+CRVPT_DEV void finishPath(PathState& st, V3 c, bool hasValue) {
+    st.color = c;
+    st.flags = hasValue ? PF_HAS_VALUE : 0u;
+}
+CRVPT_DEV void writePath(float4* out, const PathState& st) {
+    out[st.unit] = make_float4(st.color.x, st.color.y, st.color.z, (st.flags & PF_HAS_VALUE) ? 1.0f : 0.0f);
 }
 
 // Uniform direction on the sphere from two uniforms, in the basis of d (randomDirection).
@@ -344,8 +350,7 @@ CRVPT_DEV void cameraRay(const SceneParams& P, const PathState& st, V3& x, V3& w
 }
 
 template <int MODE>
-CRVPT_DEV bool startPath(PathState& st, Walker& W, const SceneParams& P, const SupervoxelGrid& S, float4* out,
-                         float4* feat) {
+CRVPT_DEV bool startPath(PathState& st, Walker& W, const SceneParams& P, const SupervoxelGrid& S, float4* feat) {
     const uint32_t npix = uint32_t(P.width) * uint32_t(P.height);
     const uint32_t s = st.unit / npix;
     st.pix = st.unit - s * npix;
@@ -363,8 +368,7 @@ CRVPT_DEV bool startPath(PathState& st, Walker& W, const SceneParams& P, const S
     V3 x, w;
     cameraRay(P, st, x, w);
     if (!walkerStart<MODE>(W, P, S, x, w, kInf)) {
-        writePath(out, st.unit, skyRadiance(P, w), false);
-        st.flags = 0;
+        finishPath(st, skyRadiance(P, w), false);
         return false;
     }
     return true;
@@ -412,7 +416,7 @@ CRVPT_DEV void recordFirst(PathState& st, const SceneParams& P, const Grid& grid
 // Real scattering collision at p of the main path (rare compared with tentative collisions).
 template <int MODE, class Grid>
 CRVPT_DEV bool scatterEvent(PathState& st, Walker& W, const SceneParams& P, const SupervoxelGrid& S, V3 p,
-                            float pAbsOverReal, float rho, const Grid& grid, float4* feat, float4* out) {
+                            float pAbsOverReal, float rho, const Grid& grid, float4* feat) {
     const uint4 e = pathBlock(P, st, kEventSpace | st.events++);
     st.flags += 1u << kBounceShift;
     const V3 nw = phaseDirection(P.phaseG, W.w, u01(e.x), u01(e.y));
@@ -466,7 +470,7 @@ CRVPT_DEV bool scatterEvent(PathState& st, Walker& W, const SceneParams& P, cons
     if (MODE == VPT_DECOMPOSITION_TRACKING) {
         // a real collision absorbs with probability 1 - albedo (DecompositionTracking.glsl:92-98)
         if (u01(e.z) < 1.0f - P.albedo) {
-            writePath(out, st.unit, v3(0.0f), true);
+            finishPath(st, v3(0.0f), true);
             return false;
         }
     }
@@ -474,7 +478,7 @@ CRVPT_DEV bool scatterEvent(PathState& st, Walker& W, const SceneParams& P, cons
         const V3 c = isNee<MODE>() ? st.color
                                    : skyRadiance(P, nw) * (isSpectral<MODE>() ? vmin(st.weights, 100000.0f)
                                                                               : v3(st.thr));
-        writePath(out, st.unit, c, true);
+        finishPath(st, c, true);
         return false;
     }
     return true;
@@ -482,12 +486,12 @@ CRVPT_DEV bool scatterEvent(PathState& st, Walker& W, const SceneParams& P, cons
 
 // End of the current segment: a finished shadow ray resumes its main path, a main path leaves the volume.
 template <int MODE>
-CRVPT_DEV bool endEvent(PathState& st, Walker& W, const SceneParams& P, const SupervoxelGrid& S, float4* out) {
+CRVPT_DEV bool endEvent(PathState& st, Walker& W, const SceneParams& P, const SupervoxelGrid& S) {
     if (isNee<MODE>() && (st.flags & PF_SHADOW)) {
         st.color += st.shadowScale * st.thr;
         st.flags &= ~PF_SHADOW;
         if (walkerStart<MODE>(W, P, S, st.mainX, st.mainW, kInf)) return true;
-        writePath(out, st.unit, st.color, true);
+        finishPath(st, st.color, true);
         return false;
     }
     const bool hv = (st.flags & PF_HAS_VALUE) != 0;
@@ -495,21 +499,20 @@ CRVPT_DEV bool endEvent(PathState& st, Walker& W, const SceneParams& P, const Su
     const V3 wts = isSpectral<MODE>() ? vmin(st.weights, 100000.0f) : v3(st.thr);
     if (isNee<MODE>()) c = hv ? st.color : st.color + skyRadiance(P, W.w) * wts;
     else c = skyRadiance(P, W.w) * wts;
-    writePath(out, st.unit, c, hv);
+    finishPath(st, c, hv);
     return false;
 }
 
 // Residual ratio tracking: end of one traversal of the box (ResidualRatioTracking.glsl:196-219). The traversal's
 // sky term is added; the reservoir sample becomes the next in-scattering point unless xi > wSum or ten iterations
 // were made.
-CRVPT_DEV bool rrtTraversalEnd(PathState& st, Walker& W, const SceneParams& P, const SupervoxelGrid& S,
-                               float4* out) {
+CRVPT_DEV bool rrtTraversalEnd(PathState& st, Walker& W, const SceneParams& P, const SupervoxelGrid& S) {
     constexpr int MODE = VPT_RESIDUAL_RATIO_TRACKING;
     const uint4 e = pathBlock(P, st, kEventSpace | st.events++);
     st.color += skyRadiance(P, W.w) * st.thr;
     const uint32_t iteration = st.flags >> kBounceShift;
     if (u01(e.x) > st.wSum || iteration >= 10u) {
-        writePath(out, st.unit, st.color, true);
+        finishPath(st, st.color, true);
         return false;
     }
     st.flags += 1u << kBounceShift;
@@ -521,7 +524,7 @@ CRVPT_DEV bool rrtTraversalEnd(PathState& st, Walker& W, const SceneParams& P, c
     st.resDist = 0.0f;
     if (!walkerStart<MODE>(W, P, S, x, nw, kInf)) {
         st.color += skyRadiance(P, nw) * st.thr;
-        writePath(out, st.unit, st.color, true);
+        finishPath(st, st.color, true);
         return false;
     }
     return true;
@@ -533,7 +536,7 @@ CRVPT_DEV bool rrtTraversalEnd(PathState& st, Walker& W, const SceneParams& P, c
 // chosen probability and are clamped to 100.
 template <int MODE, class Grid>
 CRVPT_DEV bool spectralCollision(PathState& st, Walker& W, const SceneParams& P, const SupervoxelGrid& S, V3 p,
-                                 float rho, float xi, const Grid& grid, float4* feat, float4* out) {
+                                 float rho, float xi, const Grid& grid, float4* feat) {
     const V3 ext = v3(P.extinctionRGB) * rho;
     const V3 alb = v3(P.albedoRGB);
     const V3 sa = (v3(1.0f) - alb) * ext;
@@ -546,18 +549,18 @@ CRVPT_DEV bool spectralCollision(PathState& st, Walker& W, const SceneParams& P,
     Pn /= C;
     if (xi < Pa) {
         recordFirst<MODE>(st, P, grid, feat, p, v3(0.0f), 0.0f, rho * P.extinctionMax, W.t);
-        writePath(out, st.unit, isNee<MODE>() ? st.color : v3(0.0f), true);
+        finishPath(st, isNee<MODE>() ? st.color : v3(0.0f), true);
         return false;
     }
     if (xi < Pa + Ps) {
         if (P.maxBounces != 0u && (st.flags >> kBounceShift) >= P.maxBounces) {
-            writePath(out, st.unit, isNee<MODE>() ? st.color : v3(0.0f), true);
+            finishPath(st, isNee<MODE>() ? st.color : v3(0.0f), true);
             return false;
         }
         st.weights = st.weights * (ss / v3(W.mu * Ps));
         // next-event estimation uses the weights before the clamp, as the reference's loop clamps at its end
         if (!isNee<MODE>()) st.weights = vmin(st.weights, 100.0f);
-        const bool alive = scatterEvent<MODE>(st, W, P, S, p, 0.0f, rho, grid, feat, out);
+        const bool alive = scatterEvent<MODE>(st, W, P, S, p, 0.0f, rho, grid, feat);
         st.weights = vmin(st.weights, 100.0f);
         return alive;
     }
@@ -568,7 +571,7 @@ CRVPT_DEV bool spectralCollision(PathState& st, Walker& W, const SceneParams& P,
 // One step of a live path. Returns false when the path finished (its radiance is written).
 template <int MODE, class Grid>
 CRVPT_DEV bool stepPath(PathState& st, Walker& W, const SceneParams& P, const SupervoxelGrid& S, const Grid& grid,
-                        typename Grid::Cache& gc, float4* out, float4* feat) {
+                        typename Grid::Cache& gc, float4* feat) {
     const uint4 r = pathBlock(P, st, st.step++);
     const int ev = walkAdvance<MODE>(W, P, S, -__logf(uOpen(r.x)));
     if (ev == WALK_MOVED) return true;
@@ -590,7 +593,7 @@ CRVPT_DEV bool stepPath(PathState& st, Walker& W, const SceneParams& P, const Su
         return true;
     }
     if (ev == WALK_END) {
-        if (isSpectral<MODE>()) return endEvent<MODE>(st, W, P, S, out);
+        if (isSpectral<MODE>()) return endEvent<MODE>(st, W, P, S);
         st.flags |= PF_PENDING | PF_PEND_END;
         return true;
     }
@@ -605,7 +608,7 @@ CRVPT_DEV bool stepPath(PathState& st, Walker& W, const SceneParams& P, const Su
             if (!(xm < P.extinction * rho)) return true;
         }
         if (P.maxBounces != 0u && (st.flags >> kBounceShift) >= P.maxBounces) {
-            writePath(out, st.unit, v3(0.0f), true);
+            finishPath(st, v3(0.0f), true);
             return false;
         }
         st.pendRho = 0.0f;
@@ -616,7 +619,7 @@ CRVPT_DEV bool stepPath(PathState& st, Walker& W, const SceneParams& P, const Su
     st.taps++;
     const float pReal = P.extinction * rho * rcpf(W.mu);
     if (isSpectral<MODE>() && !(st.flags & PF_SHADOW))
-        return spectralCollision<MODE>(st, W, P, S, p, rho, xi, grid, feat, out);
+        return spectralCollision<MODE>(st, W, P, S, p, rho, xi, grid, feat);
     if (isNee<MODE>() && (st.flags & PF_SHADOW)) {
         st.thr *= 1.0f - pReal;  // ratio tracking (NeeTransmittance.glsl:107-113)
         // Once the product is exactly 0 every later factor leaves it 0 and the shadow ray adds exactly 0.
@@ -633,7 +636,7 @@ CRVPT_DEV bool stepPath(PathState& st, Walker& W, const SceneParams& P, const Su
             recordFirst<MODE>(st, P, grid, feat, p, isNee<MODE>() ? W.w : v3(0.0f), 0.0f,
                               rho * (isNee<MODE>() ? P.extinctionMax : P.extinction), W.t);
         }
-        writePath(out, st.unit, isNee<MODE>() ? st.color : v3(0.0f), true);
+        finishPath(st, isNee<MODE>() ? st.color : v3(0.0f), true);
         return false;
     }
     st.pendRho = rho;
@@ -646,17 +649,17 @@ CRVPT_DEV bool stepPath(PathState& st, Walker& W, const SceneParams& P, const Su
 // whichever lane happens to need them; the path's random numbers are counter-based, so the result is the same.
 template <int MODE, class Grid>
 CRVPT_DEV bool resolvePending(PathState& st, Walker& W, const SceneParams& P, const SupervoxelGrid& S,
-                              const Grid& grid, float4* out, float4* feat) {
+                              const Grid& grid, float4* feat) {
     const bool end = (st.flags & PF_PEND_END) != 0;
     st.flags &= ~(PF_PENDING | PF_PEND_END);
     if (end) {
-        if (MODE == VPT_RESIDUAL_RATIO_TRACKING) return rrtTraversalEnd(st, W, P, S, out);
-        return endEvent<MODE>(st, W, P, S, out);
+        if (MODE == VPT_RESIDUAL_RATIO_TRACKING) return rrtTraversalEnd(st, W, P, S);
+        return endEvent<MODE>(st, W, P, S);
     }
     const V3 p = W.o + W.w * W.t;
     // analog ratio tracking weights a real collision by its scattering probability, the albedo
     return scatterEvent<MODE>(st, W, P, S, p, MODE == VPT_RATIO_TRACKING ? 1.0f - P.albedo : 0.0f, st.pendRho,
-                              grid, feat, out);
+                              grid, feat);
 }
 
 // Wave-reduced 64-bit work counters: [0] density taps, [1] supervoxel majorant / table reads, [2] paths started,
@@ -715,19 +718,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CRVPT_PATHP
                     exhausted = true;
                 } else {
                     paths++;
-                    startPath<MODE>(st, W, P, S, out, feat);
+                    if (!startPath<MODE>(st, W, P, S, feat)) writePath(out, st);
                 }
             }
         }
         if (__ballot((st.flags & PF_ALIVE) != 0) == 0) break;  // every lane is exhausted
         for (int k = 0; k < stepsPerRefill; k++) {
             if ((st.flags & (PF_ALIVE | PF_PENDING)) == PF_ALIVE) {
-                if (!stepPath<MODE, Grid>(st, W, P, S, grid, gc, out, feat)) st.flags = 0;
+                if (!stepPath<MODE, Grid>(st, W, P, S, grid, gc, feat)) {
+                    writePath(out, st);
+                    st.flags = 0;
+                }
             }
             const uint64_t pend = __ballot((st.flags & PF_PENDING) != 0);
             if (pend != 0ull && (__popcll(pend) >= kResolveLanes || k == stepsPerRefill - 1)) {
                 if (st.flags & PF_PENDING) {
-                    if (!resolvePending<MODE, Grid>(st, W, P, S, grid, out, feat)) st.flags = 0;
+                    if (!resolvePending<MODE, Grid>(st, W, P, S, grid, feat)) {
+                        writePath(out, st);
+                        st.flags = 0;
+                    }
                 }
             }
         }
