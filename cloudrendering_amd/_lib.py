@@ -86,6 +86,13 @@ SIGNATURES = {
     "crvpt_get_last_render_stats": (_c_int, [_pd, _pd, _pu64, _pi]),
     "crvpt_set_collect_stats": (_c_int, [_c_int]),
     "crvpt_get_last_render_counters": (_c_int, [_pu64]),
+    "crvpt_batch_clear": (_c_int, []),
+    "crvpt_batch_add_request": (_c_int, [_p, _c_int, _c_int, _c_int, _c_int, _pi]),
+    "crvpt_batch_render": (_c_int, [_c_u32, _p]),
+    "crvpt_batch_set_launch_config": (_c_int, [_c_int, _c_int, _c_int, _c_int]),
+    "crvpt_batch_get_launch_config": (_c_int, [_pi]),
+    "crvpt_batch_autotune": (_c_int, [_c_int, _pi]),
+    "crvpt_batch_get_stats": (_c_int, [_pd, _pu64, _pi]),
 }
 
 _lib = None

@@ -18,7 +18,7 @@ CSRC = os.path.join(PKG_DIR, "csrc")
 BUILD_DIR = os.environ.get("CRVPT_BUILD_DIR", os.path.join(PKG_DIR, "_build"))
 LIB_PATH = os.environ.get("CRVPT_LIBRARY", os.path.join(PKG_DIR, "libcrvpt.so"))
 
-HIP_SOURCES = ["render.hip", "denoise.hip", "kpn.hip"]
+HIP_SOURCES = ["render.hip", "batch.hip", "denoise.hip", "kpn.hip"]
 CXX_SOURCES = ["api.cpp", "nanovdb_build.cpp"]
 
 ARCH = os.environ.get("CRVPT_OFFLOAD_ARCH", "gfx950")

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -228,6 +229,8 @@ struct Renderer {
     int refTablesInterp = -1;
     double refTablesExtinction = -1.0;
     int cuCount = 256;
+    uint64_t volumeVersion = 0;  // bumped whenever the grid contents change (batched requests snapshot the volume)
+    uint64_t envVersion = 0;     // bumped whenever an environment image is uploaded
 
     // ---- accumulation ----
     int accW = 0, accH = 0, accC = 0;
@@ -312,6 +315,7 @@ int setDenseHost(Renderer& r, int nx, int ny, int nz, std::vector<float>&& value
     computeDenseBounds(r);
     HIP_TRY(r.dRaw.ensure(r.denseHost.size() * sizeof(float)));
     HIP_TRY(hipMemcpy(r.dRaw.ptr, r.denseHost.data(), r.denseHost.size() * sizeof(float), hipMemcpyHostToDevice));
+    r.volumeVersion++;
     r.sceneDirty = r.tablesDirty = true;
     r.refTablesValid = false;
     r.resetAccumulation();
@@ -338,6 +342,7 @@ int setSparseHost(Renderer& r, std::vector<uint8_t>&& buf, bool fromDense) {
     computeSparseBounds(r, fromDense);
     HIP_TRY(r.dNvdb.ensure(r.nvdbHost.size()));
     HIP_TRY(hipMemcpy(r.dNvdb.ptr, r.nvdbHost.data(), r.nvdbHost.size(), hipMemcpyHostToDevice));
+    r.volumeVersion++;
     r.sceneDirty = r.tablesDirty = true;
     r.refTablesValid = false;
     r.resetAccumulation();
@@ -397,6 +402,7 @@ int uploadEnv(Renderer& r) {
         HIP_TRY(hipDeviceSynchronize());
         HIP_TRY(r.dEnv.ensure(r.envImage.size() * sizeof(float)));
         HIP_TRY(hipMemcpy(r.dEnv.ptr, r.envImage.data(), r.envImage.size() * sizeof(float), hipMemcpyHostToDevice));
+        r.envVersion++;
     }
     r.envDirty = false;
     return CRVPT_OK;
@@ -650,6 +656,8 @@ Light defaultLight() {
     return L;
 }
 
+void releaseBatch();  // batched queue state (defined with the batch entry points)
+
 }  // namespace
 }  // namespace crvpt
 
@@ -697,6 +705,7 @@ int crvpt_cleanup(void) {
     if (g_refCount == 0 && g_r) {
         Renderer* r = g_r;
         (void)hipStreamSynchronize(r->stream);
+        releaseBatch();
         for (DevBuf* b : {&r->dRaw, &r->dNorm, &r->dNvdb, &r->dMajorant, &r->dRrt, &r->dDecomp, &r->dEnv, &r->dAcc,
                           &r->dPaths, &r->dCounter, &r->dStage, &r->dRays, &r->dSvMin, &r->dSvMax, &r->dSvAvg,
                           &r->dDen0, &r->dDen1, &r->dStats, &r->dFeat, &r->dFeatAcc, &r->dSvgf, &r->dKpn})
@@ -1606,6 +1615,437 @@ int crvpt_set_launch_config(int supervoxel_size, int blocks, int steps_per_refil
     }
     r.blocksSetting = blocks;
     r.stepsPerRefill = steps_per_refill > 0 ? steps_per_refill : 4;
+    return CRVPT_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------------------------
+// Batched render queue (no reference counterpart): requests snapshot the renderer's current scene (volume in its
+// active representation with its supervoxel tables, camera, medium, phase, lights, environment, seed offset and
+// bounce limit) and are traced together by one persistent launch captured in a HIP graph.
+// ------------------------------------------------------------------------------------------------------------
+namespace crvpt {
+namespace {
+
+struct BatchVolume {
+    DevBuf grid, majorant, rrt, decomp;
+    DenseGrid dense{};
+    NanoGrid nano{};
+    SupervoxelGrid S{};
+    ~BatchVolume() {
+        grid.release();
+        majorant.release();
+        rrt.release();
+        decomp.release();
+    }
+};
+
+struct BatchEnvImage {
+    DevBuf img;
+    ~BatchEnvImage() { img.release(); }
+};
+
+// Identity of the renderer's active volume and tables: a request reuses the previous request's snapshot when the
+// identity is unchanged.
+struct VolumeKey {
+    uint64_t version = ~0ull;
+    int sparse = -1, svSize = 0, refSvSize = 0, interp = -1, needRef = 0;
+    double extinction = -1.0;
+    bool operator==(const VolumeKey& o) const {
+        return version == o.version && sparse == o.sparse && svSize == o.svSize && refSvSize == o.refSvSize &&
+               interp == o.interp && needRef == o.needRef && extinction == o.extinction;
+    }
+};
+
+struct Batch {
+    std::vector<std::unique_ptr<BatchVolume>> volumes;
+    std::vector<std::unique_ptr<BatchEnvImage>> envs;
+    std::vector<BatchRequest> reqs;
+    int mode = -1;
+    VolumeKey lastVolume;
+    uint64_t lastEnvVersion = ~0ull;
+    BatchLaunch cfg{0, 256, 8, 16};
+    bool blocksAuto = true;  // numBlocks follows the occupancy of the block size
+    DevBuf dReqs, dUnitBegins, dPixBegins, dRecords, dCounter, dFrame;
+    uint32_t totalUnits = 0;
+    uint64_t totalPix = 0, totalPaths = 0;
+    bool tableDirty = true;
+    hipStream_t captureStream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timed = false;
+
+    void dropGraph() {
+        if (exec) (void)hipDeviceSynchronize();  // a launch of the graph may still run
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+    ~Batch() {
+        dropGraph();
+        for (DevBuf* b : {&dReqs, &dUnitBegins, &dPixBegins, &dRecords, &dCounter, &dFrame}) b->release();
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (captureStream) (void)hipStreamDestroy(captureStream);
+    }
+};
+
+Batch* g_batch = nullptr;
+
+void releaseBatch() {
+    if (!g_batch) return;
+    (void)hipDeviceSynchronize();  // kernels on any stream may still read the snapshots
+    delete g_batch;
+    g_batch = nullptr;
+}
+
+int requireBatch() {
+    if (int rc = requireInit()) return rc;
+    if (!g_batch) {
+        Batch* b = new Batch();
+        if (hipStreamCreateWithFlags(&b->captureStream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
+            delete b;
+            return fail(CRVPT_ERR_HIP, "crvpt: cannot create the batch stream or events");
+        }
+        g_batch = b;
+    }
+    return CRVPT_OK;
+}
+
+hipError_t copyDevice(DevBuf& dst, const void* src, size_t bytes, hipStream_t s) {
+    hipError_t e = dst.ensure(bytes);
+    if (e != hipSuccess) return e;
+    return hipMemcpyAsync(dst.ptr, src, bytes, hipMemcpyDeviceToDevice, s);
+}
+
+// Snapshot of the renderer's active volume and the tables of its mode (prepareScene has run).
+int snapshotVolume(Renderer& r, BatchVolume& v) {
+    const bool sp = sparseActive(r);
+    DenseGrid dg;
+    NanoGrid ng;
+    bool spv;
+    gridViews(r, dg, ng, spv);
+    if (sp) {
+        HIP_TRY(copyDevice(v.grid, r.dNvdb.ptr, r.nvdbHost.size(), r.stream));
+        ng.buf = v.grid.as<uint8_t>();
+        v.nano = ng;
+    } else {
+        const size_t n = size_t(r.dims[0]) * r.dims[1] * r.dims[2];
+        HIP_TRY(copyDevice(v.grid, r.dNorm.ptr, n * sizeof(float), r.stream));
+        dg.values = v.grid.as<float>();
+        v.dense = dg;
+    }
+    const size_t nm = size_t(r.svGrid[0]) * r.svGrid[1] * r.svGrid[2];
+    HIP_TRY(copyDevice(v.majorant, r.dMajorant.ptr, nm * sizeof(float), r.stream));
+    v.S.majorant = v.majorant.as<float>();
+    v.S.size = r.svSize;
+    v.S.refSize = r.refSvSize;
+    for (int a = 0; a < 3; a++) {
+        v.S.grid[a] = r.svGrid[a];
+        v.S.refGrid[a] = r.refGrid[a];
+    }
+    if (r.mode == VPT_RESIDUAL_RATIO_TRACKING || r.mode == VPT_DECOMPOSITION_TRACKING) {
+        const size_t nr = size_t(r.refGrid[0]) * r.refGrid[1] * r.refGrid[2];
+        HIP_TRY(copyDevice(v.rrt, r.dRrt.ptr, 2 * nr * sizeof(float), r.stream));
+        HIP_TRY(copyDevice(v.decomp, r.dDecomp.ptr, 2 * nr * sizeof(float), r.stream));
+        v.S.rrt = v.rrt.as<float>();
+        v.S.decomp = v.decomp.as<float>();
+    }
+    HIP_TRY(hipStreamSynchronize(r.stream));
+    return CRVPT_OK;
+}
+
+// Unit and pixel ranges of the requests for G samples per unit, uploaded with the request table.
+int uploadBatchTable(Batch& b, uint32_t samplesPerUnit) {
+    const size_t n = b.reqs.size();
+    std::vector<uint64_t> unitBegins(n), pixBegins(n);
+    uint64_t units = 0, pix = 0, paths = 0;
+    for (size_t i = 0; i < n; i++) {
+        BatchRequest& q = b.reqs[i];
+        const uint64_t npix = uint64_t(q.P.width) * uint64_t(q.P.height);
+        q.groups = (q.spp + samplesPerUnit - 1) / samplesPerUnit;
+        q.unitBegin = units;
+        q.pixBegin = pix;
+        unitBegins[i] = units;
+        pixBegins[i] = pix;
+        units += npix * q.groups;
+        pix += npix;
+        paths += npix * q.spp;
+    }
+    if (units >= (1ull << 32))
+        return fail(CRVPT_ERR_INVALID_ARGUMENT,
+                    "batch: more than 2^32 units; raise the samples per unit or split the batch");
+    HIP_TRY(b.dReqs.ensure(n * sizeof(BatchRequest)));
+    HIP_TRY(b.dUnitBegins.ensure(n * sizeof(uint64_t)));
+    HIP_TRY(b.dPixBegins.ensure(n * sizeof(uint64_t)));
+    HIP_TRY(b.dRecords.ensure(std::max<uint64_t>(units, 1) * sizeof(float4)));
+    HIP_TRY(b.dCounter.ensure(64));
+    HIP_TRY(b.dFrame.ensure(64));
+    HIP_TRY(hipMemcpy(b.dReqs.ptr, b.reqs.data(), n * sizeof(BatchRequest), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b.dUnitBegins.ptr, unitBegins.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b.dPixBegins.ptr, pixBegins.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice));
+    b.totalUnits = uint32_t(units);
+    b.totalPix = pix;
+    b.totalPaths = paths;
+    return CRVPT_OK;
+}
+
+BatchLaunch resolvedLaunch(const Batch& b, const Renderer& r, BatchLaunch c) {
+    if (c.numBlocks == 0) c.numBlocks = uint32_t(r.cuCount * batchOccupancyBlocksPerCU(b.mode, c.blockSize));
+    return c;
+}
+
+hipError_t enqueueBatch(Batch& b, const BatchLaunch& c, hipStream_t s) {
+    return launchBatch(b.mode, b.dReqs.as<BatchRequest>(), b.dUnitBegins.as<uint64_t>(), b.dPixBegins.as<uint64_t>(),
+                       int(b.reqs.size()), b.totalUnits, b.totalPix, c, b.dFrame.as<uint32_t>(),
+                       b.dRecords.as<float4>(), b.dCounter.as<uint32_t>(), nullptr, s);
+}
+
+bool validBatchLaunch(const BatchLaunch& c) {
+    return c.blockSize >= 64 && c.blockSize <= 256 && c.blockSize % 64 == 0 && c.stepsPerRefill > 0 &&
+           c.samplesPerUnit > 0;
+}
+
+}  // namespace
+}  // namespace crvpt
+
+extern "C" {
+
+int crvpt_batch_clear(void) {
+    if (int rc = requireInit()) return rc;
+    releaseBatch();
+    return CRVPT_OK;
+}
+
+int crvpt_batch_add_request(float* out, int channels, int height, int width, int spp, int32_t* request_id) {
+    if (int rc = requireBatch()) return rc;
+    if (channels != 3 && channels != 4)
+        return fail(CRVPT_ERR_INVALID_ARGUMENT, "batch_add_request: the number of channels must be 3 or 4");
+    if (!out || height <= 0 || width <= 0 || spp <= 0)
+        return fail(CRVPT_ERR_INVALID_ARGUMENT, "batch_add_request: empty image, no samples or null output");
+    Renderer& r = *g_r;
+    Batch& b = *g_batch;
+    if (b.mode >= 0 && b.mode != r.mode)
+        return fail(CRVPT_ERR_INVALID_ARGUMENT, "batch_add_request: all requests of a batch use one estimator");
+    if (int rc = prepareScene(r)) return rc;
+    HIP_TRY(hipStreamSynchronize(r.stream));
+    VolumeKey key;
+    key.version = r.volumeVersion;
+    key.sparse = sparseActive(r) ? 1 : 0;
+    key.svSize = r.svSize;
+    key.refSvSize = r.refSvSize;
+    key.needRef = (r.mode == VPT_RESIDUAL_RATIO_TRACKING || r.mode == VPT_DECOMPOSITION_TRACKING) ? 1 : 0;
+    key.interp = key.needRef ? r.refTablesInterp : -1;
+    key.extinction = key.needRef ? r.refTablesExtinction : -1.0;
+    if (b.volumes.empty() || !(key == b.lastVolume)) {
+        std::unique_ptr<BatchVolume> v(new BatchVolume());
+        if (int rc = snapshotVolume(r, *v)) return rc;
+        b.volumes.push_back(std::move(v));
+        b.lastVolume = key;
+    }
+    const BatchVolume& vol = *b.volumes.back();
+    BatchRequest q;
+    std::memset(&q, 0, sizeof(q));
+    SupervoxelGrid S;
+    fillSceneParams(r, width, height, q.P, S);
+    if (q.P.envKind == ENV_IMAGE) {
+        if (b.envs.empty() || b.lastEnvVersion != r.envVersion) {
+            std::unique_ptr<BatchEnvImage> e(new BatchEnvImage());
+            HIP_TRY(copyDevice(e->img, r.dEnv.ptr, r.envImage.size() * sizeof(float), r.stream));
+            HIP_TRY(hipStreamSynchronize(r.stream));
+            b.envs.push_back(std::move(e));
+            b.lastEnvVersion = r.envVersion;
+        }
+        q.P.envImage = b.envs.back()->img.as<float>();
+    }
+    q.P.frameBase = 0;
+    q.P.numFrames = uint32_t(spp);
+    q.P.numUnits = uint64_t(width) * uint64_t(height) * uint64_t(spp);
+    q.S = vol.S;
+    q.dense = vol.dense;
+    q.nano = vol.nano;
+    q.spp = uint32_t(spp);
+    q.out = out;
+    q.channels = channels;
+    b.reqs.push_back(q);
+    b.mode = r.mode;
+    b.tableDirty = true;
+    b.dropGraph();
+    if (request_id) *request_id = int32_t(b.reqs.size() - 1);
+    return CRVPT_OK;
+}
+
+int crvpt_batch_set_launch_config(int blocks, int block_size, int steps_per_refill, int samples_per_unit) {
+    if (int rc = requireBatch()) return rc;
+    BatchLaunch c{uint32_t(std::max(blocks, 0)), block_size, steps_per_refill, uint32_t(std::max(samples_per_unit, 0))};
+    if (blocks < 0 || !validBatchLaunch(c))
+        return fail(CRVPT_ERR_INVALID_ARGUMENT,
+                    "batch_set_launch_config: block size 64..256 in steps of 64, positive steps and samples per unit");
+    Batch& b = *g_batch;
+    b.cfg = c;
+    b.blocksAuto = blocks == 0;
+    b.tableDirty = true;
+    b.dropGraph();
+    return CRVPT_OK;
+}
+
+int crvpt_batch_get_launch_config(int32_t config[4]) {
+    if (int rc = requireBatch()) return rc;
+    if (!config) return fail(CRVPT_ERR_INVALID_ARGUMENT, "batch_get_launch_config: null output");
+    Batch& b = *g_batch;
+    const BatchLaunch c = b.mode >= 0 ? resolvedLaunch(b, *g_r, b.blocksAuto ? BatchLaunch{0, b.cfg.blockSize,
+                                                                                          b.cfg.stepsPerRefill,
+                                                                                          b.cfg.samplesPerUnit}
+                                                                         : b.cfg)
+                                      : b.cfg;
+    config[0] = int32_t(c.numBlocks);
+    config[1] = c.blockSize;
+    config[2] = c.stepsPerRefill;
+    config[3] = int32_t(c.samplesPerUnit);
+    return CRVPT_OK;
+}
+
+int crvpt_batch_render(uint32_t frame_offset, void* stream) {
+    if (int rc = requireBatch()) return rc;
+    Batch& b = *g_batch;
+    if (b.reqs.empty()) return fail(CRVPT_ERR_INVALID_ARGUMENT, "batch_render: the batch holds no request");
+    Renderer& r = *g_r;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    BatchLaunch c = b.cfg;
+    if (b.blocksAuto) c.numBlocks = 0;
+    c = resolvedLaunch(b, r, c);
+    if (b.tableDirty) {
+        HIP_TRY(hipDeviceSynchronize());  // a previous launch may still read the table
+        if (int rc = uploadBatchTable(b, c.samplesPerUnit)) return rc;
+        b.tableDirty = false;
+        b.dropGraph();
+    }
+    if (!b.exec) {
+        // zero the counter, the persistent pool, the resolve: one graph, replayed while the batch is unchanged
+        HIP_TRY(hipStreamBeginCapture(b.captureStream, hipStreamCaptureModeThreadLocal));
+        const hipError_t le = enqueueBatch(b, c, b.captureStream);
+        hipGraph_t g = nullptr;
+        const hipError_t ce = hipStreamEndCapture(b.captureStream, &g);
+        if (le != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return fail(CRVPT_ERR_HIP, std::string("batch_render: launch: ") + hipGetErrorString(le));
+        }
+        HIP_TRY(ce);
+        b.graph = g;
+        HIP_TRY(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
+    }
+    HIP_TRY(launchSetU32(b.dFrame.as<uint32_t>(), frame_offset, s));
+    HIP_TRY(hipEventRecord(b.ev0, s));
+    HIP_TRY(hipGraphLaunch(b.exec, s));
+    HIP_TRY(hipEventRecord(b.ev1, s));
+    b.timed = true;
+    return CRVPT_OK;
+}
+
+int crvpt_batch_get_stats(double* kernel_ms, uint64_t* num_paths, int32_t* num_requests) {
+    if (int rc = requireBatch()) return rc;
+    Batch& b = *g_batch;
+    if (kernel_ms) {
+        float ms = 0.0f;
+        if (b.timed) {
+            HIP_TRY(hipEventSynchronize(b.ev1));
+            HIP_TRY(hipEventElapsedTime(&ms, b.ev0, b.ev1));
+        }
+        *kernel_ms = ms;
+    }
+    if (num_paths) {
+        uint64_t p = 0;
+        for (const BatchRequest& q : b.reqs) p += uint64_t(q.P.width) * uint64_t(q.P.height) * q.spp;
+        *num_paths = p;
+    }
+    if (num_requests) *num_requests = int32_t(b.reqs.size());
+    return CRVPT_OK;
+}
+
+int crvpt_batch_autotune(int spp_divisor, int32_t config[4]) {
+    if (int rc = requireBatch()) return rc;
+    Batch& b = *g_batch;
+    if (b.reqs.empty()) return fail(CRVPT_ERR_INVALID_ARGUMENT, "batch_autotune: the batch holds no request");
+    if (spp_divisor < 1) return fail(CRVPT_ERR_INVALID_ARGUMENT, "batch_autotune: spp_divisor must be >= 1");
+    Renderer& r = *g_r;
+    HIP_TRY(hipDeviceSynchronize());
+    b.dropGraph();
+    // Tuning batch: every request with spp / spp_divisor samples (at least 1) into its own output.
+    std::vector<uint32_t> spp(b.reqs.size());
+    for (size_t i = 0; i < b.reqs.size(); i++) {
+        spp[i] = b.reqs[i].spp;
+        b.reqs[i].spp = std::max<uint32_t>(1u, spp[i] / uint32_t(spp_divisor));
+    }
+    hipStream_t s = b.captureStream;
+    auto restore = [&]() {
+        for (size_t i = 0; i < b.reqs.size(); i++) b.reqs[i].spp = spp[i];
+        b.tableDirty = true;
+    };
+    auto timeConfig = [&](BatchLaunch c, double& best) -> int {
+        c = resolvedLaunch(b, r, c);
+        if (int rc = uploadBatchTable(b, c.samplesPerUnit)) return rc;
+        HIP_TRY(launchSetU32(b.dFrame.as<uint32_t>(), 0u, s));
+        float msMin = 3.0e38f;
+        for (int rep = 0; rep < 2; rep++) {
+            HIP_TRY(hipEventRecord(b.ev0, s));
+            HIP_TRY(enqueueBatch(b, c, s));
+            HIP_TRY(hipEventRecord(b.ev1, s));
+            HIP_TRY(hipEventSynchronize(b.ev1));
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, b.ev0, b.ev1));
+            msMin = std::min(msMin, ms);
+        }
+        best = msMin;
+        return CRVPT_OK;
+    };
+    // Coordinate descent from the current configuration: compaction interval, paths per unit, block size, then the
+    // number of persistent workgroups as a fraction of the occupancy limit.
+    BatchLaunch cur = b.cfg;
+    cur.numBlocks = 0;
+    double curMs = 0.0;
+    if (int rc = timeConfig(cur, curMs)) {
+        restore();
+        return rc;
+    }
+    auto tryConfig = [&](BatchLaunch c) -> int {
+        double ms = 0.0;
+        if (int rc = timeConfig(c, ms)) return rc;
+        if (ms < curMs) {
+            curMs = ms;
+            cur = c;
+        }
+        return CRVPT_OK;
+    };
+    int rc = CRVPT_OK;
+    for (int steps : {2, 4, 8, 16, 32})
+        if (!rc && steps != cur.stepsPerRefill) rc = tryConfig(BatchLaunch{cur.numBlocks, cur.blockSize, steps,
+                                                                           cur.samplesPerUnit});
+    for (uint32_t g : {1u, 4u, 16u, 64u})
+        if (!rc && g != cur.samplesPerUnit) rc = tryConfig(BatchLaunch{cur.numBlocks, cur.blockSize,
+                                                                       cur.stepsPerRefill, g});
+    for (int bs : {64, 128, 256})
+        if (!rc && bs != cur.blockSize) rc = tryConfig(BatchLaunch{0, bs, cur.stepsPerRefill, cur.samplesPerUnit});
+    if (!rc) {
+        const uint32_t full = resolvedLaunch(b, r, BatchLaunch{0, cur.blockSize, 1, 1}).numBlocks;
+        for (int num : {1, 2})
+            if (!rc) rc = tryConfig(BatchLaunch{std::max(1u, full * uint32_t(num) / 3u), cur.blockSize,
+                                                cur.stepsPerRefill, cur.samplesPerUnit});
+    }
+    restore();
+    if (rc) return rc;
+    b.cfg = cur;
+    b.blocksAuto = cur.numBlocks == 0;
+    if (config) {
+        const BatchLaunch c = resolvedLaunch(b, r, cur);
+        config[0] = int32_t(c.numBlocks);
+        config[1] = c.blockSize;
+        config[2] = c.stepsPerRefill;
+        config[3] = int32_t(c.samplesPerUnit);
+    }
     return CRVPT_OK;
 }
 

@@ -29,6 +29,14 @@ struct LaunchConfig {
     int32_t stepsPerRefill;  // walker events per lane between two refill ballots (the compaction interval)
 };
 
+// Launch shape of the batched queue (the autotuner's parameters).
+struct BatchLaunch {
+    uint32_t numBlocks;       // persistent workgroups
+    int32_t blockSize;        // threads per workgroup: 64, 128, 192 or 256
+    int32_t stepsPerRefill;   // compaction interval: walker steps per lane between two refill ballots
+    uint32_t samplesPerUnit;  // paths per claimed unit (consecutive samples of one pixel on one lane)
+};
+
 // Seeded fBm value-noise cloud: sum over octaves o < octaves of gain^o * noise(p * baseFrequency * lacunarity^o),
 // normalised by the sum of weights, optionally shaped by an anvil height profile, then smoothstep(edge0, edge1, .).
 struct FbmParams {
@@ -46,6 +54,14 @@ struct FbmParams {
 hipError_t launchRender(const SceneParams& P, const DenseGrid* dense, const NanoGrid* nano, const SupervoxelGrid& S,
                         float4* paths, uint32_t* counter, const LaunchConfig& cfg, unsigned long long* stats,
                         float4* feat, hipStream_t stream);
+// Batched queue (batch.hip): one persistent path-pool launch over every request, then one resolve launch that writes
+// each request's image. unitBegins / pixBegins: nreq ascending first units / first pixels (device arrays).
+hipError_t launchBatch(int mode, const BatchRequest* reqs, const uint64_t* unitBegins, const uint64_t* pixBegins,
+                       int nreq, uint32_t totalUnits, uint64_t totalPix, const BatchLaunch& cfg,
+                       const uint32_t* frameOffset, float4* records, uint32_t* counter, unsigned long long* stats,
+                       hipStream_t stream);
+hipError_t launchSetU32(uint32_t* p, uint32_t v, hipStream_t stream);
+int batchOccupancyBlocksPerCU(int mode, int blockSize);
 hipError_t launchAccumulate(const float4* paths, uint32_t npix, uint32_t frames, float4* acc, uint32_t prevFrames,
                             float* outCHW, int channels, hipStream_t stream);
 hipError_t launchWriteCHW(const float4* img, uint32_t npix, float* outCHW, int channels, hipStream_t stream);

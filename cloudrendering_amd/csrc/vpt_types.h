@@ -120,4 +120,22 @@ struct SceneParams {
     uint64_t numUnits;      // width * height * numFrames
 };
 
+// One request of a batched render (the batched queue's device table; no reference counterpart). The scene
+// constants and the volume are snapshots taken when the request was added; the request's paths are the batch units
+// unitBegin + g * (W*H) + pixel for groups g < groups, each unit tracing samples [g * G, min(spp, (g + 1) * G)) of
+// its pixel (G = samples per unit of the launch).
+struct BatchRequest {
+    SceneParams P;          // P.frameBase: global frame number of sample 0; P.sparse selects the volume below
+    SupervoxelGrid S;
+    DenseGrid dense;
+    NanoGrid nano;
+    uint64_t unitBegin;     // first batch unit of the request
+    uint64_t pixBegin;      // first pixel of the request in the batch's flattened pixel range
+    uint32_t spp;           // samples per pixel
+    uint32_t groups;        // units per pixel: ceil(spp / G)
+    float* out;             // [channels, height, width] float32 image on the device
+    int32_t channels;
+    int32_t pad;
+};
+
 }  // namespace crvpt

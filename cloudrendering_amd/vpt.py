@@ -34,7 +34,8 @@ __all__ = [
     "grid_lookup_world", "get_supervoxel_tables", "set_max_bounces", "set_super_voxel_size",
     "set_use_feature_maps", "get_feature_map_from_string", "get_feature_map", "set_feature_map_type",
     "set_view_projection_matrix_as_previous", "VPT_FEATURE_MAP_NAMES", "per_pixel_kernel_forward", "inv_iter_kernel",
-    "VPT_MODE_NAMES", "CrvptError",
+    "VPT_MODE_NAMES", "CrvptError", "batch_clear", "batch_add_request", "batch_render", "batch_set_launch_config",
+    "batch_get_launch_config", "batch_autotune", "batch_get_stats",
 ]
 
 # VPT_MODE_NAMES (RenderSettings.hpp:37-41)
@@ -467,6 +468,72 @@ def set_use_majorant_grid(use_majorant_grid: bool) -> None:
 
 def set_launch_config(supervoxel_size: int = 0, blocks: int = 0, steps_per_refill: int = 4) -> None:
     call("crvpt_set_launch_config", int(supervoxel_size), int(blocks), int(steps_per_refill))
+
+
+# Batched render queue (no reference counterpart). Requests snapshot the current scene as render_frame would render
+# it; batch_render traces every request in one persistent launch (a replayed HIP graph).
+_batch_outputs: list = []
+
+
+def batch_clear() -> None:
+    call("crvpt_batch_clear")
+    _batch_outputs.clear()
+
+
+def batch_add_request(input_tensor, frame_count: int):
+    """Queues frame_count fresh samples per pixel of the current scene; returns (request id, output tensor). The
+    output is a new [C, H, W] float32 tensor on the input tensor's GPU (only its shape, device and dtype are used),
+    written by batch_render."""
+    torch = _torch()
+    if input_tensor.dim() != 3 or int(input_tensor.shape[0]) not in (3, 4):
+        raise CrvptError("batch_add_request: expected a [3 or 4, H, W] tensor")
+    if input_tensor.device.type != "cuda" or input_tensor.dtype != torch.float32:
+        raise CrvptError("batch_add_request: expected a float32 tensor on the GPU")
+    c, h, w = (int(v) for v in input_tensor.shape)
+    out = torch.zeros((c, h, w), dtype=torch.float32, device=input_tensor.device)
+    rid = ctypes.c_int()
+    call("crvpt_batch_add_request", ctypes.c_void_p(out.data_ptr()), c, h, w, int(frame_count), ctypes.byref(rid))
+    _batch_outputs.append(out)
+    return rid.value, out
+
+
+def batch_render(frame_offset: int = 0) -> list:
+    """Renders every queued request on torch's current stream; returns the output tensors in request order."""
+    torch = _torch()
+    if not _batch_outputs:
+        raise CrvptError("batch_render: the batch holds no request")
+    stream = torch.cuda.current_stream(_batch_outputs[0].device).cuda_stream
+    call("crvpt_batch_render", int(frame_offset), ctypes.c_void_p(stream))
+    return list(_batch_outputs)
+
+
+def _batch_config(cfg) -> dict:
+    return {"blocks": cfg[0], "block_size": cfg[1], "steps_per_refill": cfg[2], "samples_per_unit": cfg[3]}
+
+
+def batch_set_launch_config(blocks: int = 0, block_size: int = 256, steps_per_refill: int = 8,
+                            samples_per_unit: int = 16) -> None:
+    call("crvpt_batch_set_launch_config", int(blocks), int(block_size), int(steps_per_refill), int(samples_per_unit))
+
+
+def batch_get_launch_config() -> dict:
+    cfg = (ctypes.c_int * 4)()
+    call("crvpt_batch_get_launch_config", cfg)
+    return _batch_config(cfg)
+
+
+def batch_autotune(spp_divisor: int = 16) -> dict:
+    cfg = (ctypes.c_int * 4)()
+    call("crvpt_batch_autotune", int(spp_divisor), cfg)
+    return _batch_config(cfg)
+
+
+def batch_get_stats() -> dict:
+    ms = ctypes.c_double()
+    n = ctypes.c_uint64()
+    nreq = ctypes.c_int()
+    call("crvpt_batch_get_stats", ctypes.byref(ms), ctypes.byref(n), ctypes.byref(nreq))
+    return {"kernel_ms": ms.value, "num_paths": int(n.value), "num_requests": nreq.value}
 
 
 def get_last_render_stats() -> dict:

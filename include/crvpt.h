@@ -199,6 +199,33 @@ int crvpt_get_last_render_stats(double* path_kernel_ms, double* total_ms, uint64
 int crvpt_set_collect_stats(int collect);
 int crvpt_get_last_render_counters(uint64_t out[4]);
 
+/* Batched render queue. No reference counterpart (the reference renders one frame per render_frame call,
+ * Module.cpp:286-317); for callers that render many scenes at once, e.g. training-pair generation.
+ * crvpt_batch_add_request snapshots the current scene exactly as crvpt_render_frame would render it (volume in its
+ * active representation with the supervoxel tables of the current estimator, camera, medium, phase, lights,
+ * environment, seed offset, bounce limit) and queues spp fresh samples per pixel whose mean goes to out, a device
+ * buffer of [channels, height, width] floats (channels 3 or 4, alpha 1) that must stay valid while the batch is
+ * rendered. Consecutive requests of an unchanged volume share one snapshot. All requests of a batch use the estimator
+ * of the first. Request i draws the random numbers of a single-frame render of the same scene with frames
+ * frame_offset .. frame_offset + spp - 1, so a batch with one sample per unit reproduces those images bit for bit. */
+int crvpt_batch_clear(void);
+int crvpt_batch_add_request(float* out, int channels, int height, int width, int spp, int32_t* request_id);
+/* Renders every request with one persistent launch on stream (a hipStream_t; 0 = the null stream): the counter
+ * reset, the path pool and the image resolve are captured once in a HIP graph and replayed while the batch and its
+ * launch configuration are unchanged. Asynchronous: synchronise the stream before reading the outputs. */
+int crvpt_batch_render(uint32_t frame_offset, void* stream);
+/* Launch configuration {workgroups (0 = occupancy limit x CUs), threads per workgroup (64..256, multiple of 64),
+ * walker steps between refills (compaction interval), samples per claimed unit (paths per lane)}. */
+int crvpt_batch_set_launch_config(int blocks, int block_size, int steps_per_refill, int samples_per_unit);
+int crvpt_batch_get_launch_config(int32_t config[4]);
+/* Times candidate launch configurations on the batch with spp / spp_divisor samples per request (coordinate descent
+ * over the compaction interval, samples per unit, block size and workgroup count), keeps the fastest and returns it
+ * as in crvpt_batch_get_launch_config. The tuning renders overwrite the request outputs. */
+int crvpt_batch_autotune(int spp_divisor, int32_t config[4]);
+/* Device time of the last crvpt_batch_render (events around the graph launch; waits for it), total paths and
+ * requests of the batch. */
+int crvpt_batch_get_stats(double* kernel_ms, uint64_t* num_paths, int32_t* num_requests);
+
 #ifdef __cplusplus
 }
 #endif
