@@ -11,6 +11,10 @@ Workload "c1_dense128": dense 128^3 FP32 fBm (smoothstep(0.45, 0.6)) filling the
 a constant white environment image and one directional sun, g = 0.85, 256 x 256, 64 spp per step, at most 64
 scattering events per path.
 
+Workload "c3_batch": the batched render queue, 64 requests (dense 64^3 / 128^3 / 256^3 and NanoVDB 512^3 / 1024^3
+volumes, image side and spp log-uniform in 128-1024 and 16-1024, random camera orbits and sun directions) traced by
+one persistent launch per step with the autotuned launch configuration; prints its own JSON line.
+
 Prints one JSON line: Mpaths/s over the timed steps, the roofline of the dominant kernel (algorithmic bytes from
 device work counters of an instrumented pass with identical seeds, divided by the HIP-event time of that kernel in
 the timed passes), and the CPU oracle timed on a bounded sample of the same scene on the host cores.
@@ -77,6 +81,109 @@ def setup_c2(vpt):
     vpt.set_launch_config(16, 0, 8)
     vpt.set_super_voxel_size(16)
     vpt.set_max_bounces(C2["max_bounces"])
+
+
+C3 = {"requests": 64, "max_bounces": 128,
+      "volumes": [("dense", 64), ("dense", 128), ("dense", 256), ("sparse", 512), ("sparse", 1024)],
+      "res": (128, 1024), "spp": (16, 1024)}
+
+
+def c3_requests():
+    """The 64 requests of the batched-queue workload, drawn with request index i as the seed: volume kind uniform
+    over dense {64^3, 128^3, 256^3} and sparse {512^3, 1024^3}, image side and spp log-uniform, camera on an orbit
+    around the volume, sun (a directional light) from a random direction above the horizon."""
+    import numpy as np
+
+    reqs = []
+    for i in range(C3["requests"]):
+        rng = np.random.default_rng(i)
+        kind = int(rng.integers(len(C3["volumes"])))
+        side = int(round(math.exp(rng.uniform(math.log(C3["res"][0]), math.log(C3["res"][1])))))
+        spp = int(round(math.exp(rng.uniform(math.log(C3["spp"][0]), math.log(C3["spp"][1])))))
+        az = rng.uniform(0.0, 2.0 * math.pi)
+        el = rng.uniform(-math.pi / 6.0, math.pi / 3.0)
+        cam = [0.8 * math.cos(el) * math.sin(az), 0.8 * math.sin(el), 0.8 * math.cos(el) * math.cos(az)]
+        sun = rng.normal(size=3)
+        sun[1] = abs(sun[1]) + 0.2
+        sun = sun / np.linalg.norm(sun)
+        reqs.append({"index": i, "kind": kind, "side": side, "spp": spp, "camera": cam, "sun": sun.tolist()})
+    return reqs
+
+
+def setup_c3(vpt, torch):
+    """Queues the batched-queue workload: one seeded fBm volume per kind (dense: smoothstep(0.45, 0.6), ~30 %
+    nonzero; sparse: smoothstep(0.645, 0.795), ~12 % active in 8^3 leaves), next-event tracking with the default
+    sky and one directional sun, extinction 1024, albedo 0.9, g 0.5, at most 128 scattering events per path."""
+    reqs = c3_requests()
+    vpt.batch_clear()
+    vpt.set_vpt_mode(5)
+    vpt.set_grid_interpolation_type(1)
+    vpt.set_extinction_base([1.0, 1.0, 1.0])
+    vpt.set_extinction_scale(1024.0)
+    vpt.set_scattering_albedo([0.9, 0.9, 0.9])
+    vpt.set_phase_g(0.5)
+    vpt.set_use_builtin_environment_map("Default")
+    vpt.set_max_bounces(C3["max_bounces"])
+    vpt.set_use_lights(True)
+    vpt.clear_lights()
+    vpt.add_light()
+    vpt.set_light_property(0, "type", "Directional Light")
+    vpt.set_light_property(0, "space", "World")
+    vpt.set_light_property(0, "color", "1 0.961538462 0.884615385")
+    vpt.set_light_property(0, "intensity", "2.6")
+    for kind, (rep, n) in enumerate(C3["volumes"]):
+        mine = [q for q in reqs if q["kind"] == kind]
+        if not mine:
+            continue
+        sparse = rep == "sparse"
+        e0, e1 = (0.645, 0.795) if sparse else (0.45, 0.6)
+        vpt.generate_fbm_cloud(n, n, n, seed=100 + kind, octaves=5, lacunarity=2.0, gain=0.5, base_frequency=4.0,
+                               edge0=e0, edge1=e1)
+        vpt.set_use_sparse_grid(sparse)
+        vpt.set_super_voxel_size(16 if sparse else 8)
+        for q in mine:
+            vpt.set_camera_position(q["camera"])
+            vpt.set_camera_target([0.0, 0.0, 0.0])
+            vpt.set_light_property(0, "position", " ".join(f"{100.0 * v:.6f}" for v in q["sun"]))
+            vpt.set_seed_offset(q["index"])
+            vpt.batch_add_request(torch.zeros((3, q["side"], q["side"]), dtype=torch.float32, device="cuda"),
+                                  q["spp"])
+    return reqs
+
+
+def run_c3(args, vpt, torch):
+    reqs = setup_c3(vpt, torch)
+    t = time.perf_counter()
+    cfg = vpt.batch_autotune(16)
+    tune_s = time.perf_counter() - t
+    print(f"autotune {cfg} in {tune_s:.1f} s", file=sys.stderr, flush=True)
+    for k in range(args.warmup):
+        vpt.batch_render(k * 1024)
+    torch.cuda.synchronize()
+    ms = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        vpt.batch_render((args.warmup + k) * 1024)
+        st = vpt.batch_get_stats()
+        ms.append(st["kernel_ms"])
+        print(f"step {k}: batch {st['kernel_ms']:.1f} ms", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    paths = st["num_paths"]
+    value = paths / (wall / args.steps) / 1e6
+    line = {
+        "metric": "Mpaths/s on 1 MI355X", "value": round(value, 3), "unit": "Mpaths/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded fBm procedural clouds generated on the GPU)",
+        "config": {"workload": "c3_batch", "requests": len(reqs), "paths_per_step": paths,
+                   "volumes": "dense 64^3/128^3/256^3, NanoVDB fp32 512^3/1024^3 (one seeded fBm volume per kind)",
+                   "resolution": "log-uniform side 128-1024", "spp": "log-uniform 16-1024",
+                   "estimator": "next event tracking", "launch": "one persistent launch per step (HIP graph)",
+                   "autotuned": cfg, "autotune_s": round(tune_s, 2), "max_bounces": C3["max_bounces"]},
+        "batch_kernel_ms": [round(v, 3) for v in ms],
+    }
+    print(json.dumps(line), flush=True)
 
 
 def cpu_baseline_c1(budget_s: float = 12.0):
@@ -177,7 +284,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=12)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["c2_sparse512", "c1_dense128"], default="c2_sparse512")
+    ap.add_argument("--workload", choices=["c2_sparse512", "c1_dense128", "c3_batch"], default="c2_sparse512")
     ap.add_argument("--spp", type=int, default=0, help="samples per pixel per step (0: the workload's)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
@@ -196,6 +303,10 @@ def main() -> int:
     from cloudrendering_amd import vpt
 
     vpt.initialize()
+    if args.workload == "c3_batch":
+        run_c3(args, vpt, torch)
+        vpt.cleanup()
+        return 0
     dev = torch.device("cuda:0")
     c2 = args.workload == "c2_sparse512"
     if c2:

@@ -2024,9 +2024,16 @@ int crvpt_batch_autotune(int spp_divisor, int32_t config[4]) {
     for (int steps : {2, 4, 8, 16, 32})
         if (!rc && steps != cur.stepsPerRefill) rc = tryConfig(BatchLaunch{cur.numBlocks, cur.blockSize, steps,
                                                                            cur.samplesPerUnit});
+    // Samples per unit whose full-size batch keeps at most 2^28 unit records (4 GiB).
+    auto fullUnits = [&](uint32_t g) {
+        uint64_t u = 0;
+        for (size_t i = 0; i < b.reqs.size(); i++)
+            u += uint64_t(b.reqs[i].P.width) * uint64_t(b.reqs[i].P.height) * ((spp[i] + g - 1) / g);
+        return u;
+    };
     for (uint32_t g : {1u, 4u, 16u, 64u})
-        if (!rc && g != cur.samplesPerUnit) rc = tryConfig(BatchLaunch{cur.numBlocks, cur.blockSize,
-                                                                       cur.stepsPerRefill, g});
+        if (!rc && g != cur.samplesPerUnit && fullUnits(g) <= (1ull << 28))
+            rc = tryConfig(BatchLaunch{cur.numBlocks, cur.blockSize, cur.stepsPerRefill, g});
     for (int bs : {64, 128, 256})
         if (!rc && bs != cur.blockSize) rc = tryConfig(BatchLaunch{0, bs, cur.stepsPerRefill, cur.samplesPerUnit});
     if (!rc) {
