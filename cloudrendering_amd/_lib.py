@@ -86,6 +86,16 @@ SIGNATURES = {
     "crvpt_get_last_render_stats": (_c_int, [_pd, _pd, _pu64, _pi]),
     "crvpt_set_collect_stats": (_c_int, [_c_int]),
     "crvpt_get_last_render_counters": (_c_int, [_pu64]),
+    "crvpt_kpn_init": (_c_int, [_c_u32]),
+    "crvpt_kpn_set_parameters": (_c_int, [_p, _c_u64]),
+    "crvpt_kpn_get_parameters": (_c_int, [_p, _c_u64]),
+    "crvpt_kpn_compute_gradients": (_c_int, [_p, _p, _p, _c_int, _c_int, _c_int, _p, _p]),
+    "crvpt_kpn_get_gradients": (_c_int, [_p, _c_u64]),
+    "crvpt_kpn_adam_step": (_c_int, [_c_f, _p]),
+    "crvpt_kpn_train_step": (_c_int, [_p, _p, _p, _c_int, _c_int, _c_int, _c_f, _p, _p]),
+    "crvpt_kpn_denoise": (_c_int, [_p, _p, _p, _c_int, _c_int, _c_int, _p]),
+    "crvpt_kpn_get_logits": (_c_int, [_p, _c_int, _c_int, _c_int, _p]),
+    "crvpt_kpn_release": (_c_int, []),
     "crvpt_batch_clear": (_c_int, []),
     "crvpt_batch_add_request": (_c_int, [_p, _c_int, _c_int, _c_int, _c_int, _pi]),
     "crvpt_batch_render": (_c_int, [_c_u32, _p]),

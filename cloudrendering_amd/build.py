@@ -18,8 +18,8 @@ CSRC = os.path.join(PKG_DIR, "csrc")
 BUILD_DIR = os.environ.get("CRVPT_BUILD_DIR", os.path.join(PKG_DIR, "_build"))
 LIB_PATH = os.environ.get("CRVPT_LIBRARY", os.path.join(PKG_DIR, "libcrvpt.so"))
 
-HIP_SOURCES = ["render.hip", "batch.hip", "denoise.hip", "kpn.hip"]
-CXX_SOURCES = ["api.cpp", "nanovdb_build.cpp"]
+HIP_SOURCES = ["render.hip", "batch.hip", "denoise.hip", "kpn.hip", "kpn_net.hip"]
+CXX_SOURCES = ["api.cpp", "nanovdb_build.cpp", "kpn_model.cpp"]
 
 ARCH = os.environ.get("CRVPT_OFFLOAD_ARCH", "gfx950")
 EXTRA_HIP_FLAGS = os.environ.get("CRVPT_EXTRA_HIP_FLAGS", "").split()

@@ -17,12 +17,12 @@
 #include <vector>
 
 #include "../../include/crvpt.h"
+#include "host_common.hpp"
 #include "nanovdb_build.hpp"
 #include "render_api.hpp"
 #include "vpt_types.h"
 
 namespace crvpt {
-namespace {
 
 thread_local std::string g_lastError;
 
@@ -31,11 +31,7 @@ int fail(int code, const std::string& msg) {
     return code;
 }
 
-#define HIP_TRY(expr)                                                                              \
-    do {                                                                                           \
-        hipError_t e_ = (expr);                                                                    \
-        if (e_ != hipSuccess) return fail(CRVPT_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
-    } while (0)
+namespace {
 
 // ------------------------------------------------------------------------------------------------------------
 // Small double-precision matrix helpers (column-major 4x4 like glm).
@@ -112,37 +108,6 @@ Mat4 perspective(double fovy, double aspect, double zn, double zf) {
     P.m[14] = -(zf * zn) / (zf - zn);
     return P;
 }
-
-// ------------------------------------------------------------------------------------------------------------
-// Device buffer
-// ------------------------------------------------------------------------------------------------------------
-struct DevBuf {
-    void* ptr = nullptr;
-    size_t bytes = 0;
-    hipError_t ensure(size_t n) {
-        if (n <= bytes && ptr) return hipSuccess;
-        // A kernel on any stream may still read the old allocation.
-        if (ptr) (void)hipDeviceSynchronize();
-        release();
-        hipError_t e = hipMalloc(&ptr, n);
-        if (e != hipSuccess) {
-            ptr = nullptr;
-            bytes = 0;
-            return e;
-        }
-        bytes = n;
-        return hipSuccess;
-    }
-    void release() {
-        if (ptr) (void)hipFree(ptr);
-        ptr = nullptr;
-        bytes = 0;
-    }
-    template <class T>
-    T* as() const {
-        return reinterpret_cast<T*>(ptr);
-    }
-};
 
 const char* const kModeNames[] = {"Delta Tracking", "Delta Tracking (Spectral)", "Ratio Tracking",
                                   "Decomposition Tracking", "Residual Ratio Tracking", "Next Event Tracking",

@@ -37,6 +37,14 @@ struct BatchLaunch {
     uint32_t samplesPerUnit;  // paths per claimed unit (consecutive samples of one pixel on one lane)
 };
 
+// Kernel-prediction denoiser network (kpn_net.hip): six 3x3 convolutions over 32-channel NHWC bf16 activations.
+constexpr int kKpnChannels = 32;                       // activation channels (inputs and logits zero-padded)
+constexpr int kKpnInputs = 15;                         // input feature channels
+constexpr int kKpnTaps = 25;                           // 5x5 per-pixel filter taps (output logits)
+constexpr int kKpnLayers = 6;
+constexpr int kKpnLayerParams = kKpnChannels * kKpnChannels * 9;  // weights per layer [co][ci][ky * 3 + kx]
+constexpr int kKpnParams = kKpnLayers * kKpnLayerParams + kKpnLayers * kKpnChannels;  // weights, then biases
+
 // Seeded fBm value-noise cloud: sum over octaves o < octaves of gain^o * noise(p * baseFrequency * lacunarity^o),
 // normalised by the sum of weights, optionally shaped by an anvil height profile, then smoothstep(edge0, edge1, .).
 struct FbmParams {
@@ -61,6 +69,19 @@ hipError_t launchBatch(int mode, const BatchRequest* reqs, const uint64_t* unitB
                        const uint32_t* frameOffset, float4* records, uint32_t* counter, unsigned long long* stats,
                        hipStream_t stream);
 hipError_t launchSetU32(uint32_t* p, uint32_t v, hipStream_t stream);
+
+// KPN network kernels (kpn_net.hip). epilogue: 0 bias + ReLU -> bf16, 1 bias -> f32, 2 x (mask > 0) -> bf16.
+hipError_t launchKpnConv(int epilogue, const void* in, const void* wsw, const float* bias, const void* mask, void* out,
+                         int N, int H, int W, int flip, int blocks, hipStream_t stream);
+hipError_t launchKpnWeightGrad(const void* x, const void* dz, float* dW, float* db, int N, int H, int W, int blocks,
+                               hipStream_t stream);
+hipError_t launchKpnPack(const float* features, void* out, int N, int H, int W, hipStream_t stream);
+hipError_t launchKpnHead(const float* logits, const float* noisy, const float* target, float* pred, void* dz,
+                         float* loss, int N, int H, int W, hipStream_t stream);
+hipError_t launchKpnAdam(float* param, const float* grad, float* m1, float* m2, int n, float lr, float beta1,
+                         float beta2, float eps, float bc1, float bc2, hipStream_t stream);
+hipError_t launchKpnSwizzle(const float* w, void* fwd, void* bwd, int layers, hipStream_t stream);
+hipError_t launchKpnZero(float* p, size_t n, hipStream_t stream);
 int batchOccupancyBlocksPerCU(int mode, int blockSize);
 hipError_t launchAccumulate(const float4* paths, uint32_t npix, uint32_t frames, float4* acc, uint32_t prevFrames,
                             float* outCHW, int channels, hipStream_t stream);

@@ -199,6 +199,33 @@ int crvpt_get_last_render_stats(double* path_kernel_ms, double* total_ms, uint64
 int crvpt_set_collect_stats(int collect);
 int crvpt_get_last_render_counters(uint64_t out[4]);
 
+/* Kernel-prediction denoiser network. No reference counterpart in C++ (the reference loads a TorchScript model
+ * trained elsewhere, PyTorchDenoiser.cpp:644, and applies its kernels with modules/kpn_module Kernel.cu:38-83): six 3x3
+ * convolutions (15 -> 32 -> 32 -> 32 -> 32 -> 32 -> 25 channels, ReLU between them) in bf16 with FP32 accumulation on
+ * the matrix cores, whose 25 logits per pixel are turned by a softmax into a 5x5 filter of the noisy radiance.
+ * Parameters: CRVPT_KPN_PARAMS floats, weights [layer][co][ci][ky * 3 + kx] (32 x 32 x 9 per layer, padding channels
+ * zero), then biases [layer][co]. All tensors are device buffers on the caller's stream (a hipStream_t):
+ *   features [n, 15, h, w] f32: radiance mean (3), albedo (3), depth, depth standard deviation, transmittance
+ *   (1 - coverage), screen-space gradient (2), radiance variance (3), density; radiance and variance enter the
+ *   network as log(1 + v);
+ *   noisy, target, out [n, 3, h, w] f32 (linear radiance).
+ * Loss: mean over pixels and channels of (x - r)^2 / (r^2 + 0.01). Optimiser: Adam (0.9, 0.999, 1e-8). */
+#define CRVPT_KPN_PARAMS 55488
+int crvpt_kpn_init(uint32_t seed);
+int crvpt_kpn_set_parameters(const float* params, uint64_t count);
+int crvpt_kpn_get_parameters(float* out, uint64_t count);
+/* Forward and backward pass; gradients replace the previous ones; *loss (host, nullable) gets the batch loss. */
+int crvpt_kpn_compute_gradients(const float* features, const float* noisy, const float* target, int n, int h, int w,
+                                float* loss, void* stream);
+int crvpt_kpn_get_gradients(float* out, uint64_t count);
+int crvpt_kpn_adam_step(float lr, void* stream);
+int crvpt_kpn_train_step(const float* features, const float* noisy, const float* target, int n, int h, int w,
+                         float lr, float* loss, void* stream);
+int crvpt_kpn_denoise(const float* features, const float* noisy, float* out, int n, int h, int w, void* stream);
+/* Logits [n, h, w, 32] f32 of the last forward pass (channels 25..31 are padding). */
+int crvpt_kpn_get_logits(float* out, int n, int h, int w, void* stream);
+int crvpt_kpn_release(void);
+
 /* Batched render queue. No reference counterpart (the reference renders one frame per render_frame call,
  * Module.cpp:286-317); for callers that render many scenes at once, e.g. training-pair generation.
  * crvpt_batch_add_request snapshots the current scene exactly as crvpt_render_frame would render it (volume in its
