@@ -15,6 +15,10 @@ Workload "c3_batch": the batched render queue, 64 requests (dense 64^3 / 128^3 /
 volumes, image side and spp log-uniform in 128-1024 and 16-1024, random camera orbits and sun directions) traced by
 one persistent launch per step with the autotuned launch configuration; prints its own JSON line.
 
+Workload "c4_kpn": the kernel-prediction denoiser: training pairs rendered by the tracer (4 spp + feature maps,
+1024-spp targets in one batched launch), training on the GPU, and the equal-quality wall-time comparison with plain
+tracking against a 16k-spp reference on an unseen cloud; prints its own JSON line.
+
 Prints one JSON line: Mpaths/s over the timed steps, the roofline of the dominant kernel (algorithmic bytes from
 device work counters of an instrumented pass with identical seeds, divided by the HIP-event time of that kernel in
 the timed passes), and the CPU oracle timed on a bounded sample of the same scene on the host cores.
@@ -186,6 +190,144 @@ def run_c3(args, vpt, torch):
     print(json.dumps(line), flush=True)
 
 
+C4 = {"crops": 256, "crop": 128, "clouds": 16, "input_spp": 4, "target_spp": 1024, "batch": 16, "train_steps": 1500,
+      "lr": 1e-3, "eval_res": 1024, "ref_spp": 16384, "plain_spp": [4, 16, 64, 256, 1024, 4096], "max_bounces": 128}
+
+
+def _c4_scene(vpt, seed):
+    """Dense 128^3 seeded fBm cloud (smoothstep(0.45, 0.6)), delta tracking, extinction 1024, albedo 0.9, g 0.5,
+    default sky, at most 128 scattering events."""
+    vpt.generate_fbm_cloud(128, 128, 128, seed=seed, octaves=5, lacunarity=2.0, gain=0.5, base_frequency=4.0,
+                           edge0=0.45, edge1=0.6)
+    vpt.set_use_sparse_grid(False)
+    vpt.set_vpt_mode(0)
+    vpt.set_grid_interpolation_type(1)
+    vpt.set_extinction_base([1.0, 1.0, 1.0])
+    vpt.set_extinction_scale(1024.0)
+    vpt.set_scattering_albedo([0.9, 0.9, 0.9])
+    vpt.set_phase_g(0.5)
+    vpt.set_use_builtin_environment_map("Default")
+    vpt.set_use_lights(False)
+    vpt.set_max_bounces(C4["max_bounces"])
+
+
+def _c4_camera(vpt, rng):
+    az = rng.uniform(0.0, 2.0 * math.pi)
+    el = rng.uniform(-math.pi / 8.0, math.pi / 3.0)
+    d = rng.uniform(0.45, 0.8)
+    vpt.set_camera_position([d * math.cos(el) * math.sin(az), d * math.sin(el), d * math.cos(el) * math.cos(az)])
+    vpt.set_camera_target([0.0, 0.0, 0.0])
+
+
+def run_c4(args, vpt, torch):
+    """Kernel-prediction denoiser: training pairs rendered by the tracer (inputs: 4 spp + feature maps frame by
+    frame; targets: 1024 spp, all crops in one batched launch), training with the hand-written bf16 MFMA network, and
+    the equal-quality comparison on an unseen cloud at 1024^2 against a 16k-spp reference: the wall time of plain
+    tracking at increasing spp and of the denoised 4-spp render, each with its relative MSE."""
+    import numpy as np
+
+    from cloudrendering_amd import kpn
+
+    crops = args.kpn_crops or C4["crops"]
+    cs = C4["crop"]
+    feats = torch.zeros((crops, 15, cs, cs), dtype=torch.float32, device="cuda")
+    noisy = torch.zeros((crops, 3, cs, cs), dtype=torch.float32, device="cuda")
+    t0 = time.perf_counter()
+    vpt.batch_clear()
+    per_cloud = (crops + C4["clouds"] - 1) // C4["clouds"]
+    targets = []
+    for c in range(crops):
+        if c % per_cloud == 0:
+            _c4_scene(vpt, 1 + c // per_cloud)
+        rng = np.random.default_rng(10_000 + c)
+        _c4_camera(vpt, rng)
+        vpt.set_seed_offset(2 * c + 1)
+        targets.append(vpt.batch_add_request(torch.zeros((3, cs, cs), device="cuda"), C4["target_spp"])[1])
+        vpt.set_seed_offset(2 * c)
+        f, x = kpn.render_inputs(vpt, cs, cs, C4["input_spp"])
+        feats[c] = f
+        noisy[c] = x
+    vpt.batch_set_launch_config(0, 256, 8, 64)
+    vpt.batch_render()
+    torch.cuda.synchronize()
+    target = torch.stack(targets)
+    vpt.batch_clear()
+    gen_s = time.perf_counter() - t0
+    print(f"pairs: {crops} crops in {gen_s:.1f} s", file=sys.stderr, flush=True)
+
+    net = kpn.KpnDenoiser(seed=0)
+    steps = args.kpn_steps or C4["train_steps"]
+    g = torch.Generator(device="cpu").manual_seed(0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    losses = []
+    for s_ in range(steps):
+        idx = torch.randint(0, crops, (C4["batch"],), generator=g).cuda()
+        lr = C4["lr"] * (0.1 if s_ > 0.8 * steps else 1.0)
+        losses.append(net.train_step(feats[idx], noisy[idx], target[idx], lr))
+        if s_ % 250 == 0:
+            print(f"train step {s_}: loss {losses[-1]:.5f}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+
+    # evaluation on an unseen cloud and camera
+    res = args.kpn_eval_res or C4["eval_res"]
+    ref_spp = args.kpn_ref_spp or C4["ref_spp"]
+    _c4_scene(vpt, 999)
+    _c4_camera(vpt, np.random.default_rng(999))
+    vpt.set_seed_offset(7777)
+    vpt.batch_clear()
+    _, ref = vpt.batch_add_request(torch.zeros((3, res, res), device="cuda"), ref_spp)
+    vpt.batch_set_launch_config(0, 256, 8, 256)
+    vpt.batch_render()
+    torch.cuda.synchronize()
+    vpt.batch_clear()
+    plain = []
+    probe = torch.zeros((3, res, res), dtype=torch.float32, device="cuda")
+    for spp in C4["plain_spp"]:
+        vpt.set_seed_offset(100 + spp)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        img = vpt.render_frame(probe, spp)
+        torch.cuda.synchronize()
+        plain.append({"spp": spp, "seconds": round(time.perf_counter() - t, 4),
+                      "rel_mse": kpn.rel_mse(img, ref)})
+    vpt.set_seed_offset(55)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    f, x = kpn.render_inputs(vpt, res, res, C4["input_spp"])
+    den = net.denoise(f.unsqueeze(0), x.unsqueeze(0))[0]
+    torch.cuda.synchronize()
+    kpn_s = time.perf_counter() - t
+    kpn_err = kpn.rel_mse(den, ref)
+    noisy_err = kpn.rel_mse(x, ref)
+
+    def time_to(err):
+        """Plain-tracking seconds to reach err, interpolated log-log between the measured spp."""
+        for a, b in zip(plain, plain[1:]):
+            if a["rel_mse"] >= err >= b["rel_mse"]:
+                u = (math.log(a["rel_mse"]) - math.log(err)) / (math.log(a["rel_mse"]) - math.log(b["rel_mse"]))
+                return math.exp(math.log(a["seconds"]) + u * (math.log(b["seconds"]) - math.log(a["seconds"])))
+        return None
+
+    plain_1e3 = time_to(1e-3)
+    equal_quality = time_to(kpn_err)
+    line = {
+        "metric": "wall time to relMSE 1e-3: KPN-denoised vs plain tracking", "unit": "s",
+        "value": round(kpn_s, 4) if kpn_err <= 1e-3 else None, "higher_is_better": False, "n_gpus": 1,
+        "dtype": "bf16 (network, FP32 accumulation) / f32 (tracer)", "data": "synthetic (seeded fBm clouds)",
+        "config": {"workload": "c4_kpn", "crops": crops, "crop": cs, "input_spp": C4["input_spp"],
+                   "target_spp": C4["target_spp"], "train_steps": steps, "batch": C4["batch"],
+                   "eval_res": res, "ref_spp": ref_spp, "estimator": "delta tracking"},
+        "pair_generation_s": round(gen_s, 2), "train_s": round(train_s, 2), "final_train_loss": losses[-1],
+        "kpn": {"input_spp": C4["input_spp"], "seconds": round(kpn_s, 4), "rel_mse": kpn_err,
+                "noisy_rel_mse": noisy_err},
+        "plain": plain, "plain_seconds_to_1e-3": plain_1e3, "plain_seconds_at_kpn_quality": equal_quality,
+        "speedup_at_kpn_quality": (equal_quality / kpn_s) if equal_quality else None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def cpu_baseline_c1(budget_s: float = 12.0):
     """The oracle (FP32 restatement of the reference shaders, hybrid-Taus RNG) on the C1 scene and camera."""
     import numpy as np
@@ -284,7 +426,12 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=12)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["c2_sparse512", "c1_dense128", "c3_batch"], default="c2_sparse512")
+    ap.add_argument("--workload", choices=["c2_sparse512", "c1_dense128", "c3_batch", "c4_kpn"],
+                    default="c2_sparse512")
+    ap.add_argument("--kpn-crops", type=int, default=0)
+    ap.add_argument("--kpn-steps", type=int, default=0)
+    ap.add_argument("--kpn-eval-res", type=int, default=0)
+    ap.add_argument("--kpn-ref-spp", type=int, default=0)
     ap.add_argument("--spp", type=int, default=0, help="samples per pixel per step (0: the workload's)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
@@ -303,8 +450,8 @@ def main() -> int:
     from cloudrendering_amd import vpt
 
     vpt.initialize()
-    if args.workload == "c3_batch":
-        run_c3(args, vpt, torch)
+    if args.workload in ("c3_batch", "c4_kpn"):
+        (run_c3 if args.workload == "c3_batch" else run_c4)(args, vpt, torch)
         vpt.cleanup()
         return 0
     dev = torch.device("cuda:0")
