@@ -38,6 +38,9 @@ SIGNATURES = {
     "crvpt_get_density_field": (_c_int, [_p, _c_u64, _c_int]),
     "crvpt_get_nanovdb_buffer": (_c_int, [_p, _c_u64, _pu64]),
     "crvpt_build_nanovdb_from_dense": (_c_int, [_p, _c_int, _c_int, _c_int, _c_d, _pd, _p, _c_u64, _pu64]),
+    "crvpt_build_nanovdb_from_dense_typed": (_c_int, [_p, _c_int, _c_int, _c_int, _c_d, _pd, _c_int, _p, _c_u64,
+                                                      _pu64]),
+    "crvpt_set_sparse_value_type": (_c_int, [_c_int]),
     "crvpt_set_use_sparse_grid": (_c_int, [_c_int]),
     "crvpt_set_grid_interpolation_type": (_c_int, [_c_int]),
     "crvpt_set_vpt_mode": (_c_int, [_c_i64]),

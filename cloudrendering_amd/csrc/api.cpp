@@ -147,6 +147,7 @@ struct Renderer {
     double znear = 1.0 / 512.0, zfar = 80.0;
     double maxGridExtent = 0.25;
     bool useSparse = false;
+    int sparseValueType = 0;  // leaf type of grids built from the dense field: 0 FP32, 1 Fp16
     int denoiser = 0;  // 0 none, 1 EAW, 2 SVGF
     // feature maps (VolumetricPathTracingPass::setUseFeatureMaps / setFeatureMapType)
     std::vector<int> featureMaps;
@@ -354,7 +355,7 @@ int ensureSparse(Renderer& r) {
     for (size_t i = 0; i < norm.size(); i++) norm[i] = (r.denseHost[i] - mn) / (mx - mn);
     const double dx = double(float(r.boxMaxDense[0]) - float(r.boxMinDense[0])) / double(r.dims[0]);
     const double origin[3] = {float(r.boxMinDense[0]), float(r.boxMinDense[1]), float(r.boxMinDense[2])};
-    std::vector<uint8_t> buf = buildNanoVdbFromDense(norm.data(), r.dims, dx, origin, "tmp");
+    std::vector<uint8_t> buf = buildNanoVdbFromDense(norm.data(), r.dims, dx, origin, "tmp", r.sparseValueType);
     const bool keepDense = r.hasDense;
     int rc = setSparseHost(r, std::move(buf), true);
     r.hasDense = keepDense;
@@ -395,6 +396,7 @@ void gridViews(Renderer& r, DenseGrid& dg, NanoGrid& ng, bool& sparse) {
         ng.indexMin[i] = r.nano.indexBBoxMin[i];
     }
     ng.background = r.nano.background;
+    ng.fp16 = r.nano.valueType == 1 ? 1 : 0;
 }
 
 // Uploads the grid and rebuilds the supervoxel tables the current mode needs.
@@ -1063,14 +1065,40 @@ int crvpt_get_density_field(float* out, uint64_t count, int normalized) {
 
 int crvpt_build_nanovdb_from_dense(const float* values, int nx, int ny, int nz, double dx, const double origin[3],
                                    uint8_t* out, uint64_t capacity, uint64_t* size) {
-    if (!values || nx <= 0 || ny <= 0 || nz <= 0 || !(dx > 0.0) || !origin || !size)
+    return crvpt_build_nanovdb_from_dense_typed(values, nx, ny, nz, dx, origin, 0, out, capacity, size);
+}
+
+int crvpt_build_nanovdb_from_dense_typed(const float* values, int nx, int ny, int nz, double dx,
+                                         const double origin[3], int value_type, uint8_t* out, uint64_t capacity,
+                                         uint64_t* size) {
+    if (!values || nx <= 0 || ny <= 0 || nz <= 0 || !(dx > 0.0) || !origin || !size ||
+        (value_type != 0 && value_type != 1))
         return fail(CRVPT_ERR_INVALID_ARGUMENT, "build_nanovdb_from_dense: invalid arguments");
     const int dims[3] = {nx, ny, nz};
-    std::vector<uint8_t> b = buildNanoVdbFromDense(values, dims, dx, origin, "tmp");
+    std::vector<uint8_t> b = buildNanoVdbFromDense(values, dims, dx, origin, "tmp", value_type);
     *size = b.size();
     if (out) {
         if (capacity < b.size()) return fail(CRVPT_ERR_INVALID_ARGUMENT, "build_nanovdb_from_dense: too small");
         std::memcpy(out, b.data(), b.size());
+    }
+    return CRVPT_OK;
+}
+
+int crvpt_set_sparse_value_type(int value_type) {
+    if (int rc = requireInit()) return rc;
+    if (value_type != 0 && value_type != 1)
+        return fail(CRVPT_ERR_INVALID_ARGUMENT, "set_sparse_value_type: 0 (FP32) or 1 (Fp16)");
+    Renderer& r = *g_r;
+    if (value_type == r.sparseValueType) return CRVPT_OK;
+    r.sparseValueType = value_type;
+    // a grid built from the dense field is rebuilt with the new leaf type when next needed
+    if (r.hasDense && r.hasSparse) {
+        HIP_TRY(hipDeviceSynchronize());
+        r.hasSparse = false;
+        r.nvdbHost.clear();
+        r.sceneDirty = r.tablesDirty = true;
+        r.refTablesValid = false;
+        r.resetAccumulation();
     }
     return CRVPT_OK;
 }

@@ -1,5 +1,6 @@
 // Device read path for NanoVDB 32.3 grids with FP32 values (GridType::Float), the sparse format the reference
-// uploads (VolumetricPathTracingPass.cpp:599-674) and reads through PNanoVDB (VptUtils.glsl:496-580).
+// uploads (VolumetricPathTracingPass.cpp:599-674) and reads through PNanoVDB (VptUtils.glsl:496-580), and with
+// 16-bit quantised leaves (GridType::Fp16: 1120-byte leaves, minimum f32 at 80, quantum f32 at 84, u16 codes at 96).
 //
 // Byte layout (offsets relative to each node, FP32 value type; PNanoVDB.glsl:646-1111 lists the same table):
 //   grid  672 B: map at 296 (mInvMatF 9 floats at +36, mVecF 3 floats at +72)
@@ -24,6 +25,7 @@ struct LeafCache {
     int bi = 0x7FFFFFFF, bj = 0, bk = 0;
     uint32_t leaf = 0;  // 0: the block is a tile of value `tile`
     float tile = 0.0f;
+    float qMin = 0.0f, qScale = 0.0f;  // Fp16 leaves: the leaf's minimum and quantum
     uint32_t misses = 0;  // descents (work counter)
     CRVPT_DEV uint32_t descents() const { return misses; }
 };
@@ -34,6 +36,7 @@ struct NanoAccess {
 
     CRVPT_DEV uint32_t u32(uint32_t off) const { return *reinterpret_cast<const uint32_t*>(g.buf + off); }
     CRVPT_DEV float f32(uint32_t off) const { return *reinterpret_cast<const float*>(g.buf + off); }
+    CRVPT_DEV uint32_t u16(uint32_t off) const { return *reinterpret_cast<const uint16_t*>(g.buf + off); }
     CRVPT_DEV uint64_t u64(uint32_t off) const {
         const uint2 v = *reinterpret_cast<const uint2*>(g.buf + off);
         return uint64_t(v.x) | (uint64_t(v.y) << 32);
@@ -78,6 +81,10 @@ struct NanoAccess {
         const int bi = i >> 3, bj = j >> 3, bk = k >> 3;  // arithmetic shifts: blocks of negative indices too
         if (bi != c.bi || bj != c.bj || bk != c.bk) {
             c.leaf = locateBlock(i, j, k, c.tile);
+            if (g.fp16 && c.leaf != 0u) {
+                c.qMin = f32(c.leaf + 80u);
+                c.qScale = f32(c.leaf + 84u);
+            }
             c.misses++;
             c.bi = bi;
             c.bj = bj;
@@ -85,6 +92,8 @@ struct NanoAccess {
         }
         if (c.leaf == 0u) return c.tile;
         const uint32_t n = (uint32_t(i) & 7u) << 6 | (uint32_t(j) & 7u) << 3 | (uint32_t(k) & 7u);
+        // Fp16: code * quantum + minimum with two roundings (pnanovdb_leaf_fp_read_float, PNanoVDB.glsl:1322-1336)
+        if (g.fp16) return __fadd_rn(__fmul_rn(float(u16(c.leaf + 96u + 2u * n)), c.qScale), c.qMin);
         return f32(c.leaf + 96u + 4u * n);
     }
 

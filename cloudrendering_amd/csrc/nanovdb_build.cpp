@@ -1,8 +1,8 @@
-// NanoVDB 32.3 writer and reader for FP32 fog volumes (see nanovdb_build.hpp).
+// NanoVDB 32.3 writer and reader for FP32 and Fp16 fog volumes (see nanovdb_build.hpp).
 //
 // Serialised layout, breadth first (every node 32-byte aligned, offsets from the node that holds them):
 //   GridData 672 B | TreeData 64 B | RootData 64 B + 32 B per tile | upper nodes 270400 B | lower nodes 33856 B |
-//   leaves 2144 B.
+//   leaves 2144 B (FP32 values) or 1120 B (Fp16: per-leaf minimum and quantum, 16-bit codes).
 // Root tiles are ordered by tile origin (x, then y, then z), children of an internal node by ascending mask bit,
 // which is the order nanovdb::GridBuilder emits them in.
 #include "nanovdb_build.hpp"
@@ -19,8 +19,8 @@ namespace {
 constexpr uint64_t kMagic = 0x304244566f6e614eULL;  // "NanoVDB0"
 constexpr uint32_t kVersion = (32u << 21) | (3u << 10) | 3u;
 constexpr uint32_t kGridSize = 672, kTreeSize = 64, kRootSize = 64, kTileSize = 32;
-constexpr uint32_t kUpperSize = 270400, kLowerSize = 33856, kLeafSize = 2144;
-constexpr uint32_t kGridTypeFloat = 1;
+constexpr uint32_t kUpperSize = 270400, kLowerSize = 33856, kLeafSize = 2144, kLeafSizeFp16 = 1120;
+constexpr uint32_t kGridTypeFloat = 1, kGridTypeFp16 = 15;
 
 template <class T>
 void put(std::vector<uint8_t>& b, size_t off, T v) {
@@ -153,12 +153,15 @@ bool parseNanoVdb(const uint8_t* buf, size_t size, NanoInfo& info, std::string& 
         err = "NanoVDB buffer: unsupported major version (expected 32)";
         return false;
     }
-    // CloudData::computeSparseGridMetadata rejects non-float grids (CloudData.cpp:1292-1296).
-    if (get<uint32_t>(buf, 636) != kGridTypeFloat) {
+    // CloudData::computeSparseGridMetadata rejects non-float grids (CloudData.cpp:1292-1296); this reader also
+    // takes NanoVDB's 16-bit quantised float grids.
+    const uint32_t gridType = get<uint32_t>(buf, 636);
+    if (gridType != kGridTypeFloat && gridType != kGridTypeFp16) {
         err = "Fatal error in CloudData::computeSparseGridMetadata: The grid handle does not store a grid with value "
-              "type float.";
+              "type float (or Fp16).";
         return false;
     }
+    info.valueType = gridType == kGridTypeFp16 ? 1 : 0;
     info.gridSize = get<uint64_t>(buf, 32);
     if (info.gridSize > size) {
         err = "NanoVDB buffer: grid size exceeds the buffer";
@@ -229,7 +232,9 @@ bool extractNanoVdbGridFromFile(const std::vector<uint8_t>& file, std::vector<ui
 }
 
 std::vector<uint8_t> buildNanoVdbFromDense(const float* values, const int dims[3], double dx, const double p0[3],
-                                           const std::string& name) {
+                                           const std::string& name, int valueType) {
+    const bool fp16 = valueType == 1;
+    const uint32_t leafSize = fp16 ? kLeafSizeFp16 : kLeafSize;
     // 1. Leaves: every 8^3 block with at least one non-background voxel (GridBuilder.h:251-300); a fully active
     //    constant leaf becomes an active tile of its lower node.
     std::vector<Leaf> leaves;
@@ -370,10 +375,10 @@ std::vector<uint8_t> buildNanoVdbFromDense(const float* values, const int dims[3
     const uint64_t upperBase = rootOff + kRootSize + kTileSize * tiles;
     const uint64_t lowerBase = upperBase + uint64_t(kUpperSize) * upperOrder.size();
     const uint64_t leafBase = lowerBase + uint64_t(kLowerSize) * lowerOrder.size();
-    const uint64_t total = leafBase + uint64_t(kLeafSize) * leafOrder.size();
+    const uint64_t total = leafBase + uint64_t(leafSize) * leafOrder.size();
     for (size_t i = 0; i < upperOrder.size(); i++) uppers[size_t(upperOrder[i])].offset = upperBase + kUpperSize * i;
     for (size_t i = 0; i < lowerOrder.size(); i++) lowers[size_t(lowerOrder[i])].offset = lowerBase + kLowerSize * i;
-    for (size_t i = 0; i < leafOrder.size(); i++) leaves[size_t(leafOrder[i])].offset = leafBase + kLeafSize * i;
+    for (size_t i = 0; i < leafOrder.size(); i++) leaves[size_t(leafOrder[i])].offset = leafBase + leafSize * i;
 
     std::vector<uint8_t> b(total, 0);
     // 4. Leaves: values, masks and the statistics of GridStats::process(Node0) (GridStats.h:685-708).
@@ -382,11 +387,35 @@ std::vector<uint8_t> buildNanoVdbFromDense(const float* values, const int dims[3
     for (size_t li = 0; li < leaves.size(); li++) {
         const Leaf& L = leaves[li];
         Stats& s = leafStats[li];
+        // Fp16 (GridBuilder.h processLeafs<Fp16>, NanoVDB.h LeafFnBase / LeafData<Fp16>): minimum and quantum over
+        // all 512 values, codes uint16(65535 / (max - min) * (v - min) + 0.5) in double, statistics over the decoded
+        // values code * quantum + minimum (float), quantised back with + 0.5.
+        float minimum = 0.0f, quantum = 0.0f;
+        uint16_t code[512];
+        float value[512];
+        if (fp16) {
+            float mn = 3.402823466e38f, mx = -3.402823466e38f;
+            for (int n = 0; n < 512; n++) {
+                mn = std::min(mn, L.values[n]);
+                mx = std::max(mx, L.values[n]);
+            }
+            minimum = mn;
+            quantum = (mx - mn) / 65535.0f;
+            const double encode = 65535.0 / double(mx - mn);
+            for (int n = 0; n < 512; n++) {
+                const float d = L.values[n] - mn;
+                code[n] = uint16_t(encode * double(d) + 0.5);
+                const float scaled = float(code[n]) * quantum;  // two roundings, as the reference's decode
+                value[n] = scaled + minimum;
+            }
+        } else {
+            std::memcpy(value, L.values, sizeof(value));
+        }
         for (int n = 0; n < 512; n++)
             if ((L.mask[n >> 6] >> (n & 63)) & 1ull) {
                 const int32_t c[3] = {L.origin.x + (n >> 6), L.origin.y + ((n >> 3) & 7), L.origin.z + (n & 7)};
                 s.addBox(c, c);
-                s.addValue(L.values[n]);
+                s.addValue(value[n]);
             }
         voxelCount += s.count;
         const size_t o = size_t(L.offset);
@@ -396,11 +425,26 @@ std::vector<uint8_t> buildNanoVdbFromDense(const float* values, const int dims[3
         std::memcpy(b.data() + o + 12, dif, 3);
         b[o + 15] = 2;  // flags: the leaf holds active values
         std::memcpy(b.data() + o + 16, L.mask, 64);
-        put<float>(b, o + 80, s.mn);
-        put<float>(b, o + 84, s.mx);
-        put<float>(b, o + 88, s.avg());
-        put<float>(b, o + 92, s.stddev());
-        std::memcpy(b.data() + o + 96, L.values, 2048);
+        if (fp16) {
+            auto q = [&](float v) {
+                const float t = (v - minimum) / quantum;
+                return uint16_t(t + 0.5f);
+            };
+            put<float>(b, o + 80, minimum);
+            put<float>(b, o + 84, quantum);
+            put<uint16_t>(b, o + 88, q(s.mn));
+            put<uint16_t>(b, o + 90, q(s.mx));
+            put<uint16_t>(b, o + 92, q(s.avg()));
+            const float dq = s.stddev() / quantum;
+            put<uint16_t>(b, o + 94, uint16_t(dq + 0.5f));
+            std::memcpy(b.data() + o + 96, code, sizeof(code));
+        } else {
+            put<float>(b, o + 80, s.mn);
+            put<float>(b, o + 84, s.mx);
+            put<float>(b, o + 88, s.avg());
+            put<float>(b, o + 92, s.stddev());
+            std::memcpy(b.data() + o + 96, L.values, 2048);
+        }
     }
     // 5. Internal nodes (GridStats.h:628-680): active tiles first (value-mask order), then children.
     uint32_t tileCount[3] = {0, 0, 0};
@@ -597,7 +641,7 @@ std::vector<uint8_t> buildNanoVdbFromDense(const float* values, const int dims[3
         for (int a = 0; a < 3; a++) put<double>(b, 608 + 8 * a, m1[a] - m0[a]);
     }
     put<uint32_t>(b, 632, 0u);  // GridClass::Unknown
-    put<uint32_t>(b, 636, kGridTypeFloat);
+    put<uint32_t>(b, 636, fp16 ? kGridTypeFp16 : kGridTypeFloat);
     put<int64_t>(b, 640, 0);
     put<uint32_t>(b, 648, 0);
     // 9. Partial checksum (GridChecksum.h:135-150): CRC-32 of grid, tree and root bytes after the checksum field in

@@ -24,17 +24,20 @@ struct NanoInfo {
     int32_t indexBBoxMax[3] = {0, 0, 0};
     uint32_t nodeCount[3] = {0, 0, 0};  // leaf, lower, upper
     uint64_t activeVoxelCount = 0;
+    int valueType = 0;  // 0: FP32 leaves (GridType::Float), 1: 16-bit quantised leaves (GridType::Fp16)
 };
 
-// Validates a single-grid FP32 NanoVDB buffer and reads the fields the renderer needs.
+// Validates a single-grid FP32 or Fp16 NanoVDB buffer and reads the fields the renderer needs.
 bool parseNanoVdb(const uint8_t* buf, size_t size, NanoInfo& info, std::string& err);
 
 // Returns the first grid of an uncompressed .nvdb file (io::writeGrid layout: header, metadata, grid bytes).
 bool extractNanoVdbGridFromFile(const std::vector<uint8_t>& file, std::vector<uint8_t>& grid, std::string& err);
 
-// Builds a FogVolume FP32 NanoVDB grid from a dense field (x fastest) with background 0: voxels equal to the
+// Builds a FogVolume NanoVDB grid from a dense field (x fastest) with background 0: voxels equal to the
 // background stay inactive, constant fully-active leaves become lower-node tiles, voxel size dx and origin p0.
+// valueType 0: FP32 leaves (nanovdb::GridBuilder<float>); 1: Fp16 leaves (nanovdb::GridBuilder<float, Fp16>,
+// dithering off).
 std::vector<uint8_t> buildNanoVdbFromDense(const float* values, const int dims[3], double dx, const double p0[3],
-                                           const std::string& name);
+                                           const std::string& name, int valueType = 0);
 
 }  // namespace crvpt

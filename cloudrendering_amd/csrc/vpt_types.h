@@ -58,7 +58,7 @@ struct DenseGrid {
     float borderValue;  // normalised value of the black border texel: (0 - vmin) / (vmax - vmin)
 };
 
-// Sparse grid: a NanoVDB 32.3 buffer with FP32 leaves, resident in HBM, read with 32-bit byte offsets.
+// Sparse grid: a NanoVDB 32.3 buffer with FP32 or Fp16 leaves, resident in HBM, read with 32-bit byte offsets.
 struct NanoGrid {
     const uint8_t* buf;
     uint32_t rootOffset;  // byte offset of the root node
@@ -67,6 +67,7 @@ struct NanoGrid {
     float vec[3];         // Map::mVecF
     float background;
     int32_t indexMin[3];  // index bounding box minimum: texel (0,0,0) of the dense-equivalent grid
+    int32_t fp16;         // 1: Fp16 leaves (codes at +96, minimum at +80, quantum at +84); 0: FP32 leaves
 };
 
 // Supervoxel tables. The majorant table drives the build's local-majorant free-flight sampling; the RRT and

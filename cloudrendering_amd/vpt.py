@@ -35,7 +35,7 @@ __all__ = [
     "set_use_feature_maps", "get_feature_map_from_string", "get_feature_map", "set_feature_map_type",
     "set_view_projection_matrix_as_previous", "VPT_FEATURE_MAP_NAMES", "per_pixel_kernel_forward", "inv_iter_kernel",
     "VPT_MODE_NAMES", "CrvptError", "batch_clear", "batch_add_request", "batch_render", "batch_set_launch_config",
-    "batch_get_launch_config", "batch_autotune", "batch_get_stats",
+    "batch_get_launch_config", "batch_autotune", "batch_get_stats", "set_sparse_value_type",
 ]
 
 # VPT_MODE_NAMES (RenderSettings.hpp:37-41)
@@ -555,18 +555,24 @@ def get_last_render_counters() -> dict:
             "tree_descents": int(v[3])}
 
 
-def build_nanovdb_from_dense(density_zyx, dx: float, origin=(0.0, 0.0, 0.0)) -> bytes:
-    """Host-side dense -> NanoVDB conversion (CloudData::getSparseDensityField semantics); no GPU needed."""
+def build_nanovdb_from_dense(density_zyx, dx: float, origin=(0.0, 0.0, 0.0), value_type: int = 0) -> bytes:
+    """Host-side dense -> NanoVDB conversion (CloudData::getSparseDensityField semantics); no GPU needed.
+    value_type 0: FP32 leaves (the reference's); 1: Fp16 leaves (16-bit codes per leaf)."""
     a = np.ascontiguousarray(np.asarray(density_zyx, dtype=np.float32))
     nz, ny, nx = a.shape
     o = dvec(origin, 3)
     size = ctypes.c_uint64()
-    call("crvpt_build_nanovdb_from_dense", a.ctypes.data_as(ctypes.c_void_p), nx, ny, nz, float(dx), o, None, 0,
-         ctypes.byref(size))
+    call("crvpt_build_nanovdb_from_dense_typed", a.ctypes.data_as(ctypes.c_void_p), nx, ny, nz, float(dx), o,
+         int(value_type), None, 0, ctypes.byref(size))
     out = np.empty(size.value, dtype=np.uint8)
-    call("crvpt_build_nanovdb_from_dense", a.ctypes.data_as(ctypes.c_void_p), nx, ny, nz, float(dx), o,
-         out.ctypes.data_as(ctypes.c_void_p), out.size, ctypes.byref(size))
+    call("crvpt_build_nanovdb_from_dense_typed", a.ctypes.data_as(ctypes.c_void_p), nx, ny, nz, float(dx), o,
+         int(value_type), out.ctypes.data_as(ctypes.c_void_p), out.size, ctypes.byref(size))
     return out.tobytes()
+
+
+def set_sparse_value_type(value_type: int) -> None:
+    """Leaf value type of grids set_use_sparse_grid builds from the dense field: 0 FP32, 1 Fp16 (an extension)."""
+    call("crvpt_set_sparse_value_type", int(value_type))
 
 
 def grid_lookup(ijk) -> np.ndarray:

@@ -61,6 +61,16 @@ int crvpt_generate_fbm_cloud(int nx, int ny, int nz, uint32_t seed, int octaves,
  * voxel size dx, index (0,0,0) at world origin[3], grid name "tmp". With out == NULL only *size is written. */
 int crvpt_build_nanovdb_from_dense(const float* values, int nx, int ny, int nz, double dx, const double origin[3],
                                    uint8_t* out, uint64_t capacity, uint64_t* size);
+/* The same with a leaf value type: 0 FP32 (as above), 1 Fp16 (NanoVDB GridType::Fp16, 16-bit codes per leaf with
+ * the leaf's minimum and quantum, as nanovdb::GridBuilder<float, Fp16> writes them without dithering). No reference
+ * counterpart: the reference renders FP32 grids only. */
+int crvpt_build_nanovdb_from_dense_typed(const float* values, int nx, int ny, int nz, double dx,
+                                         const double origin[3], int value_type, uint8_t* out, uint64_t capacity,
+                                         uint64_t* size);
+/* No reference counterpart: leaf value type of the NanoVDB grid that set_use_sparse_grid builds from a dense field
+ * (0 FP32, the reference's; 1 Fp16: half the leaf memory, values quantised per leaf). Loaded .nvdb files and
+ * crvpt_set_nanovdb_buffer grids may be either type. */
+int crvpt_set_sparse_value_type(int value_type);
 /* Copies the current dense field (normalised or raw) or the current NanoVDB buffer to host memory. */
 int crvpt_get_density_field(float* out, uint64_t count, int normalized);
 int crvpt_get_nanovdb_buffer(uint8_t* out, uint64_t capacity, uint64_t* size);

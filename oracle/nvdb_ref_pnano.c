@@ -51,3 +51,22 @@ void ref_lookup_world(const uint32_t* words, int n, const float* xyz, float* out
         out[i] = pnanovdb_read_float(buf, a);
     }
 }
+
+/* Fp16 grids: the same lookups through pnanovdb_root_fp16_read_float (code * quantum + minimum at leaf level). */
+void ref_lookup_ijk16(const uint32_t* words, int n, const int32_t* ijk, float* out) {
+    pnanovdb_buf_t buf = pnanovdb_make_buf((uint32_t*)words, 0);
+    pnanovdb_grid_handle_t grid = {pnanovdb_address_null()};
+    pnanovdb_root_handle_t root = pnanovdb_tree_get_root(buf, pnanovdb_grid_get_tree(buf, grid));
+    pnanovdb_readaccessor_t acc;
+    pnanovdb_readaccessor_init(&acc, root);
+    for (int i = 0; i < n; i++) {
+        pnanovdb_coord_t c;
+        c.x = ijk[3 * i];
+        c.y = ijk[3 * i + 1];
+        c.z = ijk[3 * i + 2];
+        pnanovdb_uint32_t level = 0u;
+        pnanovdb_address_t a =
+            pnanovdb_readaccessor_get_value_address_and_level(PNANOVDB_GRID_TYPE_FP16, buf, &acc, &c, &level);
+        out[i] = pnanovdb_root_fp16_read_float(buf, a, &c, level);
+    }
+}

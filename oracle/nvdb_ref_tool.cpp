@@ -4,7 +4,9 @@
  * build's NanoVDB writer and device accessor.
  *
  *   nvdb_ref build  <dense.f32> nx ny nz dx ox oy oz <out.grid>  GridBuilder as CloudData.cpp:1383-1404
+ *   nvdb_ref build16 <dense.f32> nx ny nz dx ox oy oz <out.grid> the same with GridBuilder<float, Fp16>
  *   nvdb_ref ijk    <in.grid> <ijk.i32> <out.f32>                PNanoVDB lookups at integer coordinates
+ *   nvdb_ref ijk16  <in.grid> <ijk.i32> <out.f32>                the same on an Fp16 grid
  *   nvdb_ref world  <in.grid> <xyz.f32> <out.f32> <out_ijk.i32>  nearest world-space lookups
  */
 #include <cstdint>
@@ -17,6 +19,7 @@
 #include "nanovdb/util/GridBuilder.h"
 
 extern "C" void ref_lookup_ijk(const uint32_t* words, int n, const int32_t* ijk, float* out);
+extern "C" void ref_lookup_ijk16(const uint32_t* words, int n, const int32_t* ijk, float* out);
 extern "C" void ref_lookup_world(const uint32_t* words, int n, const float* xyz, float* out, int32_t* ijk_out);
 
 template <class T>
@@ -41,18 +44,34 @@ static void writeAll(const char* path, const T* data, size_t n) {
 int main(int argc, char** argv) {
     if (argc < 2) return 2;
     const std::string cmd = argv[1];
-    if (cmd == "build" && argc == 11) {
+    if ((cmd == "build" || cmd == "build16") && argc == 11) {
         std::vector<float> dense = readAll<float>(argv[2]);
         const int nx = std::atoi(argv[3]), ny = std::atoi(argv[4]), nz = std::atoi(argv[5]);
         const double dx = std::atof(argv[6]);
         const nanovdb::Vec3d origin(std::atof(argv[7]), std::atof(argv[8]), std::atof(argv[9]));
-        nanovdb::GridBuilder<float> builder(0.0f);
         auto op = [&](const nanovdb::Coord& ijk) -> float {
             return dense[size_t(ijk.x()) + (size_t(ijk.y()) + size_t(ijk.z()) * ny) * nx];
         };
-        builder(op, nanovdb::CoordBBox(nanovdb::Coord(0), nanovdb::Coord(nx - 1, ny - 1, nz - 1)));
-        auto handle = builder.getHandle<>(dx, origin, "tmp", nanovdb::GridClass::FogVolume);
-        writeAll(argv[10], handle.data(), handle.size());
+        const nanovdb::CoordBBox box(nanovdb::Coord(0), nanovdb::Coord(nx - 1, ny - 1, nz - 1));
+        if (cmd == "build") {
+            nanovdb::GridBuilder<float> builder(0.0f);
+            builder(op, box);
+            auto handle = builder.getHandle<>(dx, origin, "tmp", nanovdb::GridClass::FogVolume);
+            writeAll(argv[10], handle.data(), handle.size());
+        } else {
+            nanovdb::GridBuilder<float, nanovdb::Fp16> builder(0.0f);
+            builder(op, box);
+            auto handle = builder.getHandle<>(dx, origin, "tmp", nanovdb::GridClass::FogVolume);
+            writeAll(argv[10], handle.data(), handle.size());
+        }
+        return 0;
+    }
+    if (cmd == "ijk16" && argc == 5) {
+        std::vector<uint32_t> grid = readAll<uint32_t>(argv[2]);
+        std::vector<int32_t> ijk = readAll<int32_t>(argv[3]);
+        std::vector<float> out(ijk.size() / 3);
+        ref_lookup_ijk16(grid.data(), int(out.size()), ijk.data(), out.data());
+        writeAll(argv[4], out.data(), out.size());
         return 0;
     }
     if (cmd == "ijk" && argc == 5) {

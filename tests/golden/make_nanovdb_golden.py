@@ -1,7 +1,8 @@
 """Writes tests/golden/nanovdb_*.npz: NanoVDB buffers and voxel lookups produced by the reference's own NanoVDB
 32.3.3 GridBuilder and PNanoVDB accessor (oracle/_ref/nvdb_ref, built by `make -C oracle ref` from the reference
 tree). Each fixture holds the dense input, the reference grid bytes, integer-coordinate lookups and nearest
-world-space lookups (with the voxel each one selected).
+world-space lookups (with the voxel each one selected). The nanovdb_fp16_*.npz fixtures hold grids of
+GridBuilder<float, Fp16> (16-bit quantised leaves) and their integer-coordinate lookups.
 
 Run from the repository root: python tests/golden/make_nanovdb_golden.py
 """
@@ -66,6 +67,26 @@ def main():
                             origin=np.array(origin, np.float64), grid=grid, ijk=ijk, values=vals, xyz=xyz,
                             world_values=wvals, world_ijk=wijk)
         print(name, "grid bytes", grid.size, "active lookups", int((vals != 0).sum()))
+    rng = np.random.default_rng(2025)
+    for name, (dense, dx, origin) in case_inputs().items():
+        if name == "block20":
+            continue  # one constant leaf value: pruned to tiles, nothing quantised
+        nz, ny, nx = dense.shape
+        with tempfile.TemporaryDirectory() as tmp:
+            f = lambda n: os.path.join(tmp, n)  # noqa: E731
+            dense.tofile(f("d.f32"))
+            run("build16", f("d.f32"), nx, ny, nz, repr(dx), *[repr(v) for v in origin], f("g.bin"))
+            grid = np.fromfile(f("g.bin"), np.uint8)
+            ijk = np.concatenate([
+                rng.integers(-3, [nx + 3, ny + 3, nz + 3], size=(2000, 3)),
+                np.array([[0, 0, 0], [nx - 1, ny - 1, nz - 1], [-1, 0, 0], [nx, ny, nz], [7, 7, 7], [8, 8, 8]]),
+            ]).astype(np.int32)
+            ijk.tofile(f("ijk.i32"))
+            run("ijk16", f("g.bin"), f("ijk.i32"), f("v.f32"))
+            vals = np.fromfile(f("v.f32"), np.float32)
+        np.savez_compressed(os.path.join(out_dir, f"nanovdb_fp16_{name}.npz"), dense=dense, dx=np.float64(dx),
+                            origin=np.array(origin, np.float64), grid=grid, ijk=ijk, values=vals)
+        print("fp16", name, "grid bytes", grid.size, "active lookups", int((vals != 0).sum()))
 
 
 if __name__ == "__main__":

@@ -11,7 +11,8 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-GOLDEN = sorted(glob.glob(os.path.join(REPO, "tests", "golden", "nanovdb_*.npz")))
+GOLDEN = sorted(p for p in glob.glob(os.path.join(REPO, "tests", "golden", "nanovdb_*.npz")) if "fp16" not in p)
+GOLDEN16 = sorted(glob.glob(os.path.join(REPO, "tests", "golden", "nanovdb_fp16_*.npz")))
 
 
 def _cpu(t):
@@ -28,6 +29,37 @@ def test_device_lookups_bit_exact_vs_reference(vpt, path):
     vals, ijk = vpt.grid_lookup_world(g["xyz"])
     np.testing.assert_array_equal(ijk, g["world_ijk"])
     np.testing.assert_array_equal(vals.view(np.uint32), g["world_values"].view(np.uint32))
+
+
+@pytest.mark.parametrize("path", GOLDEN16, ids=[os.path.basename(p) for p in GOLDEN16])
+def test_device_fp16_lookups_bit_exact_vs_reference(vpt, path):
+    print(f"case: device Fp16 NanoVDB lookups {os.path.basename(path)}", flush=True)
+    g = np.load(path)
+    vpt.set_nanovdb_buffer(g["grid"].tobytes())
+    got = vpt.grid_lookup(g["ijk"])
+    np.testing.assert_array_equal(got.view(np.uint32), g["values"].view(np.uint32))
+
+
+def test_fp16_render_equals_fp32(vpt, oracle):
+    """The Fp16 grid quantises each leaf to 1/65535 of its range: renders agree with the FP32 grid's statistically."""
+    print("case: Fp16 vs FP32 sparse render", flush=True)
+    d = oracle.fbm_cloud(32, 32, 32, seed=17)
+    vpt.set_density_field(d)
+    vpt.set_vpt_mode(0)
+    vpt.set_grid_interpolation_type("stochastic")
+    vpt.set_extinction_scale(200.0)
+    vpt.set_scattering_albedo([0.9, 0.9, 0.9])
+    vpt.set_phase_g(0.5)
+    vpt.set_use_builtin_environment_map("Default")
+    vpt.set_use_lights(False)
+    vpt.set_use_sparse_grid(True)
+    m32, s32 = _stats(vpt, 64, 32)
+    vpt.set_sparse_value_type(1)
+    m16, s16 = _stats(vpt, 64, 32)
+    vpt.set_sparse_value_type(0)
+    vpt.set_use_sparse_grid(False)
+    tol = 3.0 * np.sqrt(s32 ** 2 + s16 ** 2) + 1e-5
+    assert np.all(np.abs(m32 - m16) <= tol), (m32, m16, tol)
 
 
 def test_coherent_lookups_through_the_memo(vpt, oracle):
