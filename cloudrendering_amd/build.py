@@ -39,7 +39,10 @@ def _compile(src: str) -> str:
         cmd = [hipcc(), "-x", "hip", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
                "-packed-fp32-ops", "-munsafe-fp-atomics", *EXTRA_HIP_FLAGS, *common, "-c", os.path.join(CSRC, src), "-o", obj]
     else:
-        cmd = [hipcc(), *common, "-c", os.path.join(CSRC, src), "-o", obj]
+        # host-only units: plain C++ against the HIP runtime headers (no device compilation)
+        rocm = os.path.dirname(os.path.dirname(os.path.realpath(hipcc())))
+        cmd = [hipcc(), "-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I", os.path.join(rocm, "include"), *common, "-c",
+               os.path.join(CSRC, src), "-o", obj]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")

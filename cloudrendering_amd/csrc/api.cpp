@@ -19,6 +19,7 @@
 #include "../../include/crvpt.h"
 #include "host_common.hpp"
 #include "nanovdb_build.hpp"
+#include "parallel_for.hpp"
 #include "render_api.hpp"
 #include "vpt_types.h"
 
@@ -268,9 +269,21 @@ int setDenseHost(Renderer& r, int nx, int ny, int nz, std::vector<float>&& value
     r.denseHost = std::move(values);
     r.denseOnHostValid = true;
     float mn = 3.4e38f, mx = -3.4e38f;
-    for (float v : r.denseHost) {
-        mn = std::min(mn, v);
-        mx = std::max(mx, v);
+    {
+        std::vector<float> pmn(16, 3.4e38f), pmx(16, -3.4e38f);
+        parallelFor(r.denseHost.size(), [&](size_t b, size_t e, size_t t) {
+            float a = 3.4e38f, c = -3.4e38f;
+            for (size_t i = b; i < e; i++) {
+                a = std::min(a, r.denseHost[i]);
+                c = std::max(c, r.denseHost[i]);
+            }
+            pmn[t] = a;
+            pmx[t] = c;
+        }, 1 << 20);
+        for (size_t t = 0; t < 16; t++) {
+            mn = std::min(mn, pmn[t]);
+            mx = std::max(mx, pmx[t]);
+        }
     }
     if (mn == mx) mn = 0.0f;  // DensityField::computeMinMax (DensityField.cpp:149-152)
     r.vmin = mn;
@@ -352,7 +365,9 @@ int ensureSparse(Renderer& r) {
     }
     std::vector<float> norm(r.denseHost.size());
     const float mn = r.vmin, mx = r.vmax;
-    for (size_t i = 0; i < norm.size(); i++) norm[i] = (r.denseHost[i] - mn) / (mx - mn);
+    parallelFor(norm.size(), [&](size_t b, size_t e, size_t) {
+        for (size_t i = b; i < e; i++) norm[i] = (r.denseHost[i] - mn) / (mx - mn);
+    }, 1 << 20);
     const double dx = double(float(r.boxMaxDense[0]) - float(r.boxMinDense[0])) / double(r.dims[0]);
     const double origin[3] = {float(r.boxMinDense[0]), float(r.boxMinDense[1]), float(r.boxMinDense[2])};
     std::vector<uint8_t> buf = buildNanoVdbFromDense(norm.data(), r.dims, dx, origin, "tmp", r.sparseValueType);

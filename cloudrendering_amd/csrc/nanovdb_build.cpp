@@ -7,6 +7,8 @@
 // which is the order nanovdb::GridBuilder emits them in.
 #include "nanovdb_build.hpp"
 
+#include "parallel_for.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -241,38 +243,57 @@ std::vector<uint8_t> buildNanoVdbFromDense(const float* values, const int dims[3
     std::map<Coord, int> leafIndex;
     const int lx = (dims[0] + 7) / 8, ly = (dims[1] + 7) / 8, lz = (dims[2] + 7) / 8;
     std::vector<std::pair<Coord, float>> leafTiles;
-    for (int bz = 0; bz < lz; bz++)
-        for (int by = 0; by < ly; by++)
-            for (int bx = 0; bx < lx; bx++) {
-                Leaf L;
-                L.origin = {bx * 8, by * 8, bz * 8};
-                std::fill(std::begin(L.values), std::end(L.values), 0.0f);
-                std::fill(std::begin(L.mask), std::end(L.mask), 0ull);
-                int active = 0;
-                for (int x = 0; x < 8; x++)
-                    for (int y = 0; y < 8; y++)
-                        for (int z = 0; z < 8; z++) {
-                            const int gx = bx * 8 + x, gy = by * 8 + y, gz = bz * 8 + z;
-                            if (gx >= dims[0] || gy >= dims[1] || gz >= dims[2]) continue;
-                            const float v = values[size_t(gx) + (size_t(gy) + size_t(gz) * dims[1]) * dims[0]];
-                            if (v == 0.0f) continue;
-                            const int n = x << 6 | y << 3 | z;
-                            L.values[n] = v;
-                            L.mask[n >> 6] |= 1ull << (n & 63);
-                            active++;
+    {
+        // blocks scanned in parallel per z slab of blocks, gathered in (bz, by, bx) order
+        std::vector<std::vector<Leaf>> slabLeaves(static_cast<size_t>(lz));
+        std::vector<std::vector<std::pair<Coord, float>>> slabTiles(static_cast<size_t>(lz));
+        parallelFor(size_t(lz), [&](size_t b0, size_t b1, size_t) {
+            for (size_t bzi = b0; bzi < b1; bzi++) {
+                const int bz = int(bzi);
+                for (int by = 0; by < ly; by++)
+                    for (int bx = 0; bx < lx; bx++) {
+                        Leaf L;
+                        L.origin = {bx * 8, by * 8, bz * 8};
+                        std::fill(std::begin(L.values), std::end(L.values), 0.0f);
+                        std::fill(std::begin(L.mask), std::end(L.mask), 0ull);
+                        int active = 0;
+                        for (int x = 0; x < 8; x++)
+                            for (int y = 0; y < 8; y++)
+                                for (int z = 0; z < 8; z++) {
+                                    const int gx = bx * 8 + x, gy = by * 8 + y, gz = bz * 8 + z;
+                                    if (gx >= dims[0] || gy >= dims[1] || gz >= dims[2]) continue;
+                                    const float v = values[size_t(gx) + (size_t(gy) + size_t(gz) * dims[1]) * dims[0]];
+                                    if (v == 0.0f) continue;
+                                    const int n = x << 6 | y << 3 | z;
+                                    L.values[n] = v;
+                                    L.mask[n >> 6] |= 1ull << (n & 63);
+                                    active++;
+                                }
+                        if (!active) continue;
+                        if (active == 512) {
+                            bool constant = true;
+                            for (int n = 1; n < 512 && constant; n++) constant = L.values[n] == L.values[0];
+                            if (constant) {
+                                slabTiles[bzi].push_back({L.origin, L.values[0]});
+                                continue;
+                            }
                         }
-                if (!active) continue;
-                if (active == 512) {
-                    bool constant = true;
-                    for (int n = 1; n < 512 && constant; n++) constant = L.values[n] == L.values[0];
-                    if (constant) {
-                        leafTiles.push_back({L.origin, L.values[0]});
-                        continue;
+                        slabLeaves[bzi].push_back(L);
                     }
-                }
+            }
+        }, 1);
+        size_t total = 0;
+        for (auto& v : slabLeaves) total += v.size();
+        leaves.reserve(total);
+        for (size_t z = 0; z < slabLeaves.size(); z++) {
+            for (Leaf& L : slabLeaves[z]) {
                 leafIndex[L.origin] = int(leaves.size());
                 leaves.push_back(L);
             }
+            std::vector<Leaf>().swap(slabLeaves[z]);
+            for (auto& t : slabTiles[z]) leafTiles.push_back(t);
+        }
+    }
     // 2. Internal nodes, then the reference's pruning (GridBuilder.h:303-343): a lower node holding only tiles of
     //    one value becomes an active upper tile, an upper node holding only tiles of one value a root tile.
     std::map<Coord, int> lowerIndex, upperIndex;
@@ -384,7 +405,9 @@ std::vector<uint8_t> buildNanoVdbFromDense(const float* values, const int dims[3
     // 4. Leaves: values, masks and the statistics of GridStats::process(Node0) (GridStats.h:685-708).
     std::vector<Stats> leafStats(leaves.size());
     uint64_t voxelCount = 0;
-    for (size_t li = 0; li < leaves.size(); li++) {
+    std::vector<uint64_t> leafVoxels(leaves.size(), 0);
+    parallelFor(leaves.size(), [&](size_t l0, size_t l1, size_t) {
+    for (size_t li = l0; li < l1; li++) {
         const Leaf& L = leaves[li];
         Stats& s = leafStats[li];
         // Fp16 (GridBuilder.h processLeafs<Fp16>, NanoVDB.h LeafFnBase / LeafData<Fp16>): minimum and quantum over
@@ -417,7 +440,7 @@ std::vector<uint8_t> buildNanoVdbFromDense(const float* values, const int dims[3
                 s.addBox(c, c);
                 s.addValue(value[n]);
             }
-        voxelCount += s.count;
+        leafVoxels[li] = s.count;
         const size_t o = size_t(L.offset);
         putCoord(b, o, s.bmin);
         const uint8_t dif[3] = {uint8_t(s.bmax[0] - s.bmin[0]), uint8_t(s.bmax[1] - s.bmin[1]),
@@ -446,10 +469,14 @@ std::vector<uint8_t> buildNanoVdbFromDense(const float* values, const int dims[3
             std::memcpy(b.data() + o + 96, L.values, 2048);
         }
     }
+    }, 256);
+    for (uint64_t v : leafVoxels) voxelCount += v;
     // 5. Internal nodes (GridStats.h:628-680): active tiles first (value-mask order), then children.
     uint32_t tileCount[3] = {0, 0, 0};
     std::vector<Stats> lowerStats(lowers.size());
-    for (size_t ni = 0; ni < lowers.size(); ni++) {
+    std::vector<uint64_t> lowerTiles(lowers.size(), 0);
+    parallelFor(lowers.size(), [&](size_t n0, size_t n1, size_t) {
+    for (size_t ni = n0; ni < n1; ni++) {
         const Lower& n = lowers[ni];
         if (n.offset == 0) continue;  // pruned into an upper tile
         Stats& s = lowerStats[ni];
@@ -461,8 +488,7 @@ std::vector<uint8_t> buildNanoVdbFromDense(const float* values, const int dims[3
                 const int32_t hi[3] = {lo[0] + 7, lo[1] + 7, lo[2] + 7};
                 s.addValue(n.tileValue[i], 512);
                 s.addBox(lo, hi);
-                tileCount[0]++;
-                voxelCount += 512;
+                lowerTiles[ni]++;
             }
         }
         for (int i = 0; i < 4096; i++) {
@@ -484,6 +510,11 @@ std::vector<uint8_t> buildNanoVdbFromDense(const float* values, const int dims[3
             put<float>(b, o + 1064, s.avg());
             put<float>(b, o + 1068, s.stddev());
         }
+    }
+    }, 1);
+    for (uint64_t t : lowerTiles) {
+        tileCount[0] += uint32_t(t);
+        voxelCount += 512ull * t;
     }
     std::vector<Stats> upperStats(uppers.size());
     for (size_t ni = 0; ni < uppers.size(); ni++) {
