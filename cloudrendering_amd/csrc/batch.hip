@@ -10,7 +10,8 @@
 // units; a lane reads its request's scene constants and volume through the device request table.
 //
 // Every path draws the same counter-based random numbers as in a single-frame render of the same request (pixel,
-// frame number, seed offset), so a batch with G = 1 reproduces the single-frame images bit for bit; larger G only
+// frame number, seed offset) and runs the same device code, so a batch with G = 1 reproduces the single-frame images
+// up to the last bits (the compiler forms fused multiply-adds differently in the two kernels); larger G also
 // regroups the floating-point sum over samples.
 #include <hip/hip_runtime.h>
 

@@ -30,6 +30,12 @@ struct LeafCache {
     CRVPT_DEV uint32_t descents() const { return misses; }
 };
 
+// a * b + c rounded after the product and after the sum (never fused into one FMA).
+CRVPT_DEV float mulThenAdd(float a, float b, float c) {
+#pragma clang fp contract(off)
+    return a * b + c;
+}
+
 struct NanoAccess {
     NanoGrid g;
     using Cache = LeafCache;
@@ -93,7 +99,7 @@ struct NanoAccess {
         if (c.leaf == 0u) return c.tile;
         const uint32_t n = (uint32_t(i) & 7u) << 6 | (uint32_t(j) & 7u) << 3 | (uint32_t(k) & 7u);
         // Fp16: code * quantum + minimum with two roundings (pnanovdb_leaf_fp_read_float, PNanoVDB.glsl:1322-1336)
-        if (g.fp16) return __fadd_rn(__fmul_rn(float(u16(c.leaf + 96u + 2u * n)), c.qScale), c.qMin);
+        if (g.fp16) return mulThenAdd(float(u16(c.leaf + 96u + 2u * n)), c.qScale, c.qMin);
         return f32(c.leaf + 96u + 4u * n);
     }
 

@@ -244,7 +244,8 @@ int crvpt_kpn_release(void);
  * buffer of [channels, height, width] floats (channels 3 or 4, alpha 1) that must stay valid while the batch is
  * rendered. Consecutive requests of an unchanged volume share one snapshot. All requests of a batch use the estimator
  * of the first. Request i draws the random numbers of a single-frame render of the same scene with frames
- * frame_offset .. frame_offset + spp - 1, so a batch with one sample per unit reproduces those images bit for bit. */
+ * frame_offset .. frame_offset + spp - 1, so a batch with one sample per unit reproduces those images up to the
+ * last bits of floating-point rounding. */
 int crvpt_batch_clear(void);
 int crvpt_batch_add_request(float* out, int channels, int height, int width, int spp, int32_t* request_id);
 /* Renders every request with one persistent launch on stream (a hipStream_t; 0 = the null stream): the counter
