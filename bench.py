@@ -82,7 +82,7 @@ def setup_c2(vpt):
     vpt.set_use_builtin_environment_map("Default")
     vpt.set_use_lights(False)
     vpt.set_use_sparse_grid(True)
-    vpt.set_launch_config(16, 0, 8)
+    vpt.set_launch_config(16, 0, 16)
     vpt.set_super_voxel_size(16)
     vpt.set_max_bounces(C2["max_bounces"])
 
@@ -247,7 +247,7 @@ def run_c4(args, vpt, torch):
         f, x = kpn.render_inputs(vpt, cs, cs, C4["input_spp"])
         feats[c] = f
         noisy[c] = x
-    vpt.batch_set_launch_config(0, 256, 8, 64)
+    vpt.batch_set_launch_config(0, 256, 16, 64)
     vpt.batch_render()
     torch.cuda.synchronize()
     target = torch.stack(targets)
@@ -278,7 +278,7 @@ def run_c4(args, vpt, torch):
     vpt.set_seed_offset(7777)
     vpt.batch_clear()
     _, ref = vpt.batch_add_request(torch.zeros((3, res, res), device="cuda"), ref_spp)
-    vpt.batch_set_launch_config(0, 256, 8, 256)
+    vpt.batch_set_launch_config(0, 256, 16, 256)
     vpt.batch_render()
     torch.cuda.synchronize()
     vpt.batch_clear()

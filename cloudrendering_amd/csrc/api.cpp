@@ -164,7 +164,7 @@ struct Renderer {
     bool useMajorantGrid = true;
     int svSizeSetting = 0;
     int blocksSetting = 0;
-    int stepsPerRefill = 4;
+    int stepsPerRefill = 16;
     bool collectStats = false;
 
     // ---- volume ----
@@ -1622,7 +1622,7 @@ int crvpt_set_launch_config(int supervoxel_size, int blocks, int steps_per_refil
         r.tablesDirty = true;
     }
     r.blocksSetting = blocks;
-    r.stepsPerRefill = steps_per_refill > 0 ? steps_per_refill : 4;
+    r.stepsPerRefill = steps_per_refill > 0 ? steps_per_refill : 16;
     return CRVPT_OK;
 }
 
@@ -1673,7 +1673,7 @@ struct Batch {
     int mode = -1;
     VolumeKey lastVolume;
     uint64_t lastEnvVersion = ~0ull;
-    BatchLaunch cfg{0, 256, 8, 16};
+    BatchLaunch cfg{0, 256, 16, 16};
     bool blocksAuto = true;  // numBlocks follows the occupancy of the block size
     DevBuf dReqs, dUnitBegins, dPixBegins, dRecords, dCounter, dFrame;
     uint32_t totalUnits = 0;

@@ -15,7 +15,7 @@ namespace crvpt {
 constexpr float kInf = 3.0e38f;
 
 #ifndef CRVPT_PATHPOOL_WAVES
-#define CRVPT_PATHPOOL_WAVES 3  // waves per SIMD the path pool is compiled for (caps VGPRs at 168)
+#define CRVPT_PATHPOOL_WAVES 4  // waves per SIMD the path pool is compiled for (caps registers at 128)
 #endif
 
 // ------------------------------------------------------------------------------------------------------------
@@ -36,7 +36,7 @@ constexpr float kInf = 3.0e38f;
 // cells before it returns, so empty space costs cell reads but no random-number blocks.
 // ------------------------------------------------------------------------------------------------------------
 #ifndef CRVPT_MAX_CROSSINGS
-#define CRVPT_MAX_CROSSINGS 8
+#define CRVPT_MAX_CROSSINGS 2
 #endif
 constexpr int kMaxCrossings = CRVPT_MAX_CROSSINGS;
 

@@ -466,7 +466,7 @@ def set_use_majorant_grid(use_majorant_grid: bool) -> None:
     call("crvpt_set_use_majorant_grid", int(bool(use_majorant_grid)))
 
 
-def set_launch_config(supervoxel_size: int = 0, blocks: int = 0, steps_per_refill: int = 4) -> None:
+def set_launch_config(supervoxel_size: int = 0, blocks: int = 0, steps_per_refill: int = 16) -> None:
     call("crvpt_set_launch_config", int(supervoxel_size), int(blocks), int(steps_per_refill))
 
 
@@ -511,7 +511,7 @@ def _batch_config(cfg) -> dict:
     return {"blocks": cfg[0], "block_size": cfg[1], "steps_per_refill": cfg[2], "samples_per_unit": cfg[3]}
 
 
-def batch_set_launch_config(blocks: int = 0, block_size: int = 256, steps_per_refill: int = 8,
+def batch_set_launch_config(blocks: int = 0, block_size: int = 256, steps_per_refill: int = 16,
                             samples_per_unit: int = 16) -> None:
     call("crvpt_batch_set_launch_config", int(blocks), int(block_size), int(steps_per_refill), int(samples_per_unit))
 
