@@ -60,7 +60,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CRVPT_PATHP
     uint32_t left = 0, sample = 0;
     V3 sum = v3(0.0f);
     bool exhausted = false;
-    uint32_t paths = 0, descents = 0;
+    uint32_t paths = 0, descents = 0, lowerLookups = 0;
     Walker W;
     W.svReads = 0;
     LeafCache gc;
@@ -88,6 +88,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CRVPT_PATHP
                         if (r != req) {
                             // the memo holds byte offsets into the previous request's grid
                             descents += gc.descents();
+                            lowerLookups += gc.lowerLookups();
                             gc = LeafCache();
                             req = r;
                         }
@@ -128,7 +129,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CRVPT_PATHP
             }
         }
     }
-    if (stats) flushCounters(stats, st.taps, W.svReads, paths, descents + gc.descents());
+    if (stats) flushCounters(stats, st.taps, W.svReads, paths, descents + gc.descents(),
+                             lowerLookups + gc.lowerLookups());
 }
 
 // Per pixel of every request: the sum of its units' records in group order, divided by spp (the single-frame

@@ -659,23 +659,17 @@ CRVPT_DEV bool resolvePending(PathState& st, Walker& W, const SceneParams& P, co
 }
 
 // Wave-reduced 64-bit work counters: [0] density taps, [1] supervoxel majorant / table reads, [2] paths started,
-// [3] sparse-tree descents (lookup memo misses).
+// [3] sparse-tree descents from the root (lookup memo misses), [4] block changes resolved in the memoised lower node.
 CRVPT_DEV void flushCounters(unsigned long long* stats, uint32_t taps, uint32_t svs, uint32_t paths,
-                             uint32_t descents) {
-    unsigned long long a = taps, b = svs, c = paths, d = descents;
+                             uint32_t descents, uint32_t lowerLookups) {
+    unsigned long long v[5] = {taps, svs, paths, descents, lowerLookups};
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        a += __shfl_down(a, off);
-        b += __shfl_down(b, off);
-        c += __shfl_down(c, off);
-        d += __shfl_down(d, off);
-    }
-    if (__lane_id() == 0) {
-        atomicAdd(&stats[0], a);
-        atomicAdd(&stats[1], b);
-        atomicAdd(&stats[2], c);
-        atomicAdd(&stats[3], d);
-    }
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int q = 0; q < 5; q++) v[q] += __shfl_down(v[q], off);
+    if (__lane_id() == 0)
+#pragma unroll
+        for (int q = 0; q < 5; q++) atomicAdd(&stats[q], v[q]);
 }
 
 #ifndef CRVPT_RESOLVE_LANES

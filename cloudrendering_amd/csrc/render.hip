@@ -1,8 +1,9 @@
 // gfx950 kernels of the volumetric path tracer.
 //
-// Work decomposition: a "unit" is one camera path (pixel p, sample s); unit = s * (W*H) + p, so a wave's 64
-// consecutive units are 64 neighbouring pixels of one sample (coherent primary rays). A persistent grid of waves
-// drains a global unit counter: whenever lanes of a wave have no live path, one lane claims as many units as the
+// Work decomposition: a "unit" is one camera path (pixel p, sample s) with record slot s * (W*H) + p. Claims walk
+// the image in 16 x 16 screen tiles, all of a launch's samples of one tile before the next, so a wave's 64
+// consecutive claims are 64 neighbouring pixels of one sample (coherent primary rays) and the paths in flight read a
+// small region of the volume. A persistent grid of waves drains a global claim counter: whenever lanes of a wave have no live path, one lane claims as many units as the
 // wave has free lanes (ballot + popcount + one atomic) and the free lanes take them in lane order (mbcnt), so
 // lanes refill at any step instead of idling until the longest path of the wave ends. Each path writes its
 // radiance to its own slot; a separate kernel sums the slots of a pixel in sample order, so images are
@@ -21,6 +22,39 @@
 
 namespace crvpt {
 
+#ifndef CRVPT_CLAIM_TILE
+#define CRVPT_CLAIM_TILE 16
+#endif
+// Claims walk the image in square screen tiles of this edge, all of a launch's samples of one tile before the next
+// (0: sample-major over the whole image). The paths in flight then cover a few tiles instead of a quarter of the
+// image, so the volume region they read is small enough to stay in the L2 / Infinity Cache.
+constexpr uint32_t kClaimTile = CRVPT_CLAIM_TILE;
+
+CRVPT_DEV uint32_t claimCount(const SceneParams& P) {
+    if (kClaimTile == 0u) return uint32_t(P.numUnits);
+    const uint32_t tx = (uint32_t(P.width) + kClaimTile - 1u) / kClaimTile;
+    const uint32_t ty = (uint32_t(P.height) + kClaimTile - 1u) / kClaimTile;
+    return tx * ty * kClaimTile * kClaimTile * P.numFrames;
+}
+
+// Claim index -> launch slot sample * W*H + pixel (the record index); false for padding claims outside the image.
+CRVPT_DEV bool claimToSlot(const SceneParams& P, uint32_t c, uint32_t& slot) {
+    if (kClaimTile == 0u) {
+        slot = c;
+        return true;
+    }
+    constexpr uint32_t tp = kClaimTile * kClaimTile;
+    const uint32_t w = uint32_t(P.width), h = uint32_t(P.height);
+    const uint32_t tx = (w + kClaimTile - 1u) / kClaimTile;
+    const uint32_t perTile = tp * P.numFrames;
+    const uint32_t tile = c / perTile, rem = c - tile * perTile;
+    const uint32_t s = rem / tp, q = rem - s * tp;
+    const uint32_t x = (tile % tx) * kClaimTile + q % kClaimTile, y = (tile / tx) * kClaimTile + q / kClaimTile;
+    if (x >= w || y >= h) return false;
+    slot = s * (w * h) + y * w + x;
+    return true;
+}
+
 template <int MODE, class Grid>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CRVPT_PATHPOOL_WAVES))) void k_pathPool(
         SceneParams P, Grid grid, SupervoxelGrid S, float4* __restrict__ out, uint32_t* __restrict__ counter,
@@ -35,6 +69,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CRVPT_PATHP
     Walker W;
     W.svReads = 0;
     typename Grid::Cache gc;
+    const uint32_t claims = claimCount(P);
     while (true) {
         // Refill until every lane holds a live path or the queue is drained: a claimed path whose camera ray misses
         // the volume finishes inside startPath, and its lane claims again before the wave steps.
@@ -47,10 +82,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CRVPT_PATHP
             if (lane == leader) base = atomicAdd(counter, uint32_t(__popcll(needMask)));
             base = __shfl(base, leader);
             if (needs) {
-                st.unit = base + uint32_t(__popcll(needMask & ((1ull << lane) - 1ull)));
-                if (uint64_t(st.unit) >= P.numUnits) {
+                const uint32_t c = base + uint32_t(__popcll(needMask & ((1ull << lane) - 1ull)));
+                if (c >= claims) {
                     exhausted = true;
-                } else {
+                } else if (claimToSlot(P, c, st.unit)) {
                     paths++;
                     if (!startPath<MODE>(st, W, P, S, feat)) writePath(out, st);
                 }
@@ -75,7 +110,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CRVPT_PATHP
             }
         }
     }
-    if (stats) flushCounters(stats, st.taps, W.svReads, paths, gc.descents());
+    if (stats) flushCounters(stats, st.taps, W.svReads, paths, gc.descents(), gc.lowerLookups());
 }
 
 // ------------------------------------------------------------------------------------------------------------
