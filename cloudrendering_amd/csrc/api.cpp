@@ -1810,7 +1810,7 @@ BatchLaunch resolvedLaunch(const Batch& b, const Renderer& r, BatchLaunch c) {
 hipError_t enqueueBatch(Batch& b, const BatchLaunch& c, hipStream_t s) {
     return launchBatch(b.mode, b.dReqs.as<BatchRequest>(), b.dUnitBegins.as<uint64_t>(), b.dPixBegins.as<uint64_t>(),
                        int(b.reqs.size()), b.totalUnits, b.totalPix, c, b.dFrame.as<uint32_t>(),
-                       b.dRecords.as<float4>(), b.dCounter.as<uint32_t>(), nullptr, s);
+                       b.dRecords.as<float4>(), b.dCounter.as<uint32_t>(), s);
 }
 
 bool validBatchLaunch(const BatchLaunch& c) {

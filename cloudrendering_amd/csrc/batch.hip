@@ -48,7 +48,7 @@ template <int MODE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CRVPT_PATHPOOL_WAVES))) void k_batchPool(
         const BatchRequest* __restrict__ reqs, const uint64_t* __restrict__ unitBegins, int nreq, uint32_t totalUnits,
         uint32_t samplesPerUnit, const uint32_t* __restrict__ frameOffset, float4* __restrict__ records,
-        uint32_t* __restrict__ counter, int stepsPerRefill, unsigned long long* __restrict__ stats) {
+        uint32_t* __restrict__ counter, int stepsPerRefill) {
     const int lane = __lane_id();
     const uint32_t frameOff = *frameOffset;
     PathState st;
@@ -60,7 +60,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CRVPT_PATHP
     uint32_t left = 0, sample = 0;
     V3 sum = v3(0.0f);
     bool exhausted = false;
-    uint32_t paths = 0, descents = 0, lowerLookups = 0;
     Walker W;
     W.svReads = 0;
     LeafCache gc;
@@ -86,10 +85,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CRVPT_PATHP
                     } else {
                         const int r = findRequest(unitBegins, nreq, u);
                         if (r != req) {
-                            // the memo holds byte offsets into the previous request's grid
-                            descents += gc.descents();
-                            lowerLookups += gc.lowerLookups();
-                            gc = LeafCache();
+                            gc = LeafCache();  // the memo holds byte offsets into the previous request's grid
                             req = r;
                         }
                         const BatchRequest& R = reqs[req];
@@ -109,7 +105,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CRVPT_PATHP
                 st.frame = R.P.frameBase + frameOff + sample;
                 sample++;
                 left--;
-                paths++;
                 if (!startPathAt<MODE>(st, W, R.P, R.S, nullptr)) finishSample();
             }
             if (__ballot(!(st.flags & PF_ALIVE) && (left > 0u || !exhausted)) == 0ull) break;
@@ -129,8 +124,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CRVPT_PATHP
             }
         }
     }
-    if (stats) flushCounters(stats, st.taps, W.svReads, paths, descents + gc.descents(),
-                             lowerLookups + gc.lowerLookups());
 }
 
 // Per pixel of every request: the sum of its units' records in group order, divided by spp (the single-frame
@@ -193,15 +186,14 @@ hipError_t launchSetU32(uint32_t* p, uint32_t v, hipStream_t stream) {
 
 hipError_t launchBatch(int mode, const BatchRequest* reqs, const uint64_t* unitBegins, const uint64_t* pixBegins,
                        int nreq, uint32_t totalUnits, uint64_t totalPix, const BatchLaunch& cfg,
-                       const uint32_t* frameOffset, float4* records, uint32_t* counter, unsigned long long* stats,
-                       hipStream_t stream) {
+                       const uint32_t* frameOffset, float4* records, uint32_t* counter, hipStream_t stream) {
     if (cfg.blockSize <= 0 || cfg.blockSize > 256 || cfg.blockSize % 64 != 0 || cfg.numBlocks == 0 ||
         cfg.samplesPerUnit == 0 || cfg.stepsPerRefill <= 0)
         return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_setU32, dim3(1), dim3(64), 0, stream, counter, 0u);
     void* args[] = {(void*)&reqs,    (void*)&unitBegins, (void*)&nreq,    (void*)&totalUnits,
                     (void*)&cfg.samplesPerUnit, (void*)&frameOffset, (void*)&records, (void*)&counter,
-                    (void*)&cfg.stepsPerRefill, (void*)&stats};
+                    (void*)&cfg.stepsPerRefill};
     hipError_t e = hipLaunchKernel(batchPoolKernel(mode), dim3(cfg.numBlocks), dim3(cfg.blockSize), args, 0, stream);
     if (e != hipSuccess) return e;
     const uint64_t blocks = (totalPix + 255) / 256;

@@ -15,6 +15,10 @@
 
 #include "device_common.hpp"
 
+#ifndef CRVPT_LOWER_MEMO
+#define CRVPT_LOWER_MEMO 1
+#endif
+
 namespace crvpt {
 
 // Per-lane memo of the last 8^3 block a lookup descended to: the block coordinate (index >> 3), and either the
@@ -103,7 +107,7 @@ struct NanoAccess {
     CRVPT_DEV float value(int i, int j, int k, LeafCache& c) const {
         const int bi = i >> 3, bj = j >> 3, bk = k >> 3;  // arithmetic shifts: blocks of negative indices too
         if (bi != c.bi || bj != c.bj || bk != c.bk) {
-            if ((((bi ^ c.bi) | (bj ^ c.bj) | (bk ^ c.bk)) >> 4) == 0) {
+            if (CRVPT_LOWER_MEMO && (((bi ^ c.bi) | (bj ^ c.bj) | (bk ^ c.bk)) >> 4) == 0) {
                 // same 128^3 region as the memoised block: its lower node, or the region's tile
                 if (c.lower != 0u) c.leaf = leafInLower(c.lower, i, j, k, c.tile);
                 else c.leaf = 0u;

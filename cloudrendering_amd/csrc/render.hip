@@ -55,7 +55,9 @@ CRVPT_DEV bool claimToSlot(const SceneParams& P, uint32_t c, uint32_t& slot) {
     return true;
 }
 
-template <int MODE, class Grid>
+// STATS: instrumented variant that keeps the per-lane work counters and adds them to `stats` at exit; in the timed
+// variant the counters are dead code and take no registers.
+template <int MODE, class Grid, bool STATS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CRVPT_PATHPOOL_WAVES))) void k_pathPool(
         SceneParams P, Grid grid, SupervoxelGrid S, float4* __restrict__ out, uint32_t* __restrict__ counter,
         int stepsPerRefill, unsigned long long* __restrict__ stats, float4* __restrict__ feat) {
@@ -110,7 +112,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CRVPT_PATHP
             }
         }
     }
-    if (stats) flushCounters(stats, st.taps, W.svReads, paths, gc.descents(), gc.lowerLookups());
+    if (STATS) flushCounters(stats, st.taps, W.svReads, paths, gc.descents(), gc.lowerLookups());
 }
 
 // ------------------------------------------------------------------------------------------------------------
@@ -686,6 +688,18 @@ __global__ void k_nanoLookupWorld(NanoAccess grid, int n, const float* __restric
 // ------------------------------------------------------------------------------------------------------------
 // Host-side launchers
 // ------------------------------------------------------------------------------------------------------------
+template <int MODE, class Grid>
+static void launchPool(dim3 grid, dim3 block, hipStream_t stream, const SceneParams& P, const Grid& g,
+                       const SupervoxelGrid& S, float4* paths, uint32_t* counter, int steps,
+                       unsigned long long* stats, float4* feat) {
+    if (stats)
+        hipLaunchKernelGGL((k_pathPool<MODE, Grid, true>), grid, block, 0, stream, P, g, S, paths, counter, steps,
+                           stats, feat);
+    else
+        hipLaunchKernelGGL((k_pathPool<MODE, Grid, false>), grid, block, 0, stream, P, g, S, paths, counter, steps,
+                           stats, feat);
+}
+
 template <class Grid>
 static hipError_t launchRenderT(const SceneParams& P, const Grid& grid, const SupervoxelGrid& S, float4* paths,
                                 uint32_t* counter, const LaunchConfig& cfg, unsigned long long* stats, float4* feat,
@@ -693,34 +707,32 @@ static hipError_t launchRenderT(const SceneParams& P, const Grid& grid, const Su
     hipLaunchKernelGGL(k_zeroU32, dim3(1), dim3(64), 0, stream, counter, 1);
     const dim3 block(256);
     const dim3 gridDim(cfg.numBlocks);
+    const int st = cfg.stepsPerRefill;
     switch (P.mode) {
         case VPT_RATIO_TRACKING:
-            hipLaunchKernelGGL((k_pathPool<VPT_RATIO_TRACKING, Grid>), gridDim, block, 0, stream, P, grid, S, paths,
-                               counter, cfg.stepsPerRefill, stats, feat);
+            launchPool<VPT_RATIO_TRACKING>(gridDim, block, stream, P, grid, S, paths, counter, st, stats, feat);
             break;
         case VPT_NEXT_EVENT_TRACKING:
-            hipLaunchKernelGGL((k_pathPool<VPT_NEXT_EVENT_TRACKING, Grid>), gridDim, block, 0, stream, P, grid, S,
-                               paths, counter, cfg.stepsPerRefill, stats, feat);
+            launchPool<VPT_NEXT_EVENT_TRACKING>(gridDim, block, stream, P, grid, S, paths, counter, st, stats, feat);
             break;
         case VPT_SPECTRAL_DELTA_TRACKING:
-            hipLaunchKernelGGL((k_pathPool<VPT_SPECTRAL_DELTA_TRACKING, Grid>), gridDim, block, 0, stream, P, grid,
-                               S, paths, counter, cfg.stepsPerRefill, stats, feat);
+            launchPool<VPT_SPECTRAL_DELTA_TRACKING>(gridDim, block, stream, P, grid, S, paths, counter, st, stats,
+                                                    feat);
             break;
         case VPT_NEXT_EVENT_TRACKING_SPECTRAL:
-            hipLaunchKernelGGL((k_pathPool<VPT_NEXT_EVENT_TRACKING_SPECTRAL, Grid>), gridDim, block, 0, stream, P,
-                               grid, S, paths, counter, cfg.stepsPerRefill, stats, feat);
+            launchPool<VPT_NEXT_EVENT_TRACKING_SPECTRAL>(gridDim, block, stream, P, grid, S, paths, counter, st,
+                                                         stats, feat);
             break;
         case VPT_DECOMPOSITION_TRACKING:
-            hipLaunchKernelGGL((k_pathPool<VPT_DECOMPOSITION_TRACKING, Grid>), gridDim, block, 0, stream, P, grid,
-                               S, paths, counter, cfg.stepsPerRefill, stats, feat);
+            launchPool<VPT_DECOMPOSITION_TRACKING>(gridDim, block, stream, P, grid, S, paths, counter, st, stats,
+                                                   feat);
             break;
         case VPT_RESIDUAL_RATIO_TRACKING:
-            hipLaunchKernelGGL((k_pathPool<VPT_RESIDUAL_RATIO_TRACKING, Grid>), gridDim, block, 0, stream, P,
-                               grid, S, paths, counter, cfg.stepsPerRefill, stats, feat);
+            launchPool<VPT_RESIDUAL_RATIO_TRACKING>(gridDim, block, stream, P, grid, S, paths, counter, st, stats,
+                                                    feat);
             break;
         default:
-            hipLaunchKernelGGL((k_pathPool<VPT_DELTA_TRACKING, Grid>), gridDim, block, 0, stream, P, grid, S, paths,
-                               counter, cfg.stepsPerRefill, stats, feat);
+            launchPool<VPT_DELTA_TRACKING>(gridDim, block, stream, P, grid, S, paths, counter, st, stats, feat);
             break;
     }
     return hipGetLastError();
@@ -842,14 +854,14 @@ hipError_t launchNanoLookupWorld(const NanoGrid& nano, int n, const float* xyz, 
 template <class Grid>
 static const void* pathPoolKernel(int mode) {
     switch (mode) {
-        case VPT_RATIO_TRACKING: return (const void*)k_pathPool<VPT_RATIO_TRACKING, Grid>;
-        case VPT_NEXT_EVENT_TRACKING: return (const void*)k_pathPool<VPT_NEXT_EVENT_TRACKING, Grid>;
-        case VPT_SPECTRAL_DELTA_TRACKING: return (const void*)k_pathPool<VPT_SPECTRAL_DELTA_TRACKING, Grid>;
+        case VPT_RATIO_TRACKING: return (const void*)k_pathPool<VPT_RATIO_TRACKING, Grid, false>;
+        case VPT_NEXT_EVENT_TRACKING: return (const void*)k_pathPool<VPT_NEXT_EVENT_TRACKING, Grid, false>;
+        case VPT_SPECTRAL_DELTA_TRACKING: return (const void*)k_pathPool<VPT_SPECTRAL_DELTA_TRACKING, Grid, false>;
         case VPT_NEXT_EVENT_TRACKING_SPECTRAL:
-            return (const void*)k_pathPool<VPT_NEXT_EVENT_TRACKING_SPECTRAL, Grid>;
-        case VPT_DECOMPOSITION_TRACKING: return (const void*)k_pathPool<VPT_DECOMPOSITION_TRACKING, Grid>;
-        case VPT_RESIDUAL_RATIO_TRACKING: return (const void*)k_pathPool<VPT_RESIDUAL_RATIO_TRACKING, Grid>;
-        default: return (const void*)k_pathPool<VPT_DELTA_TRACKING, Grid>;
+            return (const void*)k_pathPool<VPT_NEXT_EVENT_TRACKING_SPECTRAL, Grid, false>;
+        case VPT_DECOMPOSITION_TRACKING: return (const void*)k_pathPool<VPT_DECOMPOSITION_TRACKING, Grid, false>;
+        case VPT_RESIDUAL_RATIO_TRACKING: return (const void*)k_pathPool<VPT_RESIDUAL_RATIO_TRACKING, Grid, false>;
+        default: return (const void*)k_pathPool<VPT_DELTA_TRACKING, Grid, false>;
     }
 }
 

@@ -66,8 +66,7 @@ hipError_t launchRender(const SceneParams& P, const DenseGrid* dense, const Nano
 // each request's image. unitBegins / pixBegins: nreq ascending first units / first pixels (device arrays).
 hipError_t launchBatch(int mode, const BatchRequest* reqs, const uint64_t* unitBegins, const uint64_t* pixBegins,
                        int nreq, uint32_t totalUnits, uint64_t totalPix, const BatchLaunch& cfg,
-                       const uint32_t* frameOffset, float4* records, uint32_t* counter, unsigned long long* stats,
-                       hipStream_t stream);
+                       const uint32_t* frameOffset, float4* records, uint32_t* counter, hipStream_t stream);
 hipError_t launchSetU32(uint32_t* p, uint32_t v, hipStream_t stream);
 
 // KPN network kernels (kpn_net.hip). epilogue: 0 bias + ReLU -> bf16, 1 bias -> f32, 2 x (mask > 0) -> bf16.
