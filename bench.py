@@ -41,9 +41,9 @@ C2 = {"max_bounces": 128, "grid": 512, "edge0": 0.645, "edge1": 0.795, "width": 
       "modes": [(0, "delta tracking"), (2, "ratio tracking"), (4, "residual ratio tracking"),
                 (3, "decomposition tracking")], "gs": [0.0, 0.5, 0.85]}
 C1 = {"max_bounces": 64, "grid": 128, "width": 256, "height": 256, "spp": 64}
-MODE_KERNELS = {0: "k_pathPool<DELTA_TRACKING, NanoAccess>", 2: "k_pathPool<RATIO_TRACKING, NanoAccess>",
-                4: "k_pathPool<RESIDUAL_RATIO_TRACKING, NanoAccess>",
-                3: "k_pathPool<DECOMPOSITION_TRACKING, NanoAccess>"}
+MODE_KERNELS = {0: "k_pathPool<DELTA_TRACKING, NanoAccess, false>", 2: "k_pathPool<RATIO_TRACKING, NanoAccess, false>",
+                4: "k_pathPool<RESIDUAL_RATIO_TRACKING, NanoAccess, false>",
+                3: "k_pathPool<DECOMPOSITION_TRACKING, NanoAccess, false>"}
 
 
 def setup_c1(vpt, torch):
@@ -500,7 +500,7 @@ def main() -> int:
     dom_ms = sum(by_mode[dom]) / len(by_mode[dom])
     # Instrumented passes (untimed) of the dominant estimator's combinations: device work counters over the same
     # image and sample count (the counters add wave-reduced atomics, so these passes are not timed).
-    taps = svr = paths = desc = lowl = 0
+    taps = svr = paths = desc = 0
     n_c = 0
     vpt.set_collect_stats(True)
     for k in range(len(combos)):
@@ -514,15 +514,13 @@ def main() -> int:
         svr += c["supervoxel_reads"]
         paths += c["paths"]
         desc += c["tree_descents"]
-        lowl += c["lower_lookups"]
         n_c += 1
     vpt.set_collect_stats(False)
-    taps, svr, paths, desc, lowl = (v / max(n_c, 1) for v in (taps, svr, paths, desc, lowl))
+    taps, svr, paths, desc = (v / max(n_c, 1) for v in (taps, svr, paths, desc))
     # Algorithmic bytes per render: 4-byte FP32 density taps, 4-byte majorant / 8-byte (control, residual) table
-    # reads, 40 bytes per sparse-tree descent from the root (root tile key and child 8 + 8, upper and lower child-mask
-    # words 4 + 4, upper and lower table entries 8 + 8), 12 bytes per child lookup in the memoised lower node (mask
-    # word 4 + table entry 8), and the 16-byte radiance record of each path.
-    b_alg = 4 * taps + (4 if dom in (0, 2, 5) else 8) * svr + 40 * desc + 12 * lowl + 16 * paths
+    # reads, 40 bytes per sparse-tree descent (root tile key and child 8 + 8, upper and lower child-mask words 4 + 4,
+    # upper and lower table entries 8 + 8), and the 16-byte radiance record of each path.
+    b_alg = 4 * taps + (4 if dom in (0, 2, 5) else 8) * svr + 40 * desc + 16 * paths
     achieved = b_alg / (dom_ms * 1e-3) / 1e9
     if c2:
         kernel = MODE_KERNELS[dom]
@@ -532,7 +530,7 @@ def main() -> int:
                   "extinction": 1024.0, "environment": "default sky", "max_bounces": C2["max_bounces"],
                   "majorant": "16^3 supervoxel grid", "steps_cycle": "estimator x g (12 combinations)"}
     else:
-        kernel = "k_pathPool<NEXT_EVENT_TRACKING, DenseAccess>"
+        kernel = "k_pathPool<NEXT_EVENT_TRACKING, DenseAccess, false>"
         config = {"workload": args.workload, "volume": "dense 128^3 fp32 fBm seed 0", "image": f"{W}x{H}",
                   "spp_per_step": spp, "estimator": "next event tracking", "phase_g": 0.85, "albedo": 0.9,
                   "extinction": 1024.0, "environment": "constant white image + directional sun",
@@ -557,7 +555,7 @@ def main() -> int:
                      "kernel": kernel, "launches_per_step": launches,
                      "launch_ms": round(dom_ms / launches, 4), "algorithmic_bytes_per_launch": int(b_alg / launches),
                      "density_taps": int(taps), "supervoxel_reads": int(svr), "tree_descents": int(desc),
-                     "lower_node_lookups": int(lowl), "paths": int(paths)},
+                     "paths": int(paths)},
         "kernel_ms_per_mode_g": per_combo,
     }
     if not args.no_cpu_baseline:

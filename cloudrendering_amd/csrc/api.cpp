@@ -736,8 +736,8 @@ int crvpt_render_frame(float* out, int channels, int height, int width, int fram
     HIP_TRY(r.dCounter.ensure(64));
     unsigned long long* stats = nullptr;
     if (r.collectStats) {
-        HIP_TRY(r.dStats.ensure(5 * sizeof(unsigned long long)));
-        HIP_TRY(hipMemsetAsync(r.dStats.ptr, 0, 5 * sizeof(unsigned long long), s));
+        HIP_TRY(r.dStats.ensure(4 * sizeof(unsigned long long)));
+        HIP_TRY(hipMemsetAsync(r.dStats.ptr, 0, 4 * sizeof(unsigned long long), s));
         stats = r.dStats.as<unsigned long long>();
     }
     // SVGF wants non-accumulated input and the global frame number (SVGF.hpp:81-82, DISABLE_ACCUMULATION): every
@@ -967,16 +967,16 @@ int crvpt_set_collect_stats(int collect) {
     return CRVPT_OK;
 }
 
-int crvpt_get_last_render_counters(uint64_t out[5]) {
+int crvpt_get_last_render_counters(uint64_t out[4]) {
     if (int rc = requireInit()) return rc;
     Renderer& r = *g_r;
     if (!r.collectStats || !r.dStats.ptr) return fail(CRVPT_ERR_INVALID_ARGUMENT, "get_last_render_counters: "
                                                       "enable crvpt_set_collect_stats before rendering");
-    unsigned long long v[5];
+    unsigned long long v[4];
     HIP_TRY(hipStreamSynchronize(r.stream));
     HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMemcpy(v, r.dStats.ptr, sizeof(v), hipMemcpyDeviceToHost));
-    for (int i = 0; i < 5; i++) out[i] = v[i];
+    for (int i = 0; i < 4; i++) out[i] = v[i];
     return CRVPT_OK;
 }
 

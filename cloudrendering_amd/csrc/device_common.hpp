@@ -276,7 +276,6 @@ CRVPT_DEV V3 lightRadiance(const SceneParams& P, V3 pos, int lightIdx) {
 // Dense grids need no lookup memo: a voxel is one load.
 struct NoCache {
     CRVPT_DEV uint32_t descents() const { return 0u; }
-    CRVPT_DEV uint32_t lowerLookups() const { return 0u; }
 };
 
 struct DenseAccess {

@@ -15,28 +15,21 @@
 
 #include "device_common.hpp"
 
-#ifndef CRVPT_LOWER_MEMO
-#define CRVPT_LOWER_MEMO 1
-#endif
-
 namespace crvpt {
 
 // Per-lane memo of the last 8^3 block a lookup descended to: the block coordinate (index >> 3), and either the
-// leaf's byte offset or, when the block is covered by a tile (no leaf), the tile value; plus the byte offset of the
-// lower node (128^3 voxels) holding that block, 0 when the whole 128^3 region is one upper-level or root-level tile
-// (then `tile` is its value). A path's consecutive lookups are about a voxel apart: most hit the block memo; most
-// of the others stay in the same lower node and need one child-table read instead of the root-upper-lower descent
-// (six dependent loads). Every answer equals the full descent's.
+// leaf's byte offset or, when the block is covered by a tile (no leaf), the tile value. A path's consecutive
+// lookups are about a voxel apart, so most of them hit the memo and skip the root-upper-lower descent (six dependent
+// loads); a hit returns the same value the descent would. (Also memoising the block's lower node, so that block
+// changes inside it need one child-table read, measured slower on MI355X: the extra registers cost more than the
+// descents they save.)
 struct LeafCache {
     int bi = 0x7FFFFFFF, bj = 0, bk = 0;
     uint32_t leaf = 0;  // 0: the block is a tile of value `tile`
     float tile = 0.0f;
-    uint32_t lower = 0;  // lower node of the block's 128^3 region; 0: the region is a tile of value `tile`
     float qMin = 0.0f, qScale = 0.0f;  // Fp16 leaves: the leaf's minimum and quantum
-    uint32_t misses = 0;     // full descents from the root (work counter)
-    uint32_t lowerHits = 0;  // block changes resolved in the memoised lower node (work counter)
+    uint32_t misses = 0;  // descents (work counter)
     CRVPT_DEV uint32_t descents() const { return misses; }
-    CRVPT_DEV uint32_t lowerLookups() const { return lowerHits; }
 };
 
 // a * b + c rounded after the product and after the sum (never fused into one FMA).
@@ -69,9 +62,7 @@ struct NanoAccess {
     }
 
     // Descends to the 8^3 block holding (i, j, k): returns the leaf offset, or 0 with the tile value in `tile`.
-    // lowerOut: the lower node on the way, 0 when the block lies in an upper-level or root-level tile.
-    CRVPT_DEV uint32_t locateBlock(int i, int j, int k, float& tile, uint32_t& lowerOut) const {
-        lowerOut = 0u;
+    CRVPT_DEV uint32_t locateBlock(int i, int j, int k, float& tile) const {
         const uint64_t key = uint64_t(uint32_t(k) >> 12) | (uint64_t(uint32_t(j) >> 12) << 21) |
                              (uint64_t(uint32_t(i) >> 12) << 42);
         uint32_t rt = g.rootOffset + 64u;
@@ -96,26 +87,14 @@ struct NanoAccess {
             return 0u;
         }
         const uint32_t lower = upper + uint32_t(u64(upper + 8256u + 8u * nu));
-        lowerOut = lower;
         return leafInLower(lower, i, j, k, tile);
-    }
-    CRVPT_DEV uint32_t locateBlock(int i, int j, int k, float& tile) const {
-        uint32_t lower;
-        return locateBlock(i, j, k, tile, lower);
     }
 
     CRVPT_DEV float value(int i, int j, int k, LeafCache& c) const {
         const int bi = i >> 3, bj = j >> 3, bk = k >> 3;  // arithmetic shifts: blocks of negative indices too
         if (bi != c.bi || bj != c.bj || bk != c.bk) {
-            if (CRVPT_LOWER_MEMO && (((bi ^ c.bi) | (bj ^ c.bj) | (bk ^ c.bk)) >> 4) == 0) {
-                // same 128^3 region as the memoised block: its lower node, or the region's tile
-                if (c.lower != 0u) c.leaf = leafInLower(c.lower, i, j, k, c.tile);
-                else c.leaf = 0u;
-                c.lowerHits++;
-            } else {
-                c.leaf = locateBlock(i, j, k, c.tile, c.lower);
-                c.misses++;
-            }
+            c.leaf = locateBlock(i, j, k, c.tile);
+            c.misses++;
             if (g.fp16 && c.leaf != 0u) {
                 c.qMin = f32(c.leaf + 80u);
                 c.qScale = f32(c.leaf + 84u);

@@ -659,17 +659,17 @@ CRVPT_DEV bool resolvePending(PathState& st, Walker& W, const SceneParams& P, co
 }
 
 // Wave-reduced 64-bit work counters: [0] density taps, [1] supervoxel majorant / table reads, [2] paths started,
-// [3] sparse-tree descents from the root (lookup memo misses), [4] block changes resolved in the memoised lower node.
+// [3] sparse-tree descents (lookup memo misses).
 CRVPT_DEV void flushCounters(unsigned long long* stats, uint32_t taps, uint32_t svs, uint32_t paths,
-                             uint32_t descents, uint32_t lowerLookups) {
-    unsigned long long v[5] = {taps, svs, paths, descents, lowerLookups};
+                             uint32_t descents) {
+    unsigned long long v[4] = {taps, svs, paths, descents};
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1)
 #pragma unroll
-        for (int q = 0; q < 5; q++) v[q] += __shfl_down(v[q], off);
+        for (int q = 0; q < 4; q++) v[q] += __shfl_down(v[q], off);
     if (__lane_id() == 0)
 #pragma unroll
-        for (int q = 0; q < 5; q++) atomicAdd(&stats[q], v[q]);
+        for (int q = 0; q < 4; q++) atomicAdd(&stats[q], v[q]);
 }
 
 #ifndef CRVPT_RESOLVE_LANES

@@ -112,7 +112,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CRVPT_PATHP
             }
         }
     }
-    if (STATS) flushCounters(stats, st.taps, W.svReads, paths, gc.descents(), gc.lowerLookups());
+    if (STATS) flushCounters(stats, st.taps, W.svReads, paths, gc.descents());
 }
 
 // ------------------------------------------------------------------------------------------------------------

@@ -549,10 +549,10 @@ def set_collect_stats(collect: bool) -> None:
 
 
 def get_last_render_counters() -> dict:
-    v = (ctypes.c_uint64 * 5)()
+    v = (ctypes.c_uint64 * 4)()
     call("crvpt_get_last_render_counters", v)
     return {"density_taps": int(v[0]), "supervoxel_reads": int(v[1]), "paths": int(v[2]),
-            "tree_descents": int(v[3]), "lower_lookups": int(v[4])}
+            "tree_descents": int(v[3])}
 
 
 def build_nanovdb_from_dense(density_zyx, dx: float, origin=(0.0, 0.0, 0.0), value_type: int = 0) -> bytes:

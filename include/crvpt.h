@@ -203,12 +203,11 @@ int crvpt_kpn_inv_iter_kernel(const float* x, const float* low_res_pred, const f
 int crvpt_get_last_render_stats(double* path_kernel_ms, double* total_ms, uint64_t* num_paths, int* launches);
 
 /* No reference counterpart: work counters of the path kernels for the algorithmic-byte roofline. When enabled,
- * each crvpt_render_frame accumulates [density taps, supervoxel majorant / table reads, paths, NanoVDB tree
- * descents from the root (lookups outside the memoised 8^3 block and its 128^3 lower node), NanoVDB lower-node
- * child lookups (lookups outside the memoised block but inside its lower node)] (wave-reduced atomics at kernel
- * exit); crvpt_get_last_render_counters reads the totals of the last call. */
+ * each crvpt_render_frame runs the instrumented variant of the path kernel and accumulates [density taps,
+ * supervoxel majorant / table reads, paths, NanoVDB tree descents (lookups that missed the per-lane 8^3-block
+ * memo)] (wave-reduced atomics at kernel exit); crvpt_get_last_render_counters reads the totals of the last call. */
 int crvpt_set_collect_stats(int collect);
-int crvpt_get_last_render_counters(uint64_t out[5]);
+int crvpt_get_last_render_counters(uint64_t out[4]);
 
 /* Kernel-prediction denoiser network. No reference counterpart in C++ (the reference loads a TorchScript model
  * trained elsewhere, PyTorchDenoiser.cpp:644, and applies its kernels with modules/kpn_module Kernel.cu:38-83): six 3x3
