@@ -513,8 +513,8 @@ bool featuresActive(const Renderer& r) {
 FeatureAcc featureAcc(Renderer& r, uint32_t npix) {
     float4* b = r.dFeatAcc.as<float4>();
     FeatureAcc a;
-    float4** slots[kFeatureAccArrays] = {&a.firstX, &a.firstW, &a.normal, &a.cloudOnly, &a.background,
-                                         &a.albedo, &a.depth, &a.dens, &a.flow, &a.flowRev};
+    float4** slots[kFeatureAccArrays] = {&a.firstX, &a.firstW,  &a.normal, &a.cloudOnly, &a.background, &a.albedo,
+                                         &a.depth,  &a.dens,    &a.flow,   &a.flowRev,   &a.radianceSq};
     for (int i = 0; i < kFeatureAccArrays; i++) *slots[i] = b + size_t(i) * npix;
     return a;
 }
@@ -842,7 +842,7 @@ int crvpt_render_frame(float* out, int channels, int height, int width, int fram
 namespace {
 // FeatureMapTypeVpt values the gfx950 path produces (VolumetricPathTracingPass.hpp:67-73); Depth Blended, Depth
 // Nearest Opaque, the transmittance volume and the absorption moments belong to renderers outside this path.
-bool featureMapSupported(int m) { return (m >= 0 && m <= 14) || m == 17; }
+bool featureMapSupported(int m) { return (m >= 0 && m <= 14) || m == 17 || m == kFeatureMapRadianceVariance; }
 }  // namespace
 
 int crvpt_set_use_feature_maps(const int32_t* maps, int count) {

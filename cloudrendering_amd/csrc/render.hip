@@ -255,7 +255,7 @@ __global__ void k_accumulateFeatures(SceneParams P, const float4* __restrict__ p
     const float px = float(p % uint32_t(P.width)), py = float(p / uint32_t(P.width));
     float4 firstX = A.firstX[p], firstW = A.firstW[p], normal = A.normal[p], cloudOnly = A.cloudOnly[p];
     float4 background = A.background[p], albedo = A.albedo[p], depth = A.depth[p], dens = A.dens[p];
-    float4 flow = A.flow[p], flowRev = A.flowRev[p];
+    float4 flow = A.flow[p], flowRev = A.flowRev[p], radSq = A.radianceSq[p];
     const float* M = P.prevViewProj;
     for (uint32_t f = 0; f < frames; f++) {
         const uint32_t frame = prevFrames + f;
@@ -267,7 +267,7 @@ __global__ void k_accumulateFeatures(SceneParams P, const float4* __restrict__ p
         const bool has = f0.w >= 0.0f;
         if (frame == 0) {
             const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            firstX = normal = cloudOnly = background = albedo = depth = dens = z;
+            firstX = normal = cloudOnly = background = albedo = depth = dens = radSq = z;
             flow = make_float4(0.0f, 0.0f, -1.0f, -1.0f);
         }
         PathState st;
@@ -279,6 +279,7 @@ __global__ void k_accumulateFeatures(SceneParams P, const float4* __restrict__ p
             return make_float4(o.x * b + v.x * a, o.y * b + v.y * a, o.z * b + v.z * a, o.w * b + v.w * a);
         };
         cloudOnly = mix4(cloudOnly, has ? make_float4(res.x, res.y, res.z, 1.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+        radSq = mix4(radSq, make_float4(res.x * res.x, res.y * res.y, res.z * res.z, 0.0f));
         const V3 sky = sampleSky(P, w);
         background = mix4(background, has ? make_float4(sky.x, sky.y, sky.z, 1.0f) : make_float4(res.x, res.y, res.z, 1.0f));
         firstX = mix4(firstX, has ? make_float4(f0.x, f0.y, f0.z, 1.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f));
@@ -343,6 +344,7 @@ __global__ void k_accumulateFeatures(SceneParams P, const float4* __restrict__ p
     A.dens[p] = dens;
     A.flow[p] = flow;
     A.flowRev[p] = flowRev;
+    A.radianceSq[p] = radSq;
 }
 
 // FeatureMapTypeVpt (VolumetricPathTracingPass.hpp:67-73) blitted to RGBA as the reference's readback does: RG and R
@@ -381,6 +383,13 @@ __global__ void k_featureOut(int map, FeatureAcc A, const float4* __restrict__ r
         case 13: v = A.background[p]; break;
         case 14: v = make_float4(A.flow[p].z, A.flow[p].w, 0.0f, 1.0f); break;
         case 17: v = A.albedo[p]; break;
+        case kFeatureMapRadianceVariance: {
+            // E[x^2] - E[x]^2 of the accumulated samples (population variance of the running mean's samples)
+            const float4 q = A.radianceSq[p], m = result[p];
+            v = make_float4(fmaxf(0.0f, q.x - m.x * m.x), fmaxf(0.0f, q.y - m.y * m.y), fmaxf(0.0f, q.z - m.z * m.z),
+                            1.0f);
+            break;
+        }
         default: v = make_float4(result[p].x, result[p].y, result[p].z, 1.0f); break;
     }
     outCHW[p] = v.x;

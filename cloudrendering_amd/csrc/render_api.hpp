@@ -21,8 +21,12 @@ struct FeatureAcc {
     float4* dens;        // (mean density * 1e-3, its mean square, Depth (nabla) x, y)
     float4* flow;        // (Flow x, y of the last sample, Reprojected UV)
     float4* flowRev;     // (Flow Reverse x, y of the last sample, 0, 0)
+    float4* radianceSq;  // running mean of the squared sample radiance per channel (the Radiance Variance map)
 };
-constexpr int kFeatureAccArrays = 10;
+constexpr int kFeatureAccArrays = 11;
+// Feature map index of the per-pixel sample variance of the radiance: an extension after the reference's
+// FeatureMapTypeVpt values (0..20), for denoisers that weight by the noise level.
+constexpr int kFeatureMapRadianceVariance = 21;
 
 struct LaunchConfig {
     uint32_t numBlocks;      // persistent workgroups of 256 threads

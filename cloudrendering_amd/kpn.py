@@ -112,23 +112,15 @@ class KpnDenoiser:
 
 
 def render_inputs(vpt, height: int, width: int, spp: int = 4):
-    """Renders spp samples per pixel of the renderer's current scene one frame at a time and returns (features
-    [15, H, W], noisy radiance [3, H, W]) on the GPU: the running-mean radiance, its per-pixel sample variance, and
-    the reference's first-event feature maps (Albedo, Depth, First X coverage, Depth (nabla), Density)."""
+    """Renders spp samples per pixel of the renderer's current scene and returns (features [15, H, W], noisy radiance
+    [3, H, W]) on the GPU: the radiance mean and its per-pixel sample variance (the Radiance Variance map, scaled to
+    the unbiased estimate), and the reference's first-event feature maps (Albedo, Depth, First X coverage,
+    Depth (nabla), Density)."""
     torch = _torch()
-    vpt.set_use_feature_maps(["Albedo", "Depth", "First X", "Depth (nabla)", "Density"])
+    vpt.set_use_feature_maps(["Albedo", "Depth", "First X", "Depth (nabla)", "Density", "Radiance Variance"])
     probe = torch.zeros((4, height, width), dtype=torch.float32, device="cuda")
-    s1 = torch.zeros((3, height, width), dtype=torch.float32, device="cuda")
-    s2 = torch.zeros_like(s1)
-    prev = torch.zeros_like(s1)
-    for k in range(spp):
-        mean = vpt.render_frame(probe, 1)[:3]
-        x = (k + 1) * mean - k * prev  # this frame's sample from the running means
-        prev = mean
-        s1 += x
-        s2 += x * x
-    mu = s1 / spp
-    var = torch.clamp(s2 / spp - mu * mu, min=0.0) * (spp / max(spp - 1, 1))
+    mu = vpt.render_frame(probe, spp)[:3]
+    var = vpt.get_feature_map(probe, 21)[:3] * (spp / max(spp - 1, 1))
     alb = vpt.get_feature_map(probe, 17)
     dep = vpt.get_feature_map(probe, 6)
     fx = vpt.get_feature_map(probe, 1)

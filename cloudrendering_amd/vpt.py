@@ -91,6 +91,7 @@ VPT_FEATURE_MAP_NAMES = [
     "Depth Blended", "Depth Nearest Opaque", "Albedo",
     "Transmittance Volume",
     "Primary Ray Absorption Moments", "Scatter Ray Absorption Moments",
+    "Radiance Variance",  # extension: per-pixel sample variance of the accumulated radiance
 ]
 
 

@@ -123,3 +123,31 @@ def test_map_means_vs_oracle(vpt, oracle, name, mode):
         mr, sr = r.mean(0), r.std(0, ddof=1) / math.sqrt(nb)
         tol = 4.0 * np.sqrt(sg ** 2 + sr ** 2) + 1e-4 * (1 + np.abs(mr))
         assert np.all(np.abs(mg - mr) <= tol), f"{name}/{k}: gpu {mg} oracle {mr} tol {tol}"
+
+
+def test_radiance_variance_map(vpt, oracle):
+    """The Radiance Variance map (an extension) is E[x^2] - E[x]^2 of the accumulated samples: the same statistic
+    computed from the running means of 1-spp frames rendered with the same random numbers."""
+    print("case: radiance variance map delta 20^3", flush=True)
+    torch = _torch()
+    _setup(vpt, oracle.fbm_cloud(20, 20, 20, seed=33), 0)
+    vpt.set_use_feature_maps(["Radiance Variance"])
+    t = torch.zeros((4, RES, RES), dtype=torch.float32, device="cuda")
+    frames = 6
+    s1 = np.zeros((3, RES, RES), np.float64)
+    s2 = np.zeros((3, RES, RES), np.float64)
+    prev = np.zeros((3, RES, RES), np.float64)
+    for k in range(frames):
+        mean = vpt.render_frame(t, 1).cpu().numpy()[:3].astype(np.float64)
+        x = (k + 1) * mean - k * prev
+        prev = mean
+        s1 += x
+        s2 += x * x
+    var = vpt.get_feature_map(t, 21).cpu().numpy()
+    ref = np.maximum(s2 / frames - (s1 / frames) ** 2, 0.0)
+    np.testing.assert_allclose(var[:3], ref, rtol=2e-4, atol=2e-5)
+    np.testing.assert_array_equal(var[3], 1.0)
+    vpt.set_seed_offset(0)
+    one = vpt.render_frame(t, frames)  # the same samples in one call: the same map
+    np.testing.assert_allclose(vpt.get_feature_map(t, 21).cpu().numpy(), var, rtol=1e-4, atol=1e-6)
+    assert one is not None
