@@ -190,7 +190,7 @@ def run_c3(args, vpt, torch):
     print(json.dumps(line), flush=True)
 
 
-C4 = {"crops": 384, "crop": 128, "clouds": 16, "input_spp": [4, 16, 64], "target_spp": 1024, "batch": 16,
+C4 = {"crops": 384, "crop": 128, "clouds": 16, "input_spp": [4, 16, 64, 256], "target_spp": 1024, "batch": 16,
       "train_steps": 3000, "lr": 1e-3, "eval_res": 1024, "ref_spp": 16384, "plain_spp": [4, 16, 64, 256, 1024, 4096],
       "max_bounces": 128}
 
@@ -221,10 +221,10 @@ def _c4_camera(vpt, rng):
 
 
 def run_c4(args, vpt, torch):
-    """Kernel-prediction denoiser: training pairs rendered by the tracer (inputs: 4, 16 or 64 spp with the feature
+    """Kernel-prediction denoiser: training pairs rendered by the tracer (inputs: 4, 16, 64 or 256 spp with the feature
     maps; targets: 1024 spp, all crops in one batched launch), training with the hand-written bf16 MFMA network, and
     the comparison on an unseen cloud at 1024^2 against a 16k-spp reference: the wall time and relative MSE of plain
-    tracking at increasing spp and of the denoised 4 / 16 / 64-spp renders (render + features + inference)."""
+    tracking at increasing spp and of the denoised 4 / 16 / 64 / 256-spp renders (render + features + inference)."""
     import numpy as np
 
     from cloudrendering_amd import kpn
