@@ -110,24 +110,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CRVPT_PATHP
             if (__ballot(!(st.flags & PF_ALIVE) && (left > 0u || !exhausted)) == 0ull) break;
         }
         if (__ballot((st.flags & PF_ALIVE) != 0) == 0ull) break;  // every lane is exhausted
-        const bool queueOpen = __ballot(exhausted) == 0ull;
         for (int k = 0; k < stepsPerRefill; k++) {
             if ((st.flags & (PF_ALIVE | PF_PENDING)) == PF_ALIVE) {
                 const BatchRequest& R = reqs[req];
                 if (!stepPath<MODE, BatchGrid>(st, W, R.P, R.S, BatchGrid{&R}, gc, nullptr)) finishSample();
             }
-            // enough idle lanes that a refill can use (new units while the queue lasts, or the next sample of their
-            // unit) to refill now instead of at the end of the loop
-            const bool idle = !(st.flags & PF_ALIVE);
-            const bool refill = __popcll(__ballot(idle && (queueOpen || left > 0u))) >= kRefillLanes;
             const uint64_t pend = __ballot((st.flags & PF_PENDING) != 0);
-            if (pend != 0ull && (__popcll(pend) >= kResolveLanes || k == stepsPerRefill - 1 || refill)) {
+            if (pend != 0ull && (__popcll(pend) >= kResolveLanes || k == stepsPerRefill - 1)) {
                 if (st.flags & PF_PENDING) {
                     const BatchRequest& R = reqs[req];
                     if (!resolvePending<MODE, BatchGrid>(st, W, R.P, R.S, BatchGrid{&R}, nullptr)) finishSample();
                 }
             }
-            if (refill) break;
         }
     }
 }

@@ -676,9 +676,5 @@ CRVPT_DEV void flushCounters(unsigned long long* stats, uint32_t taps, uint32_t 
 #define CRVPT_RESOLVE_LANES 16
 #endif
 constexpr int kResolveLanes = CRVPT_RESOLVE_LANES;  // pending lanes that trigger a resolve pass of the wave
-#ifndef CRVPT_REFILL_LANES
-#define CRVPT_REFILL_LANES 16
-#endif
-constexpr int kRefillLanes = CRVPT_REFILL_LANES;  // finished lanes that end the step loop early for a refill
 
 }  // namespace crvpt

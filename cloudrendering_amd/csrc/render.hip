@@ -94,7 +94,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CRVPT_PATHP
             }
         }
         if (__ballot((st.flags & PF_ALIVE) != 0) == 0) break;  // every lane is exhausted
-        const bool queueOpen = __ballot(exhausted) == 0ull;
         for (int k = 0; k < stepsPerRefill; k++) {
             if ((st.flags & (PF_ALIVE | PF_PENDING)) == PF_ALIVE) {
                 if (!stepPath<MODE, Grid>(st, W, P, S, grid, gc, feat)) {
@@ -102,10 +101,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CRVPT_PATHP
                     st.flags = 0;
                 }
             }
-            // enough finished lanes to refill now (while the queue lasts) instead of idling to the end of the loop
-            const bool refill = queueOpen && __popcll(__ballot(!(st.flags & PF_ALIVE))) >= kRefillLanes;
             const uint64_t pend = __ballot((st.flags & PF_PENDING) != 0);
-            if (pend != 0ull && (__popcll(pend) >= kResolveLanes || k == stepsPerRefill - 1 || refill)) {
+            if (pend != 0ull && (__popcll(pend) >= kResolveLanes || k == stepsPerRefill - 1)) {
                 if (st.flags & PF_PENDING) {
                     if (!resolvePending<MODE, Grid>(st, W, P, S, grid, feat)) {
                         writePath(out, st);
@@ -113,7 +110,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CRVPT_PATHP
                     }
                 }
             }
-            if (refill) break;
         }
     }
     if (STATS) flushCounters(stats, st.taps, W.svReads, paths, gc.descents());
