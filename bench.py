@@ -19,6 +19,11 @@ Workload "c4_kpn": the kernel-prediction denoiser: training pairs rendered by th
 1024-spp targets in one batched launch), training on the GPU, and the equal-quality wall-time comparison with plain
 tracking against a 16k-spp reference on an unseen cloud; prints its own JSON line.
 
+Workload "c5_fp16_2048": the large sparse production frame: NanoVDB with Fp16 leaves over a 2048^3 domain (6-octave
+anvil fBm, about 5 % active), 1920 x 1080, 1024 spp, residual ratio tracking, at most 256 scattering events, plus in
+the same loop per frame an 8-spp render with feature maps (training-pair input), kernel-prediction training steps on
+crops of the pairs and the network's inference on the full frame; prints its own JSON line.
+
 Prints one JSON line: Mpaths/s over the timed steps, the roofline of the dominant kernel (algorithmic bytes from
 device work counters of an instrumented pass with identical seeds, divided by the HIP-event time of that kernel in
 the timed passes), and the CPU oracle timed on a bounded sample of the same scene on the host cores.
@@ -333,6 +338,159 @@ def run_c4(args, vpt, torch):
     print(json.dumps(line), flush=True)
 
 
+C5 = {"grid": 2048, "octaves": 6, "edge0": 0.78, "edge1": 0.93, "width": 1920, "height": 1080, "spp": 1024,
+      "max_bounces": 256, "pair_spp": 8, "crop": 128, "batch": 16, "train_steps": 100, "lr": 1e-3}
+
+
+def setup_c5(vpt):
+    """Large sparse production frame: NanoVDB with Fp16 leaves over a 2048^3 domain holding a 6-octave seeded fBm
+    with the anvil height profile, smoothstep(0.78, 0.93) (about 5 % of the voxels active), 16^3 supervoxels,
+    residual ratio tracking with the per-supervoxel control extinction, extinction 1024, albedo 0.9, g 0.5, default
+    sky, at most 256 scattering events per path."""
+    g = C5["grid"]
+    print(f"c5: generating the {g}^3 fBm field", file=sys.stderr, flush=True)
+    vpt.set_sparse_value_type(1)
+    vpt.generate_fbm_cloud(g, g, g, seed=0, octaves=C5["octaves"], lacunarity=2.0, gain=0.5, base_frequency=4.0,
+                           edge0=C5["edge0"], edge1=C5["edge1"], height_falloff=True)
+    vpt.set_use_sparse_grid(True)
+    vpt.set_super_voxel_size(16)
+    vpt.set_vpt_mode(4)
+    vpt.set_grid_interpolation_type(1)
+    vpt.set_extinction_base([1.0, 1.0, 1.0])
+    vpt.set_extinction_scale(1024.0)
+    vpt.set_scattering_albedo([0.9, 0.9, 0.9])
+    vpt.set_phase_g(0.5)
+    vpt.set_use_builtin_environment_map("Default")
+    vpt.set_use_lights(False)
+    vpt.set_max_bounces(C5["max_bounces"])
+
+
+def _c5_camera(vpt, k):
+    az = 0.35 * k
+    el = 0.15
+    vpt.set_camera_position([0.8 * math.cos(el) * math.sin(az), 0.8 * math.sin(el), 0.8 * math.cos(el) * math.cos(az)])
+    vpt.set_camera_target([0.0, 0.0, 0.0])
+
+
+def run_c5(args, vpt, torch):
+    """One production frame per step: the 1920x1080, 1024-spp residual-ratio render of the 2048^3 Fp16 grid, then
+    in the same loop an 8-spp render of the same view with the denoiser's feature maps (the low-spp half of a
+    training pair whose high-spp half is the frame itself), training steps of the kernel-prediction network on
+    128^2 crops of every pair so far, and the network's inference on the full 8-spp frame. The camera moves on an
+    orbit from frame to frame."""
+    import ctypes
+
+    import numpy as np
+
+    from cloudrendering_amd import kpn
+
+    t0 = time.perf_counter()
+    setup_c5(vpt)
+    W, H, spp, pspp = C5["width"], C5["height"], args.spp or C5["spp"], C5["pair_spp"]
+    target = torch.zeros((3, H, W), dtype=torch.float32, device="cuda")
+    _c5_camera(vpt, 0)
+    print(f"c5: field generated in {time.perf_counter() - t0:.1f} s; building the Fp16 grid and the tables",
+          file=sys.stderr, flush=True)
+    vpt.render_frame(torch.zeros((3, 64, 64), dtype=torch.float32, device="cuda"), 1)  # builds grid and tables
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    size = ctypes.c_uint64()
+    vpt.call("crvpt_get_nanovdb_buffer", None, 0, ctypes.byref(size))
+    nano = {"bytes": int(size.value), "leaf_bytes": 1120}
+    print(f"c5 setup (generate 2048^3, Fp16 NanoVDB, tables) {setup_s:.1f} s", file=sys.stderr, flush=True)
+    net = kpn.KpnDenoiser(seed=0)
+    gen = torch.Generator(device="cpu").manual_seed(0)
+    pairs = []
+    cs = C5["crop"]
+
+    def frame(k, log):
+        ts = {}
+        _c5_camera(vpt, k)
+        vpt.set_use_feature_maps([])
+        vpt.set_seed_offset(2 * k)
+        t = time.perf_counter()
+        img = vpt.render_frame(target, spp).clone()
+        torch.cuda.synchronize()
+        ts["render_s"] = time.perf_counter() - t
+        ts["path_kernel_ms"] = vpt.get_last_render_stats()["path_kernel_ms"]
+        t = time.perf_counter()
+        vpt.set_seed_offset(2 * k + 1)
+        f, x = kpn.render_inputs(vpt, H, W, pspp)
+        pairs.append((f, x, img))
+        torch.cuda.synchronize()
+        ts["pair_s"] = time.perf_counter() - t
+        t = time.perf_counter()
+        for _ in range(C5["train_steps"]):
+            fs, xs, ys = [], [], []
+            for _ in range(C5["batch"]):
+                pf, px, py = pairs[int(torch.randint(0, len(pairs), (1,), generator=gen))]
+                oy = int(torch.randint(0, H - cs + 1, (1,), generator=gen))
+                ox = int(torch.randint(0, W - cs + 1, (1,), generator=gen))
+                fs.append(pf[:, oy:oy + cs, ox:ox + cs])
+                xs.append(px[:, oy:oy + cs, ox:ox + cs])
+                ys.append(py[:, oy:oy + cs, ox:ox + cs])
+            loss = net.train_step(torch.stack(fs), torch.stack(xs), torch.stack(ys), C5["lr"])
+        torch.cuda.synchronize()
+        ts["train_s"] = time.perf_counter() - t
+        t = time.perf_counter()
+        den = net.denoise(f.unsqueeze(0), x.unsqueeze(0))[0]
+        torch.cuda.synchronize()
+        ts["inference_s"] = time.perf_counter() - t
+        ts["train_loss"] = loss
+        ts["noisy_rel_mse"] = kpn.rel_mse(x, img)
+        ts["denoised_rel_mse"] = kpn.rel_mse(den, img)
+        print(f"{log} {k}: " + ", ".join(f"{a} {b:.4g}" for a, b in ts.items()), file=sys.stderr, flush=True)
+        return ts
+
+    for k in range(args.warmup):
+        frame(k, "warmup frame")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    per = [frame(args.warmup + k, "frame") for k in range(args.steps)]
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    paths = W * H * (spp + pspp)
+    value = paths / (wall / args.steps) / 1e6
+    # instrumented pass of the production render (untimed): device work counters for the algorithmic bytes
+    vpt.set_use_feature_maps([])
+    vpt.set_collect_stats(True)
+    vpt.set_seed_offset(0)
+    _c5_camera(vpt, 0)
+    vpt.render_frame(target, spp)
+    torch.cuda.synchronize()
+    c = vpt.get_last_render_counters()
+    vpt.set_collect_stats(False)
+    kernel_ms = float(np.mean([p["path_kernel_ms"] for p in per]))
+    launches = vpt.get_last_render_stats()["launches"]
+    # Fp16 leaves: 2-byte codes per tap; a descent also reads the leaf's minimum and quantum (8 bytes)
+    b_alg = 2 * c["density_taps"] + 8 * c["supervoxel_reads"] + 48 * c["tree_descents"] + 16 * c["paths"]
+    achieved = b_alg / (kernel_ms * 1e-3) / 1e9
+    keys = ("render_s", "pair_s", "train_s", "inference_s")
+    line = {
+        "metric": "Mpaths/s on 1 MI355X", "value": round(value, 3), "unit": "Mpaths/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded fBm procedural cloud generated on the GPU)",
+        "config": {"workload": "c5_fp16_2048", "volume": f"NanoVDB Fp16 leaves, 2048^3 anvil fBm seed 0, "
+                   f"smoothstep({C5['edge0']}, {C5['edge1']})", "image": f"{W}x{H}", "spp_per_frame": spp,
+                   "pair_spp": pspp, "estimator": "residual ratio tracking", "phase_g": 0.5, "albedo": 0.9,
+                   "extinction": 1024.0, "max_bounces": C5["max_bounces"], "majorant": "16^3 supervoxel grid",
+                   "kpn_train_steps_per_frame": C5["train_steps"], "kpn_batch": C5["batch"], "crop": cs,
+                   "paths_per_step": paths},
+        "setup_s": round(setup_s, 2), "nanovdb": nano,
+        "per_frame_s": {k: round(float(np.mean([p[k] for p in per])), 4) for k in keys},
+        "rel_mse_vs_frame": {"noisy_8spp": [round(p["noisy_rel_mse"], 6) for p in per],
+                             "denoised_8spp": [round(p["denoised_rel_mse"], 6) for p in per]},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": load_traffic("c5_fp16_2048"),
+                     "kernel": "k_pathPool<RESIDUAL_RATIO_TRACKING, NanoAccess, false>", "launches_per_frame": launches,
+                     "kernel_ms_per_frame": round(kernel_ms, 3), "algorithmic_bytes_per_frame": int(b_alg),
+                     "counters": {k: int(v) for k, v in c.items()}},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def cpu_baseline_c1(budget_s: float = 12.0):
     """The oracle (FP32 restatement of the reference shaders, hybrid-Taus RNG) on the C1 scene and camera."""
     import numpy as np
@@ -429,9 +587,9 @@ def copy_calibration(torch):
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=12)
+    ap.add_argument("--steps", type=int, default=0, help="timed steps (0: 12, or 3 for c3_batch and c5_fp16_2048)")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["c2_sparse512", "c1_dense128", "c3_batch", "c4_kpn"],
+    ap.add_argument("--workload", choices=["c2_sparse512", "c1_dense128", "c3_batch", "c4_kpn", "c5_fp16_2048"],
                     default="c2_sparse512")
     ap.add_argument("--kpn-crops", type=int, default=0)
     ap.add_argument("--kpn-steps", type=int, default=0)
@@ -441,6 +599,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
+    if args.steps <= 0:
+        args.steps = 3 if args.workload in ("c3_batch", "c5_fp16_2048") else 12
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path tracer is single-GPU here",
               file=sys.stderr)
@@ -455,8 +615,8 @@ def main() -> int:
     from cloudrendering_amd import vpt
 
     vpt.initialize()
-    if args.workload in ("c3_batch", "c4_kpn"):
-        (run_c3 if args.workload == "c3_batch" else run_c4)(args, vpt, torch)
+    if args.workload in ("c3_batch", "c4_kpn", "c5_fp16_2048"):
+        {"c3_batch": run_c3, "c4_kpn": run_c4, "c5_fp16_2048": run_c5}[args.workload](args, vpt, torch)
         vpt.cleanup()
         return 0
     dev = torch.device("cuda:0")
