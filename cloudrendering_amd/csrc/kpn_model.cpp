@@ -25,13 +25,13 @@ constexpr size_t kFragBytes = size_t(kKpnLayers) * 9 * 2 * 64 * 16;  // bf16x8 f
 
 struct KpnModel {
     DevBuf params, grads, m1, m2, wFwd, wBwd, loss;
-    DevBuf act, logits, dzA, dzB, pred;
+    DevBuf act, logits, dzA, dzB, pred, gradWork;
     size_t capPixels = 0;
     int step = 0;
     bool ready = false;
     int cuCount = 256;
     ~KpnModel() {
-        for (DevBuf* b : {&params, &grads, &m1, &m2, &wFwd, &wBwd, &loss, &act, &logits, &dzA, &dzB, &pred})
+        for (DevBuf* b : {&params, &grads, &m1, &m2, &wFwd, &wBwd, &loss, &act, &logits, &dzA, &dzB, &pred, &gradWork})
             b->release();
     }
 };
@@ -191,14 +191,15 @@ int crvpt_kpn_compute_gradients(const float* features, const float* noisy, const
     HIP_TRY(launchKpnZero(k.loss.as<float>(), 1, s));
     HIP_TRY(launchKpnHead(k.logits.as<float>(), noisy, target, k.pred.as<float>(), k.dzA.ptr, k.loss.as<float>(), n,
                           h, w, s));
-    const int blocks = k.cuCount * 4;
+    const int blocks = k.cuCount * 2;  // gradient workgroups: one row of partial sums each
+    HIP_TRY(k.gradWork.ensure(kpnGradWorkFloats(blocks) * sizeof(float)));
     void* dz = k.dzA.ptr;
     void* dzNext = k.dzB.ptr;
     float* gw = k.grads.as<float>();
     float* gb = gw + size_t(kKpnLayers) * kKpnLayerParams;
     for (int l = kKpnLayers - 1; l >= 0; l--) {
         HIP_TRY(launchKpnWeightGrad(actOf(k, l, P), dz, gw + size_t(l) * kKpnLayerParams, gb + size_t(l) * kKpnChannels,
-                                    n, h, w, blocks, s));
+                                    n, h, w, blocks, k.gradWork.as<float>(), s));
         if (l > 0) {
             // gradient of the layer's input, through the ReLU that produced it
             HIP_TRY(launchKpnConv(2, dz, fragOf(k.wBwd, l), nullptr, actOf(k, l, P), dzNext, n, h, w, 1,

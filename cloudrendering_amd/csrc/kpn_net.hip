@@ -126,13 +126,18 @@ __global__ __launch_bounds__(256) void k_conv3x3(const __bf16* __restrict__ in, 
     }
 }
 
-// Weight gradient of one layer: dW[co][ci][tap] += sum over pixels p of dZ[p][co] * X[p + offset(tap)][ci].
+// Weight gradient of one layer: dW[co][ci][tap] = sum over pixels p of dZ[p][co] * X[p + offset(tap)][ci], a GEMM
+// with a 9216-element output and the pixels as its reduction dimension, split over the workgroups (split-K).
 // A workgroup stages per 16x16 tile the output gradient transposed (channel-major, 16-B rows of 8 pixels) and three
 // copies of the input halo shifted by 0, 1 and 2 columns (channel-major rows of 16 pixels), so that every MFMA
 // operand is one aligned 16-B LDS read. Wave w owns the (co, ci) tile pair (w / 2, w % 2) and nine accumulators
-// (one per tap) over all tiles the workgroup visits; they are added to dW with atomics at the end.
+// (one per tap) over all tiles the workgroup visits, and stores them as the workgroup's row of partial sums:
+// part[block][((tap * 4 + wave) * 4 + i) * 64 + lane] (coalesced); k_gradReduce adds the rows in a fixed order.
+constexpr int kWgradPartial = 9 * 4 * 4 * 64;  // floats per workgroup row (= 32 x 32 x 9)
+static_assert(kWgradPartial == kKpnLayerParams, "one partial per weight");
+
 __global__ __launch_bounds__(256) void k_conv3x3WeightGrad(const __bf16* __restrict__ x, const __bf16* __restrict__ dz,
-                                                           float* __restrict__ dW, int N, int H, int W) {
+                                                           float* __restrict__ part, int N, int H, int W) {
     __shared__ __attribute__((aligned(16))) __bf16 dzT[kC * kTile * kTile];            // [co][256 pixels]
     __shared__ __attribute__((aligned(16))) __bf16 xs[3 * kC * kHalo * kTile];         // [kx][ci][18 rows][16 cols]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -190,17 +195,16 @@ __global__ __launch_bounds__(256) void k_conv3x3WeightGrad(const __bf16* __restr
             }
         }
     }
+    float* row = part + size_t(blockIdx.x) * kWgradPartial;
 #pragma unroll
     for (int t = 0; t < 9; t++)
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int co = mt * 16 + kb * 4 + i, ci = nt * 16 + m;
-            atomicAdd(&dW[(co * kC + ci) * 9 + t], acc[t][i]);
-        }
+        for (int i = 0; i < 4; i++) row[((t * 4 + wave) * 4 + i) * 64 + lane] = acc[t][i];
 }
 
-// Bias gradient: db[c] += sum over pixels of dZ[p][c]. Thread (pixel slot, 8-channel group); block reduction in LDS.
-__global__ __launch_bounds__(256) void k_biasGrad(const __bf16* __restrict__ dz, float* __restrict__ db, size_t P) {
+// Bias gradient partials: part[block][c] = sum over the block's pixels of dZ[p][c]. Thread (pixel slot, 8-channel
+// group); block reduction in LDS.
+__global__ __launch_bounds__(256) void k_biasGrad(const __bf16* __restrict__ dz, float* __restrict__ part, size_t P) {
     __shared__ float red[64][kC + 1];
     const int tid = threadIdx.x, c8 = tid & 3, slot = tid >> 2;
     float s[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
@@ -215,8 +219,48 @@ __global__ __launch_bounds__(256) void k_biasGrad(const __bf16* __restrict__ dz,
     if (tid < kC) {
         float t = 0.0f;
         for (int k = 0; k < 64; k++) t += red[k][tid];
-        atomicAdd(&db[tid], t);
+        part[size_t(blockIdx.x) * kC + tid] = t;
     }
+}
+
+// Sums the partial rows of k_conv3x3WeightGrad (wRows rows) and k_biasGrad (bRows rows) in a fixed order and adds
+// them to dW [co][ci][tap] and db [co]. Workgroups 0..143 own 64 weight partial columns each, workgroup 144 the 32
+// bias columns; each thread sums every 4th (weights) or 8th (biases) row with 8 loads in flight, then LDS combines
+// the row groups.
+constexpr int kGradReduceBlocks = kWgradPartial / 64 + 1;
+
+__global__ __launch_bounds__(256) void k_gradReduce(const float* __restrict__ wPart, int wRows,
+                                                    const float* __restrict__ bPart, int bRows, float* __restrict__ dW,
+                                                    float* __restrict__ db) {
+    __shared__ float red[256];
+    const int tid = threadIdx.x;
+    const bool bias = blockIdx.x == kGradReduceBlocks - 1;
+    const int cols = bias ? kC : 64, groups = 256 / cols;
+    const int col = tid % cols, g = tid / cols;
+    const int stride = bias ? kC : kWgradPartial;
+    const float* src = bias ? bPart + col : wPart + blockIdx.x * 64 + col;
+    const int rows = bias ? bRows : wRows;
+    float s8[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    int r = g;
+    for (; r + 7 * groups < rows; r += 8 * groups) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) s8[j] += src[size_t(r + j * groups) * stride];
+    }
+    for (; r < rows; r += groups) s8[0] += src[size_t(r) * stride];
+    red[tid] = ((s8[0] + s8[1]) + (s8[2] + s8[3])) + ((s8[4] + s8[5]) + (s8[6] + s8[7]));
+    __syncthreads();
+    if (g != 0) return;
+    float t = 0.0f;
+    for (int k = 0; k < groups; k++) t += red[k * cols + col];
+    if (bias) {
+        db[col] += t;
+        return;
+    }
+    // partial column e = ((tap * 4 + wave) * 4 + i) * 64 + lane -> (co, ci, tap) as k_conv3x3WeightGrad's lanes hold it
+    const int e = blockIdx.x * 64 + col;
+    const int lane = e & 63, i = (e >> 6) & 3, wave = (e >> 8) & 3, tap = e >> 10;
+    const int co = (wave >> 1) * 16 + (lane >> 4) * 4 + i, ci = (wave & 1) * 16 + (lane & 15);
+    dW[(co * kC + ci) * 9 + tap] += t;
 }
 
 // Input features [N, 15, H, W] f32 -> [N, H, W, 32] bf16 (channels 15..31 zero). Radiance (0-2) and radiance variance
@@ -377,15 +421,23 @@ hipError_t launchKpnConv(int epilogue, const void* in, const void* wsw, const fl
     return hipGetLastError();
 }
 
+size_t kpnGradWorkFloats(int blocks) { return size_t(blocks) * (kWgradPartial + kC); }
+
 hipError_t launchKpnWeightGrad(const void* x, const void* dz, float* dW, float* db, int N, int H, int W, int blocks,
-                               hipStream_t stream) {
+                               float* work, hipStream_t stream) {
     const int tiles = N * ((H + kTile - 1) / kTile) * ((W + kTile - 1) / kTile);
-    hipLaunchKernelGGL(k_conv3x3WeightGrad, dim3(unsigned(tiles < blocks ? tiles : blocks)), dim3(256), 0, stream,
-                       reinterpret_cast<const __bf16*>(x), reinterpret_cast<const __bf16*>(dz), dW, N, H, W);
+    const int wRows = tiles < blocks ? tiles : blocks;
+    float* wPart = work;
+    float* bPart = work + size_t(blocks) * kWgradPartial;
+    hipLaunchKernelGGL(k_conv3x3WeightGrad, dim3(unsigned(wRows)), dim3(256), 0, stream,
+                       reinterpret_cast<const __bf16*>(x), reinterpret_cast<const __bf16*>(dz), wPart, N, H, W);
     const size_t P = size_t(N) * H * W;
     const size_t bb = (P + 63) / 64;
-    hipLaunchKernelGGL(k_biasGrad, dim3(unsigned(bb < size_t(blocks) ? bb : size_t(blocks))), dim3(256), 0, stream,
-                       reinterpret_cast<const __bf16*>(dz), db, P);
+    const int bRows = int(bb < size_t(blocks) ? bb : size_t(blocks));
+    hipLaunchKernelGGL(k_biasGrad, dim3(unsigned(bRows)), dim3(256), 0, stream, reinterpret_cast<const __bf16*>(dz),
+                       bPart, P);
+    hipLaunchKernelGGL(k_gradReduce, dim3(kGradReduceBlocks), dim3(256), 0, stream, wPart, wRows, bPart, bRows, dW,
+                       db);
     return hipGetLastError();
 }
 

@@ -76,8 +76,10 @@ hipError_t launchSetU32(uint32_t* p, uint32_t v, hipStream_t stream);
 // KPN network kernels (kpn_net.hip). epilogue: 0 bias + ReLU -> bf16, 1 bias -> f32, 2 x (mask > 0) -> bf16.
 hipError_t launchKpnConv(int epilogue, const void* in, const void* wsw, const float* bias, const void* mask, void* out,
                          int N, int H, int W, int flip, int blocks, hipStream_t stream);
+// Weight and bias gradients of one layer, added to dW / db; work: kpnGradWorkFloats(blocks) floats of partial sums.
+size_t kpnGradWorkFloats(int blocks);
 hipError_t launchKpnWeightGrad(const void* x, const void* dz, float* dW, float* db, int N, int H, int W, int blocks,
-                               hipStream_t stream);
+                               float* work, hipStream_t stream);
 hipError_t launchKpnPack(const float* features, void* out, int N, int H, int W, hipStream_t stream);
 hipError_t launchKpnHead(const float* logits, const float* noisy, const float* target, float* pred, void* dz,
                          float* loss, int N, int H, int W, hipStream_t stream);
