@@ -317,6 +317,27 @@ def run_c4(args, vpt, torch):
                 return math.exp(math.log(a["seconds"]) + u * (math.log(b["seconds"]) - math.log(a["seconds"])))
         return None
 
+    # Denoiser inference throughput on the full evaluation frame (HIP events on the stream the network runs on):
+    # algorithmic FLOPs per pixel = 2 * 9 * (15 * 32 + 4 * 32 * 32 + 32 * 25) for the six 3x3 convolutions.
+    f, x = kpn.render_inputs(vpt, res, res, 4)
+    f, x = f.unsqueeze(0), x.unsqueeze(0)
+    net.denoise(f, x)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 20
+    e0.record()
+    for _ in range(reps):
+        net.denoise(f, x)
+    e1.record()
+    torch.cuda.synchronize()
+    inf_ms = e0.elapsed_time(e1) / reps
+    conv_flops = 2 * 9 * (15 * 32 + 4 * 32 * 32 + 32 * 25) * res * res
+    inf_tflops = conv_flops / (inf_ms * 1e-3) / 1e12
+    inference = {"pixels": res * res, "ms_per_frame": round(inf_ms, 4), "conv_flops_per_frame": conv_flops,
+                 "roofline": {"bound": "mfma", "achieved": round(inf_tflops, 2), "peak": 2500.0, "unit": "TFLOP/s",
+                              "frac": round(inf_tflops / 2500.0, 5),
+                              "what": "whole denoise call (feature packing, six bf16 MFMA convolutions, softmax "
+                                      "5x5 filter) against the dense bf16 MFMA peak"}}
+
     plain_1e3 = time_to(1e-3)
     for d in denoised:
         d["plain_seconds_at_same_rel_mse"] = time_to(d["rel_mse"])
@@ -332,6 +353,7 @@ def run_c4(args, vpt, torch):
                    "target_spp": C4["target_spp"], "train_steps": steps, "batch": C4["batch"],
                    "eval_res": res, "ref_spp": ref_spp, "estimator": "delta tracking"},
         "pair_generation_s": round(gen_s, 2), "train_s": round(train_s, 2), "final_train_loss": losses[-1],
+        "train_steps_per_s": round(steps / train_s, 1), "inference": inference,
         "kpn": denoised, "plain": plain, "plain_seconds_to_1e-3": plain_1e3,
         "speedup_to_1e-3": (plain_1e3 / kpn_1e3) if (plain_1e3 and kpn_1e3) else None,
     }
@@ -462,8 +484,10 @@ def run_c5(args, vpt, torch):
     vpt.set_collect_stats(False)
     kernel_ms = float(np.mean([p["path_kernel_ms"] for p in per]))
     launches = vpt.get_last_render_stats()["launches"]
-    # Fp16 leaves: 2-byte codes per tap; a descent also reads the leaf's minimum and quantum (8 bytes)
-    b_alg = 2 * c["density_taps"] + 8 * c["supervoxel_reads"] + 48 * c["tree_descents"] + 16 * c["paths"]
+    # Fp16 leaves: 2-byte codes per tap; a leaf-block lookup reads the 4-byte block-index entry and the leaf's
+    # minimum and quantum (8 bytes). The frame's launches trace equal path counts, so per-launch figures are the
+    # frame's divided by the launch count.
+    b_alg = 2 * c["density_taps"] + 8 * c["supervoxel_reads"] + 12 * c["tree_descents"] + 16 * c["paths"]
     achieved = b_alg / (kernel_ms * 1e-3) / 1e9
     keys = ("render_s", "pair_s", "train_s", "inference_s")
     line = {
@@ -484,8 +508,8 @@ def run_c5(args, vpt, torch):
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": load_traffic("c5_fp16_2048"),
                      "kernel": "k_pathPool<RESIDUAL_RATIO_TRACKING, NanoAccess, false>", "launches_per_frame": launches,
-                     "kernel_ms_per_frame": round(kernel_ms, 3), "algorithmic_bytes_per_frame": int(b_alg),
-                     "counters": {k: int(v) for k, v in c.items()}},
+                     "launch_ms": round(kernel_ms / launches, 3), "algorithmic_bytes_per_launch": int(b_alg / launches),
+                     "counters_per_frame": {k: int(v) for k, v in c.items()}},
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
@@ -683,9 +707,9 @@ def main() -> int:
     vpt.set_collect_stats(False)
     taps, svr, paths, desc = (v / max(n_c, 1) for v in (taps, svr, paths, desc))
     # Algorithmic bytes per render: 4-byte FP32 density taps, 4-byte majorant / 8-byte (control, residual) table
-    # reads, 40 bytes per sparse-tree descent (root tile key and child 8 + 8, upper and lower child-mask words 4 + 4,
-    # upper and lower table entries 8 + 8), and the 16-byte radiance record of each path.
-    b_alg = 4 * taps + (4 if dom in (0, 2, 5) else 8) * svr + 40 * desc + 16 * paths
+    # reads, 8 bytes per leaf-block lookup on a leaf-cache miss (the 4-byte entry of the flat 8^3-block index and the
+    # 4-byte tile value of a block without a leaf), and the 16-byte radiance record of each path.
+    b_alg = 4 * taps + (4 if dom in (0, 2, 5) else 8) * svr + 8 * desc + 16 * paths
     achieved = b_alg / (dom_ms * 1e-3) / 1e9
     if c2:
         kernel = MODE_KERNELS[dom]

@@ -183,7 +183,11 @@ struct Renderer {
     // ---- device state ----
     hipStream_t stream = nullptr;
     DevBuf dRaw, dNorm, dNvdb, dMajorant, dRrt, dDecomp, dEnv, dAcc, dPaths, dCounter, dStage, dRays;
-    DevBuf dSvMin, dSvMax, dSvAvg, dDen0, dDen1, dStats, dFeat, dFeatAcc, dSvgf, dKpn;
+    DevBuf dSvMin, dSvMax, dSvAvg, dDen0, dDen1, dStats, dFeat, dFeatAcc, dSvgf, dKpn, dBlockLeaf, dBlockTile;
+    // flat 8^3-block index of the NanoVDB grid (NanoGrid::blockLeaf), valid for volume version blockTableVersion
+    bool blockTableOk = false;
+    uint64_t blockTableVersion = ~0ull;
+    int blockMin[3] = {0, 0, 0}, blockDim[3] = {0, 0, 0};
     bool sceneDirty = true;     // grid contents changed
     bool tablesDirty = true;    // supervoxel tables must be rebuilt
     bool envDirty = true;
@@ -412,6 +416,43 @@ void gridViews(Renderer& r, DenseGrid& dg, NanoGrid& ng, bool& sparse) {
     }
     ng.background = r.nano.background;
     ng.fp16 = r.nano.valueType == 1 ? 1 : 0;
+    const bool table = r.blockTableOk && r.blockTableVersion == r.volumeVersion;
+    ng.blockLeaf = table ? r.dBlockLeaf.as<uint32_t>() : nullptr;
+    ng.blockTile = table ? r.dBlockTile.as<float>() : nullptr;
+    for (int a = 0; a < 3; a++) {
+        ng.blockMin[a] = table ? r.blockMin[a] : 0;
+        ng.blockDim[a] = table ? r.blockDim[a] : 0;
+    }
+}
+
+// Builds the flat 8^3-block index over the grid's index bounding box (at most 2^27 blocks: 1 GiB of index; larger
+// sparse domains keep descending the tree).
+int buildBlockTable(Renderer& r) {
+    r.blockTableOk = false;
+    r.blockTableVersion = r.volumeVersion;
+    auto floorDiv8 = [](int v) { return v >= 0 ? v / 8 : -((-v + 7) / 8); };
+    int b0[3], nb[3];
+    size_t n = 1;
+    for (int a = 0; a < 3; a++) {
+        b0[a] = floorDiv8(r.nano.indexBBoxMin[a]);
+        nb[a] = floorDiv8(r.nano.indexBBoxMax[a]) - b0[a] + 1;
+        if (nb[a] <= 0) return CRVPT_OK;
+        n *= size_t(nb[a]);
+    }
+    if (n > (size_t(1) << 27)) return CRVPT_OK;
+    HIP_TRY(r.dBlockLeaf.ensure(n * sizeof(uint32_t)));
+    HIP_TRY(r.dBlockTile.ensure(n * sizeof(float)));
+    DenseGrid dg;
+    NanoGrid ng;
+    bool sp;
+    gridViews(r, dg, ng, sp);
+    HIP_TRY(launchBlockTable(ng, b0, nb, r.dBlockLeaf.as<uint32_t>(), r.dBlockTile.as<float>(), r.stream));
+    for (int a = 0; a < 3; a++) {
+        r.blockMin[a] = b0[a];
+        r.blockDim[a] = nb[a];
+    }
+    r.blockTableOk = true;
+    return CRVPT_OK;
 }
 
 // Uploads the grid and rebuilds the supervoxel tables the current mode needs.
@@ -430,6 +471,10 @@ int prepareScene(Renderer& r) {
         }
         r.sceneDirty = false;
         r.tablesDirty = true;
+    }
+    if (sparse && r.blockTableVersion != r.volumeVersion) {
+        rc = buildBlockTable(r);
+        if (rc) return rc;
     }
     DenseGrid dg;
     NanoGrid ng;
@@ -690,7 +735,8 @@ int crvpt_cleanup(void) {
         releaseBatch();
         for (DevBuf* b : {&r->dRaw, &r->dNorm, &r->dNvdb, &r->dMajorant, &r->dRrt, &r->dDecomp, &r->dEnv, &r->dAcc,
                           &r->dPaths, &r->dCounter, &r->dStage, &r->dRays, &r->dSvMin, &r->dSvMax, &r->dSvAvg,
-                          &r->dDen0, &r->dDen1, &r->dStats, &r->dFeat, &r->dFeatAcc, &r->dSvgf, &r->dKpn})
+                          &r->dDen0, &r->dDen1, &r->dStats, &r->dFeat, &r->dFeatAcc, &r->dSvgf, &r->dKpn,
+                          &r->dBlockLeaf, &r->dBlockTile})
             b->release();
         (void)hipEventDestroy(r->ev0);
         (void)hipEventDestroy(r->ev1);
@@ -1640,7 +1686,7 @@ namespace crvpt {
 namespace {
 
 struct BatchVolume {
-    DevBuf grid, majorant, rrt, decomp;
+    DevBuf grid, majorant, rrt, decomp, blockLeaf, blockTile;
     DenseGrid dense{};
     NanoGrid nano{};
     SupervoxelGrid S{};
@@ -1649,6 +1695,8 @@ struct BatchVolume {
         majorant.release();
         rrt.release();
         decomp.release();
+        blockLeaf.release();
+        blockTile.release();
     }
 };
 
@@ -1743,6 +1791,14 @@ int snapshotVolume(Renderer& r, BatchVolume& v) {
     if (sp) {
         HIP_TRY(copyDevice(v.grid, r.dNvdb.ptr, r.nvdbHost.size(), r.stream));
         ng.buf = v.grid.as<uint8_t>();
+        if (ng.blockLeaf) {
+            // leaf offsets are relative to the grid buffer, so the index stays valid for the copy
+            const size_t nb = size_t(ng.blockDim[0]) * ng.blockDim[1] * ng.blockDim[2];
+            HIP_TRY(copyDevice(v.blockLeaf, ng.blockLeaf, nb * sizeof(uint32_t), r.stream));
+            HIP_TRY(copyDevice(v.blockTile, ng.blockTile, nb * sizeof(float), r.stream));
+            ng.blockLeaf = v.blockLeaf.as<uint32_t>();
+            ng.blockTile = v.blockTile.as<float>();
+        }
         v.nano = ng;
     } else {
         const size_t n = size_t(r.dims[0]) * r.dims[1] * r.dims[2];

@@ -17,18 +17,18 @@
 
 namespace crvpt {
 
-// Per-lane memo of the last 8^3 block a lookup descended to: the block coordinate (index >> 3), and either the
-// leaf's byte offset or, when the block is covered by a tile (no leaf), the tile value. A path's consecutive
-// lookups are about a voxel apart, so most of them hit the memo and skip the root-upper-lower descent (six dependent
-// loads); a hit returns the same value the descent would. (Also memoising the block's lower node, so that block
-// changes inside it need one child-table read, measured slower on MI355X: the extra registers cost more than the
-// descents they save.)
+// Per-lane memo of the last 8^3 block a lookup read, in two registers (the path kernels run at the register cap,
+// where every extra live value spills): `key` = the block's position in the grid's flat block index, with bit 31
+// set when the block is a tile (no leaf), and `leafOrTile` = the leaf's byte offset, or the tile value's bits. A
+// path's consecutive lookups are about a voxel apart, so most hit the memo; a hit returns the value the tree
+// descent would. Lookups outside the flat index (or of a grid without one) descend the tree every time.
+// (Memoising the block coordinates and the lower node as well measured slower on MI355X: the extra registers cost
+// more than the loads they save.)
 struct LeafCache {
-    int bi = 0x7FFFFFFF, bj = 0, bk = 0;
-    uint32_t leaf = 0;  // 0: the block is a tile of value `tile`
-    float tile = 0.0f;
-    float qMin = 0.0f, qScale = 0.0f;  // Fp16 leaves: the leaf's minimum and quantum
-    uint32_t misses = 0;  // descents (work counter)
+    static constexpr uint32_t kTileBit = 0x80000000u;
+    uint32_t key = 0x40000000u;  // matches no block index (indices are below 2^27)
+    uint32_t leafOrTile = 0;
+    uint32_t misses = 0;  // block lookups and descents (work counter)
     CRVPT_DEV uint32_t descents() const { return misses; }
 };
 
@@ -91,23 +91,35 @@ struct NanoAccess {
     }
 
     CRVPT_DEV float value(int i, int j, int k, LeafCache& c) const {
-        const int bi = i >> 3, bj = j >> 3, bk = k >> 3;  // arithmetic shifts: blocks of negative indices too
-        if (bi != c.bi || bj != c.bj || bk != c.bk) {
-            c.leaf = locateBlock(i, j, k, c.tile);
-            c.misses++;
-            if (g.fp16 && c.leaf != 0u) {
-                c.qMin = f32(c.leaf + 80u);
-                c.qScale = f32(c.leaf + 84u);
+        // block position in the flat index (arithmetic shifts: blocks of negative indices too)
+        const uint32_t ti = uint32_t((i >> 3) - g.blockMin[0]), tj = uint32_t((j >> 3) - g.blockMin[1]),
+                       tk = uint32_t((k >> 3) - g.blockMin[2]);
+        uint32_t leaf;
+        if (ti < uint32_t(g.blockDim[0]) && tj < uint32_t(g.blockDim[1]) && tk < uint32_t(g.blockDim[2])) {
+            const uint32_t b = (tk * uint32_t(g.blockDim[1]) + tj) * uint32_t(g.blockDim[0]) + ti;
+            if ((c.key & ~LeafCache::kTileBit) != b) {
+                // one 4-byte load from the flat block index instead of the root / upper / lower chain of six
+                const uint32_t l = g.blockLeaf[b];
+                c.leafOrTile = l ? l : __float_as_uint(g.blockTile[b]);
+                c.key = l ? b : (b | LeafCache::kTileBit);
+                c.misses++;
             }
-            c.bi = bi;
-            c.bj = bj;
-            c.bk = bk;
+            if (c.key & LeafCache::kTileBit) return __uint_as_float(c.leafOrTile);
+            leaf = c.leafOrTile;
+        } else {
+            float tile = 0.0f;
+            leaf = locateBlock(i, j, k, tile);
+            c.misses++;
+            if (leaf == 0u) return tile;
         }
-        if (c.leaf == 0u) return c.tile;
         const uint32_t n = (uint32_t(i) & 7u) << 6 | (uint32_t(j) & 7u) << 3 | (uint32_t(k) & 7u);
-        // Fp16: code * quantum + minimum with two roundings (pnanovdb_leaf_fp_read_float, PNanoVDB.glsl:1322-1336)
-        if (g.fp16) return mulThenAdd(float(u16(c.leaf + 96u + 2u * n)), c.qScale, c.qMin);
-        return f32(c.leaf + 96u + 4u * n);
+        if (g.fp16) {
+            // code * quantum + minimum with two roundings (pnanovdb_leaf_fp_read_float, PNanoVDB.glsl:1322-1336);
+            // the leaf's minimum (+80) and quantum (+84) are read next to the code
+            const float2 q = *reinterpret_cast<const float2*>(g.buf + leaf + 80u);
+            return mulThenAdd(float(u16(leaf + 96u + 2u * n)), q.y, q.x);
+        }
+        return f32(leaf + 96u + 4u * n);
     }
 
     // Root tile scan (one tile for every domain inside one 4096^3 tile), then upper, lower and leaf.

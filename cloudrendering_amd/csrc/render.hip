@@ -556,6 +556,21 @@ __global__ void k_majorant(Grid grid, int svSize, int gx, int gy, int gz, float*
     maj[si] = m;
 }
 
+// Flat index of the 8^3 blocks of a NanoVDB grid: per block in [b0, b0 + nb) the leaf offset the tree descent
+// returns, or 0 and the tile value. Built once per uploaded grid; the accessor then replaces each descent by one load.
+__global__ void k_blockTable(NanoAccess grid, int bx0, int by0, int bz0, int nbx, int nby, int nbz,
+                             uint32_t* __restrict__ leaf, float* __restrict__ tile) {
+    const size_t n = size_t(nbx) * nby * nbz;
+    for (size_t b = size_t(blockIdx.x) * blockDim.x + threadIdx.x; b < n; b += size_t(gridDim.x) * blockDim.x) {
+        const int bi = int(b % size_t(nbx)), bj = int((b / size_t(nbx)) % size_t(nby)),
+                  bk = int(b / (size_t(nbx) * nby));
+        float t = 0.0f;
+        const uint32_t off = grid.locateBlock((bx0 + bi) * 8, (by0 + bj) * 8, (bz0 + bk) * 8, t);
+        leaf[b] = off;
+        tile[b] = off ? 0.0f : t;
+    }
+}
+
 // Reference tables (SuperVoxelGrid.cpp:101-211, 269-405): min, max and average of the normalised density over
 // S^3 (NEAREST) or the (S+2)^3 halo (other modes), zero outside the grid, summed in the reference's z, y, x order.
 template <class Grid>
@@ -793,6 +808,18 @@ hipError_t launchTransmittance(const SceneParams& P, const DenseGrid* dense, con
         hipLaunchKernelGGL(k_transmittance<DenseAccess>, grid, block, 0, stream, P, DenseAccess{*dense}, S,
                            estimator, o, d, tmax, nRays, nSamples, seed, mean, m2);
     }
+    return hipGetLastError();
+}
+
+hipError_t launchBlockTable(const NanoGrid& nano, const int b0[3], const int nb[3], uint32_t* leaf, float* tile,
+                            hipStream_t stream) {
+    NanoGrid g = nano;
+    g.blockLeaf = nullptr;  // the table is built from the tree itself
+    g.blockTile = nullptr;
+    const size_t n = size_t(nb[0]) * nb[1] * nb[2];
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_blockTable, dim3(unsigned(blocks)), dim3(256), 0, stream, NanoAccess{g}, b0[0], b0[1], b0[2],
+                       nb[0], nb[1], nb[2], leaf, tile);
     return hipGetLastError();
 }
 

@@ -68,6 +68,13 @@ struct NanoGrid {
     float background;
     int32_t indexMin[3];  // index bounding box minimum: texel (0,0,0) of the dense-equivalent grid
     int32_t fp16;         // 1: Fp16 leaves (codes at +96, minimum at +80, quantum at +84); 0: FP32 leaves
+    // Optional flat index of the 8^3 blocks in [blockMin, blockMin + blockDim) (block coordinates, x fastest): per
+    // block the leaf's byte offset, or 0 with the block's tile value in blockTile, exactly what the tree descent
+    // returns. Without one (blockDim 0, null pointers) every lookup descends the tree.
+    const uint32_t* blockLeaf;
+    const float* blockTile;
+    int32_t blockMin[3];
+    int32_t blockDim[3];
 };
 
 // Supervoxel tables. The majorant table drives the build's local-majorant free-flight sampling; the RRT and
