@@ -58,8 +58,8 @@ struct Walker {
     V3 o, w;            // ray origin / direction (world)
     float t, tEnd;      // current distance along the ray, end of the segment
     int ix, iy, iz;     // current cell
-    float nx, ny, nz;   // distance of the next cell plane per axis
-    float dx, dy, dz;   // plane spacing along the ray per axis
+    float nx, ny, nz;   // distance of the next cell plane per axis (the spacing, cell edge / |w_a|, is recomputed
+                        // at each crossing rather than held: the path kernels run at the register cap)
     float mu;           // tentative-collision rate of the current cell, extinction units
     float muC;          // control rate of the current cell (decomposition and residual ratio tracking)
     float ctrl;         // control optical depth walked since the caller last cleared it (residual ratio tracking)
@@ -154,7 +154,7 @@ CRVPT_DEV bool walkerStart(Walker& W, const SceneParams& P, const SupervoxelGrid
     const float tc = tMin + 1e-4f * (tMax - tMin);
     const V3 p = x + w * tc;
     int idx[3];
-    float nxt[3], dlt[3];
+    float nxt[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         const float wa = comp(w, a), pa = comp(p, a), ia = comp(inv, a);
@@ -167,26 +167,31 @@ CRVPT_DEV bool walkerStart(Walker& W, const SceneParams& P, const SupervoxelGrid
         // components.
         if (fabsf(wa) <= 0.000001f) {
             nxt[a] = kInf;
-            dlt[a] = kInf;
         } else if (wa > 0.0f) {
             nxt[a] = tc + (P.boxMin[a] + float(i + 1) * cw - pa) * ia;
-            dlt[a] = cw * ia;
         } else if (wa < 0.0f) {
             nxt[a] = tc + (P.boxMin[a] + float(i) * cw - pa) * ia;
-            dlt[a] = -cw * ia;
         } else {
             nxt[a] = kInf;
-            dlt[a] = kInf;
         }
     }
     W.ix = idx[0]; W.iy = idx[1]; W.iz = idx[2];
     W.nx = nxt[0]; W.ny = nxt[1]; W.nz = nxt[2];
-    W.dx = dlt[0]; W.dy = dlt[1]; W.dz = dlt[2];
     cellRates<MODE>(W, P, S);
     return true;
 }
 
 enum : int { WALK_MOVED = 0, WALK_COLLIDE = 1, WALK_END = 2 };
+
+// Distance along the ray between consecutive cell planes of axis a: the cell edge times |1 / w_a| (the same
+// operations as walkerStart's plane distances, so the walk is unchanged). Only called for an axis whose next plane
+// is finite, i.e. |w_a| > 1e-6.
+template <int MODE>
+CRVPT_DEV float planeSpacing(const SceneParams& P, const SupervoxelGrid& S, float wa, int a) {
+    const float cw = float(cellEdge<MODE>(S)) * rcpf(P.worldToTex[a]);
+    const float ia = rcpf(wa);
+    return wa > 0.0f ? cw * ia : -cw * ia;
+}
 
 // Advances to the next tentative collision (W.t at the collision), the segment end, or, after kMaxCrossings cell
 // crossings without either, returns WALK_MOVED. e is an Exp(1) sample, consumed only when no budget is pending.
@@ -209,13 +214,13 @@ CRVPT_DEV int walkAdvance(Walker& W, const SceneParams& P, const SupervoxelGrid&
         if (tExit >= W.tEnd) return WALK_END;
         if (W.nx <= W.ny && W.nx <= W.nz) {
             W.ix += W.w.x > 0.0f ? 1 : -1;
-            W.nx += W.dx;
+            W.nx += planeSpacing<MODE>(P, S, W.w.x, 0);
         } else if (W.ny <= W.nz) {
             W.iy += W.w.y > 0.0f ? 1 : -1;
-            W.ny += W.dy;
+            W.ny += planeSpacing<MODE>(P, S, W.w.y, 1);
         } else {
             W.iz += W.w.z > 0.0f ? 1 : -1;
-            W.nz += W.dz;
+            W.nz += planeSpacing<MODE>(P, S, W.w.z, 2);
         }
         if ((unsigned)W.ix >= (unsigned)cellGridDim<MODE>(S, 0) ||
             (unsigned)W.iy >= (unsigned)cellGridDim<MODE>(S, 1) ||

@@ -83,6 +83,28 @@ def test_coherent_lookups_through_the_memo(vpt, oracle):
     np.testing.assert_array_equal(got.view(np.uint32), expect.view(np.uint32))
 
 
+def test_tile_blocks_through_the_block_index(vpt):
+    """A grid whose interior holds constant 8^3 blocks (stored as tiles, not leaves) next to leaf blocks: lookups go
+    through the flat block index and the lane memo (tile blocks memoise the tile value), and queries outside the
+    index box descend the tree; every answer equals the voxel written (zero outside)."""
+    print("case: NanoVDB tile blocks 48x40x32", flush=True)
+    rng = np.random.default_rng(21)
+    d = np.zeros((32, 40, 48), np.float32)
+    d[8:24, 8:24, 8:32] = 1.0
+    d[3:7, 30:35, 20:29] = rng.uniform(0.1, 1.0, (4, 5, 9)).astype(np.float32)
+    d[20:29, 0:3, 40:47] = 0.5
+    vpt.set_nanovdb_buffer(vpt.build_nanovdb_from_dense(d, 0.01))
+    k, j, i = np.meshgrid(np.arange(-9, 41), np.arange(-9, 49), np.arange(-9, 57), indexing="ij")
+    q = np.stack([i.ravel(), j.ravel(), k.ravel()], 1).astype(np.int32)
+    walk = np.clip(np.cumsum(rng.integers(-2, 3, size=(30000, 3)), 0) + np.array([24, 20, 16]), -12, 60)
+    q = np.concatenate([q, walk.astype(np.int32)])
+    got = vpt.grid_lookup(q)
+    inside = np.all((q >= 0) & (q < np.array([48, 40, 32])), 1)
+    expect = np.zeros(len(q), np.float32)
+    expect[inside] = d[q[inside, 2], q[inside, 1], q[inside, 0]]
+    np.testing.assert_array_equal(got.view(np.uint32), expect.view(np.uint32))
+
+
 @pytest.mark.parametrize("interp", ["nearest", "stochastic"])
 @pytest.mark.parametrize("sparse", [False, True])
 @pytest.mark.parametrize("sv", [8, 16])
