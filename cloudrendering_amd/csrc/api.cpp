@@ -583,6 +583,7 @@ void fillSceneParams(Renderer& r, int w, int h, SceneParams& P, SupervoxelGrid& 
         P.gridMin[a] = sparse ? float(r.gridMinSparse[a]) : 0.0f;
         P.gridMax[a] = sparse ? float(r.gridMaxSparse[a]) : 1.0f;
         P.worldToTex[a] = float(activeDims(r)[a]) / (P.boxMax[a] - P.boxMin[a]);
+        P.boxToGrid[a] = float(double(P.gridMax[a] - P.gridMin[a]) / double(P.boxMax[a] - P.boxMin[a]));
     }
     P.extinction = extinctionX(r);
     P.albedo = float(r.albedo[0]);

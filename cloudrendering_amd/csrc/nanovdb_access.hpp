@@ -145,10 +145,11 @@ struct NanoAccess {
     }
 
     // sampleCloud for NanoVDB (VptUtils.glsl:515-580, 674-684): raw values, no normalisation; (jx, jy, jz) in
-    // (-0.5, 0.5) is the stochastic sub-voxel jitter, ignored by the other filters.
+    // (-0.5, 0.5) is the stochastic sub-voxel jitter, ignored by the other filters. The box -> grid map
+    // (pos - boxMin) / (boxMax - boxMin) * (gridMax - gridMin) + gridMin is one multiply by the precomputed scale
+    // (as a shader's division by a uniform compiles to a reciprocal multiply) instead of three IEEE divisions per tap.
     CRVPT_DEV float lookupWorldJ(const SceneParams& P, V3 pos, float jx, float jy, float jz, LeafCache& cache) const {
-        V3 coord = (pos - v3(P.boxMin)) / (v3(P.boxMax) - v3(P.boxMin));
-        coord = coord * (v3(P.gridMax) - v3(P.gridMin)) + v3(P.gridMin);
+        const V3 coord = (pos - v3(P.boxMin)) * v3(P.boxToGrid) + v3(P.gridMin);
         V3 idx = worldToIndex(coord);
         if (P.interpolation == INTERP_TRILINEAR) {
             idx = idx - v3(0.5f);

@@ -100,6 +100,7 @@ struct SceneParams {
     float worldToTex[3];    // grid dim / (boxMax - boxMin): texel coordinates of a world position
     float gridMin[3];       // sparse grids: NanoVDB world bounding box the box maps onto (sampleCloud)
     float gridMax[3];
+    float boxToGrid[3];     // (gridMax - gridMin) / (boxMax - boxMin): the box -> grid scale, one multiply per tap
     float extinction;       // extinction.x = majorant of the reference trackers
     float albedo;           // scatteringAlbedo.x
     float extinctionRGB[3]; // extinction = extinctionBase * extinctionScale (spectral trackers)
