@@ -416,6 +416,26 @@ void gridViews(Renderer& r, DenseGrid& dg, NanoGrid& ng, bool& sparse) {
     }
     ng.background = r.nano.background;
     ng.fp16 = r.nano.valueType == 1 ? 1 : 0;
+    // composed tap map: idx = invMat * ((p - boxMin) * scale + gridMin - vec), scale = (gridMax - gridMin) /
+    // (boxMax - boxMin), with the float values the path kernels see for the box and the grid bounds
+    double scale[3], shift[3];
+    for (int a = 0; a < 3; a++) {
+        const float bMin = float(r.boxMinSparse[a]), bMax = float(r.boxMaxSparse[a]);
+        const float gMin = float(r.gridMinSparse[a]), gMax = float(r.gridMaxSparse[a]);
+        scale[a] = double(gMax - gMin) / double(bMax - bMin);
+        shift[a] = double(gMin) - double(r.nano.vecF[a]) - double(bMin) * scale[a];
+    }
+    ng.tapDiag = 1;
+    for (int row = 0; row < 3; row++) {
+        double off = 0.0;
+        for (int c = 0; c < 3; c++) {
+            const double m = double(r.nano.invMatF[3 * row + c]);
+            ng.tapMat[3 * row + c] = float(m * scale[c]);
+            off += m * shift[c];
+            if (row != c && m != 0.0) ng.tapDiag = 0;
+        }
+        ng.tapOff[row] = float(off);
+    }
     const bool table = r.blockTableOk && r.blockTableVersion == r.volumeVersion;
     ng.blockLeaf = table ? r.dBlockLeaf.as<uint32_t>() : nullptr;
     ng.blockTile = table ? r.dBlockTile.as<float>() : nullptr;
@@ -583,7 +603,6 @@ void fillSceneParams(Renderer& r, int w, int h, SceneParams& P, SupervoxelGrid& 
         P.gridMin[a] = sparse ? float(r.gridMinSparse[a]) : 0.0f;
         P.gridMax[a] = sparse ? float(r.gridMaxSparse[a]) : 1.0f;
         P.worldToTex[a] = float(activeDims(r)[a]) / (P.boxMax[a] - P.boxMin[a]);
-        P.boxToGrid[a] = float(double(P.gridMax[a] - P.gridMin[a]) / double(P.boxMax[a] - P.boxMin[a]));
     }
     P.extinction = extinctionX(r);
     P.albedo = float(r.albedo[0]);

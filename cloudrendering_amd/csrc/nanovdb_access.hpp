@@ -146,11 +146,19 @@ struct NanoAccess {
 
     // sampleCloud for NanoVDB (VptUtils.glsl:515-580, 674-684): raw values, no normalisation; (jx, jy, jz) in
     // (-0.5, 0.5) is the stochastic sub-voxel jitter, ignored by the other filters. The box -> grid map
-    // (pos - boxMin) / (boxMax - boxMin) * (gridMax - gridMin) + gridMin is one multiply by the precomputed scale
-    // (as a shader's division by a uniform compiles to a reciprocal multiply) instead of three IEEE divisions per tap.
+    // (pos - boxMin) / (boxMax - boxMin) * (gridMax - gridMin) + gridMin followed by world_to_indexf is one
+    // precomposed affine map (three FMAs for axis-aligned voxels) instead of three divisions and twelve
+    // multiply-adds per tap; the index coordinate differs from the step-by-step evaluation only in rounding.
     CRVPT_DEV float lookupWorldJ(const SceneParams& P, V3 pos, float jx, float jy, float jz, LeafCache& cache) const {
-        const V3 coord = (pos - v3(P.boxMin)) * v3(P.boxToGrid) + v3(P.gridMin);
-        V3 idx = worldToIndex(coord);
+        V3 idx;
+        if (g.tapDiag) {
+            idx = v3(pos.x * g.tapMat[0] + g.tapOff[0], pos.y * g.tapMat[4] + g.tapOff[1],
+                     pos.z * g.tapMat[8] + g.tapOff[2]);
+        } else {
+            idx = v3(pos.x * g.tapMat[0] + pos.y * g.tapMat[1] + pos.z * g.tapMat[2] + g.tapOff[0],
+                     pos.x * g.tapMat[3] + pos.y * g.tapMat[4] + pos.z * g.tapMat[5] + g.tapOff[1],
+                     pos.x * g.tapMat[6] + pos.y * g.tapMat[7] + pos.z * g.tapMat[8] + g.tapOff[2]);
+        }
         if (P.interpolation == INTERP_TRILINEAR) {
             idx = idx - v3(0.5f);
             const float fx = floorf(idx.x), fy = floorf(idx.y), fz = floorf(idx.z);

@@ -75,6 +75,12 @@ struct NanoGrid {
     const float* blockTile;
     int32_t blockMin[3];
     int32_t blockDim[3];
+    // Render-box world position -> index coordinate in one affine map, tapMat * p + tapOff: the box -> grid map of
+    // sampleCloud composed with world_to_indexf (composed in double, rounded once). tapDiag: the map has no
+    // off-diagonal terms (grids whose voxels are axis-aligned boxes), so a tap needs three FMAs.
+    float tapMat[9];
+    float tapOff[3];
+    int32_t tapDiag;
 };
 
 // Supervoxel tables. The majorant table drives the build's local-majorant free-flight sampling; the RRT and
@@ -100,7 +106,6 @@ struct SceneParams {
     float worldToTex[3];    // grid dim / (boxMax - boxMin): texel coordinates of a world position
     float gridMin[3];       // sparse grids: NanoVDB world bounding box the box maps onto (sampleCloud)
     float gridMax[3];
-    float boxToGrid[3];     // (gridMax - gridMin) / (boxMax - boxMin): the box -> grid scale, one multiply per tap
     float extinction;       // extinction.x = majorant of the reference trackers
     float albedo;           // scatteringAlbedo.x
     float extinctionRGB[3]; // extinction = extinctionBase * extinctionScale (spectral trackers)
