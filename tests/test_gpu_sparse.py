@@ -40,12 +40,15 @@ def test_device_fp16_lookups_bit_exact_vs_reference(vpt, path):
     np.testing.assert_array_equal(got.view(np.uint32), g["values"].view(np.uint32))
 
 
-def test_fp16_render_equals_fp32(vpt, oracle):
-    """The Fp16 grid quantises each leaf to 1/65535 of its range: renders agree with the FP32 grid's statistically."""
-    print("case: Fp16 vs FP32 sparse render", flush=True)
+@pytest.mark.parametrize("mode", [0, 4, 3])
+def test_fp16_render_equals_fp32(vpt, oracle, mode):
+    """The Fp16 grid quantises each leaf to 1/65535 of its range: renders agree with the FP32 grid's statistically
+    (delta, residual ratio and decomposition tracking; residual ratio tracking on Fp16 leaves is the large
+    production-frame workload's estimator)."""
+    print(f"case: Fp16 vs FP32 sparse render mode={mode}", flush=True)
     d = oracle.fbm_cloud(32, 32, 32, seed=17)
     vpt.set_density_field(d)
-    vpt.set_vpt_mode(0)
+    vpt.set_vpt_mode(mode)
     vpt.set_grid_interpolation_type("stochastic")
     vpt.set_extinction_scale(200.0)
     vpt.set_scattering_albedo([0.9, 0.9, 0.9])
