@@ -20,7 +20,7 @@
  * The equirectangular environment image is sampled uniformly over the sphere (pdf 1/4pi) for next-event estimation:
  * the reference's octahedral importance mips are outside the hot path.
  *
- * Part 2 is an FP64 exact optical-depth integrator used as the closed-form judge of transmittance estimators.
+ * Part 2 is an FP64 exact optical-depth integrator used as the closed-form check of transmittance estimators.
  */
 #include "vpt_oracle.h"
 

@@ -9,7 +9,7 @@
  *      with the reference's own hybrid-Taus RNG, seeding and running-mean accumulation, so per-path arithmetic
  *      follows the reference shaders statement by statement (citations at each function in vpt_oracle.cpp).
  *   2. An FP64 exact optical-depth integrator (3D-DDA over voxels, Gauss-Legendre on trilinear cells) used as the
- *      closed-form judge of every transmittance estimator.
+ *      closed-form check of every transmittance estimator.
  */
 #ifndef CRVPT_ORACLE_H
 #define CRVPT_ORACLE_H

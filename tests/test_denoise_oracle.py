@@ -1,4 +1,4 @@
-"""CPU pins of the denoiser restatement (oracle/denoise_ref.py) before it judges the gfx950 kernels.
+"""CPU pins of the denoiser restatement (oracle/denoise_ref.py) before it checks the gfx950 kernels.
 
 EAW: with only the colour weight (the reference's constant exp(-1/phiColor)) each iteration is a plain dilated 5x5
 Gaussian (weights exp(-(x^2+y^2)/2)) with clamped edges, so it must equal scipy's correlation; zero iterations are

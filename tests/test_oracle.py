@@ -1,4 +1,4 @@
-"""CPU checks that pin the oracle (oracle/vpt_oracle.cpp) before it judges the GPU path.
+"""CPU checks that pin the oracle (oracle/vpt_oracle.cpp) before it checks the GPU path.
 
 Each test fixes a property the mathematics or the reference's own tests determine, chosen so that a plausible
 restatement mistake (wrong constant, flipped axis, dropped term) fails it:
