@@ -14,9 +14,6 @@ namespace crvpt {
 
 constexpr float kInf = 3.0e38f;
 
-#ifndef CRVPT_PATHPOOL_WAVES
-#define CRVPT_PATHPOOL_WAVES 4  // waves per SIMD the path pool is compiled for (caps registers at 128)
-#endif
 
 // ------------------------------------------------------------------------------------------------------------
 // Cell walker
@@ -32,13 +29,23 @@ constexpr float kInf = 3.0e38f;
 //    and the control transmittance exp(-integral of mu_c) accumulated along the way (ResidualRatioTracking.glsl:
 //    33-76).
 // An Exp(1) optical budget carries over cell boundaries (the process is memoryless, so this samples the same
-// collision distribution as the reference's fresh draw per supervoxel). One call crosses up to kMaxCrossings
+// collision distribution as the reference's fresh draw per supervoxel). One call crosses up to maxCrossings<MODE>()
 // cells before it returns, so empty space costs cell reads but no random-number blocks.
 // ------------------------------------------------------------------------------------------------------------
-#ifndef CRVPT_MAX_CROSSINGS
-#define CRVPT_MAX_CROSSINGS 2
-#endif
-constexpr int kMaxCrossings = CRVPT_MAX_CROSSINGS;
+
+// Per-estimator launch tuning, measured on MI355X (1024^2 renders of the 512^3 sparse grid, 64-256 spp): cell
+// crossings per walker call (more crossings per step means fewer steps but more lanes idle while the others
+// cross), and the waves per SIMD the pool kernels are compiled for (128 registers per lane at 4 waves, 102 at 5).
+// Residual ratio and next-event tracking carry the most per-lane state and run best at 2 crossings, 4 waves.
+template <int MODE>
+constexpr int maxCrossings() {
+    return (MODE == VPT_RESIDUAL_RATIO_TRACKING || MODE == VPT_NEXT_EVENT_TRACKING ||
+            MODE == VPT_NEXT_EVENT_TRACKING_SPECTRAL) ? 2 : 3;
+}
+template <int MODE>
+constexpr int poolWaves() {
+    return (MODE == VPT_DELTA_TRACKING || MODE == VPT_RATIO_TRACKING) ? 5 : 4;
+}
 
 template <int MODE>
 constexpr bool walksReferenceTable() {
@@ -193,7 +200,7 @@ CRVPT_DEV float planeSpacing(const SceneParams& P, const SupervoxelGrid& S, floa
     return wa > 0.0f ? cw * ia : -cw * ia;
 }
 
-// Advances to the next tentative collision (W.t at the collision), the segment end, or, after kMaxCrossings cell
+// Advances to the next tentative collision (W.t at the collision), the segment end, or, after maxCrossings cell
 // crossings without either, returns WALK_MOVED. e is an Exp(1) sample, consumed only when no budget is pending.
 template <int MODE>
 CRVPT_DEV int walkAdvance(Walker& W, const SceneParams& P, const SupervoxelGrid& S, float e) {
@@ -229,7 +236,7 @@ CRVPT_DEV int walkAdvance(Walker& W, const SceneParams& P, const SupervoxelGrid&
             return WALK_END;
         }
         cellRates<MODE>(W, P, S);
-        if (++crossings >= kMaxCrossings) return WALK_MOVED;
+        if (++crossings >= maxCrossings<MODE>()) return WALK_MOVED;
     }
 }
 

@@ -58,7 +58,7 @@ CRVPT_DEV bool claimToSlot(const SceneParams& P, uint32_t c, uint32_t& slot) {
 // STATS: instrumented variant that keeps the per-lane work counters and adds them to `stats` at exit; in the timed
 // variant the counters are dead code and take no registers.
 template <int MODE, class Grid, bool STATS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CRVPT_PATHPOOL_WAVES))) void k_pathPool(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(poolWaves<MODE>()))) void k_pathPool(
         SceneParams P, Grid grid, SupervoxelGrid S, float4* __restrict__ out, uint32_t* __restrict__ counter,
         int stepsPerRefill, unsigned long long* __restrict__ stats, float4* __restrict__ feat) {
     const int lane = __lane_id();
