@@ -87,7 +87,7 @@ def setup_c2(vpt):
     vpt.set_use_builtin_environment_map("Default")
     vpt.set_use_lights(False)
     vpt.set_use_sparse_grid(True)
-    vpt.set_launch_config(16, 0, 16)
+    vpt.set_launch_config(16, 0, 0)
     vpt.set_super_voxel_size(16)
     vpt.set_max_bounces(C2["max_bounces"])
 

@@ -164,7 +164,7 @@ struct Renderer {
     bool useMajorantGrid = true;
     int svSizeSetting = 0;
     int blocksSetting = 0;
-    int stepsPerRefill = 16;
+    int stepsPerRefill = 0;  // walker steps between refills; 0: the estimator's default (refillInterval)
     bool collectStats = false;
 
     // ---- volume ----
@@ -659,6 +659,15 @@ void fillSceneParams(Renderer& r, int w, int h, SceneParams& P, SupervoxelGrid& 
     }
 }
 
+// Walker steps between refill ballots: the setting, else 24 for residual ratio and next-event tracking (their
+// segment ends and scatter events are rarer per step) and 16 for the other estimators (measured on MI355X, 1024^2
+// renders of the 512^3 sparse grid).
+int refillInterval(const Renderer& r) {
+    if (r.stepsPerRefill > 0) return r.stepsPerRefill;
+    return (r.mode == VPT_RESIDUAL_RATIO_TRACKING || r.mode == VPT_NEXT_EVENT_TRACKING ||
+            r.mode == VPT_NEXT_EVENT_TRACKING_SPECTRAL) ? 24 : 16;
+}
+
 int launchBlocks(Renderer& r) {
     if (r.blocksSetting > 0) return r.blocksSetting;
     return r.cuCount * occupancyBlocksPerCU(r.mode, sparseActive(r));
@@ -822,7 +831,7 @@ int crvpt_render_frame(float* out, int channels, int height, int width, int fram
     HIP_TRY(r.dPaths.ensure(size_t(npix) * framesPerLaunch * sizeof(float4)));
     LaunchConfig cfg;
     cfg.numBlocks = uint32_t(launchBlocks(r));
-    cfg.stepsPerRefill = r.stepsPerRefill;
+    cfg.stepsPerRefill = refillInterval(r);
     float4* feat = nullptr;
     FeatureAcc facc{};
     if (features) {
@@ -1691,7 +1700,7 @@ int crvpt_set_launch_config(int supervoxel_size, int blocks, int steps_per_refil
         r.tablesDirty = true;
     }
     r.blocksSetting = blocks;
-    r.stepsPerRefill = steps_per_refill > 0 ? steps_per_refill : 16;
+    r.stepsPerRefill = steps_per_refill;
     return CRVPT_OK;
 }
 

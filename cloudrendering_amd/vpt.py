@@ -467,7 +467,7 @@ def set_use_majorant_grid(use_majorant_grid: bool) -> None:
     call("crvpt_set_use_majorant_grid", int(bool(use_majorant_grid)))
 
 
-def set_launch_config(supervoxel_size: int = 0, blocks: int = 0, steps_per_refill: int = 16) -> None:
+def set_launch_config(supervoxel_size: int = 0, blocks: int = 0, steps_per_refill: int = 0) -> None:
     call("crvpt_set_launch_config", int(supervoxel_size), int(blocks), int(steps_per_refill))
 
 

@@ -162,7 +162,8 @@ int crvpt_get_supervoxel_tables(int* supervoxel_size, int grid_size[3], float* d
 /* No reference counterpart: performance controls of the gfx950 path pool. use_majorant_grid: local majorants
  * from the supervoxel grid (1, default) or the reference's global majorant (0); supervoxel_size: edge in voxels of
  * the majorant grid (default 8 dense, 16 sparse when 0); blocks: persistent workgroups (0 = occupancy based);
- * steps_per_refill: walker events between refill ballots. */
+ * steps_per_refill: walker events between refill ballots (0 = the estimator's default: 24 for residual ratio and
+ * next-event tracking, 16 for the others). */
 int crvpt_set_use_majorant_grid(int use_majorant_grid);
 int crvpt_set_launch_config(int supervoxel_size, int blocks, int steps_per_refill);
 
