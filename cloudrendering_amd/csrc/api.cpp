@@ -2117,7 +2117,7 @@ int crvpt_batch_autotune(int spp_divisor, int32_t config[4]) {
         return CRVPT_OK;
     };
     int rc = CRVPT_OK;
-    for (int steps : {2, 4, 8, 16, 32})
+    for (int steps : {2, 4, 8, 16, 24, 32})
         if (!rc && steps != cur.stepsPerRefill) rc = tryConfig(BatchLaunch{cur.numBlocks, cur.blockSize, steps,
                                                                            cur.samplesPerUnit});
     // Samples per unit whose full-size batch keeps at most 2^28 unit records (4 GiB).
