@@ -183,7 +183,7 @@ struct Renderer {
     // ---- device state ----
     hipStream_t stream = nullptr;
     DevBuf dRaw, dNorm, dNvdb, dMajorant, dRrt, dDecomp, dEnv, dAcc, dPaths, dCounter, dStage, dRays;
-    DevBuf dSvMin, dSvMax, dSvAvg, dDen0, dDen1, dStats, dFeat, dFeatAcc, dSvgf, dKpn, dBlockLeaf, dBlockTile;
+    DevBuf dSvMin, dSvMax, dSvAvg, dDen0, dDen1, dStats, dFeat, dFeatAcc, dSvgf, dKpn, dBlockLeaf, dBlockTile, dBlockBad;
     // flat 8^3-block index of the NanoVDB grid (NanoGrid::blockLeaf), valid for volume version blockTableVersion
     bool blockTableOk = false;
     uint64_t blockTableVersion = ~0ull;
@@ -407,7 +407,7 @@ void gridViews(Renderer& r, DenseGrid& dg, NanoGrid& ng, bool& sparse) {
     for (int a = 0; a < 3; a++) dg.dim[a] = r.dims[a];
     dg.borderValue = (0.0f - r.vmin) / (r.vmax - r.vmin);
     ng.buf = r.dNvdb.as<uint8_t>();
-    ng.rootOffset = uint32_t(r.nano.rootOffset);
+    ng.rootOffset = r.nano.rootOffset;
     ng.tileCount = r.nano.tileCount;
     for (int i = 0; i < 9; i++) ng.invMat[i] = r.nano.invMatF[i];
     for (int i = 0; i < 3; i++) {
@@ -416,6 +416,7 @@ void gridViews(Renderer& r, DenseGrid& dg, NanoGrid& ng, bool& sparse) {
     }
     ng.background = r.nano.background;
     ng.fp16 = r.nano.valueType == 1 ? 1 : 0;
+    ng.wide = r.nvdbHost.size() >= (size_t(1) << 32) ? 1 : 0;
     // composed tap map: idx = invMat * ((p - boxMin) * scale + gridMin - vec), scale = (gridMax - gridMin) /
     // (boxMax - boxMin), with the float values the path kernels see for the box and the grid bounds
     double scale[3], shift[3];
@@ -446,7 +447,7 @@ void gridViews(Renderer& r, DenseGrid& dg, NanoGrid& ng, bool& sparse) {
 }
 
 // Builds the flat 8^3-block index over the grid's index bounding box (at most 2^27 blocks: 1 GiB of index; larger
-// sparse domains keep descending the tree).
+// sparse domains, and grids whose leaves lie beyond 128 GiB, keep descending the tree).
 int buildBlockTable(Renderer& r) {
     r.blockTableOk = false;
     r.blockTableVersion = r.volumeVersion;
@@ -466,7 +467,14 @@ int buildBlockTable(Renderer& r) {
     NanoGrid ng;
     bool sp;
     gridViews(r, dg, ng, sp);
-    HIP_TRY(launchBlockTable(ng, b0, nb, r.dBlockLeaf.as<uint32_t>(), r.dBlockTile.as<float>(), r.stream));
+    HIP_TRY(r.dBlockBad.ensure(sizeof(uint32_t)));
+    HIP_TRY(hipMemsetAsync(r.dBlockBad.ptr, 0, sizeof(uint32_t), r.stream));
+    HIP_TRY(launchBlockTable(ng, b0, nb, r.dBlockLeaf.as<uint32_t>(), r.dBlockTile.as<float>(),
+                             r.dBlockBad.as<uint32_t>(), r.stream));
+    uint32_t bad = 0;
+    HIP_TRY(hipMemcpyAsync(&bad, r.dBlockBad.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost, r.stream));
+    HIP_TRY(hipStreamSynchronize(r.stream));
+    if (bad) return CRVPT_OK;  // offsets the index cannot encode: lookups descend the tree
     for (int a = 0; a < 3; a++) {
         r.blockMin[a] = b0[a];
         r.blockDim[a] = nb[a];
@@ -765,7 +773,7 @@ int crvpt_cleanup(void) {
         for (DevBuf* b : {&r->dRaw, &r->dNorm, &r->dNvdb, &r->dMajorant, &r->dRrt, &r->dDecomp, &r->dEnv, &r->dAcc,
                           &r->dPaths, &r->dCounter, &r->dStage, &r->dRays, &r->dSvMin, &r->dSvMax, &r->dSvAvg,
                           &r->dDen0, &r->dDen1, &r->dStats, &r->dFeat, &r->dFeatAcc, &r->dSvgf, &r->dKpn,
-                          &r->dBlockLeaf, &r->dBlockTile})
+                          &r->dBlockLeaf, &r->dBlockTile, &r->dBlockBad})
             b->release();
         (void)hipEventDestroy(r->ev0);
         (void)hipEventDestroy(r->ev1);

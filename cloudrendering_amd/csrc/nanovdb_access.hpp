@@ -13,13 +13,15 @@
 // Root keys pack the 4096^3 tile coordinate as (k >> 12) | (j >> 12) << 21 | (i >> 12) << 42.
 #pragma once
 
+#include <type_traits>
+
 #include "device_common.hpp"
 
 namespace crvpt {
 
 // Per-lane memo of the last 8^3 block a lookup read, in two registers (the path kernels run at the register cap,
 // where every extra live value spills): `key` = the block's position in the grid's flat block index, with bit 31
-// set when the block is a tile (no leaf), and `leafOrTile` = the leaf's byte offset, or the tile value's bits. A
+// set when the block is a tile (no leaf), and `leafOrTile` = the leaf's byte offset / 32, or the tile value's bits. A
 // path's consecutive lookups are about a voxel apart, so most hit the memo; a hit returns the value the tree
 // descent would. Lookups outside the flat index (or of a grid without one) descend the tree every time.
 // (Memoising the block coordinates and the lower node as well measured slower on MI355X: the extra registers cost
@@ -27,7 +29,7 @@ namespace crvpt {
 struct LeafCache {
     static constexpr uint32_t kTileBit = 0x80000000u;
     uint32_t key = 0x40000000u;  // matches no block index (indices are below 2^27)
-    uint32_t leafOrTile = 0;
+    uint32_t leafOrTile = 0;  // leaf byte offset / 32 (nodes are 32-byte aligned), or the tile value's bits
     uint32_t misses = 0;  // block lookups and descents (work counter)
     CRVPT_DEV uint32_t descents() const { return misses; }
 };
@@ -38,34 +40,38 @@ CRVPT_DEV float mulThenAdd(float a, float b, float c) {
     return a * b + c;
 }
 
-struct NanoAccess {
+// WIDE: 64-bit byte offsets, for grids of 4 GiB and more (up to the 128 GiB the block index encodes); the path
+// kernels run the 32-bit variant on smaller grids, where the 64-bit arithmetic would cost registers in the hot loop.
+template <bool WIDE>
+struct NanoAccessT {
+    using Off = typename std::conditional<WIDE, uint64_t, uint32_t>::type;
     NanoGrid g;
     using Cache = LeafCache;
 
-    CRVPT_DEV uint32_t u32(uint32_t off) const { return *reinterpret_cast<const uint32_t*>(g.buf + off); }
-    CRVPT_DEV float f32(uint32_t off) const { return *reinterpret_cast<const float*>(g.buf + off); }
-    CRVPT_DEV uint32_t u16(uint32_t off) const { return *reinterpret_cast<const uint16_t*>(g.buf + off); }
-    CRVPT_DEV uint64_t u64(uint32_t off) const {
+    CRVPT_DEV uint32_t u32(Off off) const { return *reinterpret_cast<const uint32_t*>(g.buf + off); }
+    CRVPT_DEV float f32(Off off) const { return *reinterpret_cast<const float*>(g.buf + off); }
+    CRVPT_DEV uint32_t u16(Off off) const { return *reinterpret_cast<const uint16_t*>(g.buf + off); }
+    CRVPT_DEV uint64_t u64(Off off) const {
         const uint2 v = *reinterpret_cast<const uint2*>(g.buf + off);
         return uint64_t(v.x) | (uint64_t(v.y) << 32);
     }
 
     // Child of lower node `lower` holding (i, j, k): the leaf offset, or 0 with the block's tile value in `tile`.
-    CRVPT_DEV uint32_t leafInLower(uint32_t lower, int i, int j, int k, float& tile) const {
+    CRVPT_DEV Off leafInLower(Off lower, int i, int j, int k, float& tile) const {
         const uint32_t nl = ((uint32_t(i) & 127u) >> 3) << 8 | ((uint32_t(j) & 127u) >> 3) << 4 |
                             ((uint32_t(k) & 127u) >> 3);
         if (((u32(lower + 544u + 4u * (nl >> 5)) >> (nl & 31u)) & 1u) == 0u) {
             tile = f32(lower + 1088u + 8u * nl);
             return 0u;
         }
-        return lower + uint32_t(u64(lower + 1088u + 8u * nl));
+        return lower + Off(u64(lower + 1088u + 8u * nl));
     }
 
     // Descends to the 8^3 block holding (i, j, k): returns the leaf offset, or 0 with the tile value in `tile`.
-    CRVPT_DEV uint32_t locateBlock(int i, int j, int k, float& tile) const {
+    CRVPT_DEV Off locateBlock(int i, int j, int k, float& tile) const {
         const uint64_t key = uint64_t(uint32_t(k) >> 12) | (uint64_t(uint32_t(j) >> 12) << 21) |
                              (uint64_t(uint32_t(i) >> 12) << 42);
-        uint32_t rt = g.rootOffset + 64u;
+        Off rt = Off(g.rootOffset) + 64u;
         uint32_t t = 0;
         for (; t < g.tileCount; t++, rt += 32u) {
             if (u64(rt) == key) break;
@@ -79,14 +85,14 @@ struct NanoAccess {
             tile = f32(rt + 20u);
             return 0u;
         }
-        const uint32_t upper = g.rootOffset + uint32_t(child);
+        const Off upper = Off(g.rootOffset) + Off(child);
         const uint32_t nu = ((uint32_t(i) & 4095u) >> 7) << 10 | ((uint32_t(j) & 4095u) >> 7) << 5 |
                             ((uint32_t(k) & 4095u) >> 7);
         if (((u32(upper + 4128u + 4u * (nu >> 5)) >> (nu & 31u)) & 1u) == 0u) {
             tile = f32(upper + 8256u + 8u * nu);
             return 0u;
         }
-        const uint32_t lower = upper + uint32_t(u64(upper + 8256u + 8u * nu));
+        const Off lower = upper + Off(u64(upper + 8256u + 8u * nu));
         return leafInLower(lower, i, j, k, tile);
     }
 
@@ -94,18 +100,19 @@ struct NanoAccess {
         // block position in the flat index (arithmetic shifts: blocks of negative indices too)
         const uint32_t ti = uint32_t((i >> 3) - g.blockMin[0]), tj = uint32_t((j >> 3) - g.blockMin[1]),
                        tk = uint32_t((k >> 3) - g.blockMin[2]);
-        uint32_t leaf;
+        Off leaf;
         if (ti < uint32_t(g.blockDim[0]) && tj < uint32_t(g.blockDim[1]) && tk < uint32_t(g.blockDim[2])) {
             const uint32_t b = (tk * uint32_t(g.blockDim[1]) + tj) * uint32_t(g.blockDim[0]) + ti;
             if ((c.key & ~LeafCache::kTileBit) != b) {
                 // one 4-byte load from the flat block index instead of the root / upper / lower chain of six
+                // (entries hold leaf offset / 32)
                 const uint32_t l = g.blockLeaf[b];
                 c.leafOrTile = l ? l : __float_as_uint(g.blockTile[b]);
                 c.key = l ? b : (b | LeafCache::kTileBit);
                 c.misses++;
             }
             if (c.key & LeafCache::kTileBit) return __uint_as_float(c.leafOrTile);
-            leaf = c.leafOrTile;
+            leaf = Off(c.leafOrTile) << 5;
         } else {
             float tile = 0.0f;
             leaf = locateBlock(i, j, k, tile);
@@ -197,5 +204,7 @@ struct NanoAccess {
         return lookupWorldJ(P, pos, jx, jy, jz);
     }
 };
+using NanoAccess = NanoAccessT<true>;        // any grid (lookups, tables, transmittance, batched queue)
+using NanoAccessNarrow = NanoAccessT<false>;  // grids below 4 GiB (the single-frame path kernels)
 
 }  // namespace crvpt

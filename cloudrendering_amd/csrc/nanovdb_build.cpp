@@ -190,8 +190,8 @@ bool parseNanoVdb(const uint8_t* buf, size_t size, NanoInfo& info, std::string& 
     }
     info.tileCount = get<uint32_t>(buf, info.rootOffset + 24);
     info.background = get<float>(buf, info.rootOffset + 28);
-    if (info.gridSize >= (uint64_t(1) << 32)) {
-        err = "NanoVDB buffer: grids of 4 GiB or more need 64-bit offsets";
+    if (info.rootOffset + kRootSize + uint64_t(kTileSize) * info.tileCount > size) {
+        err = "NanoVDB buffer: root tiles outside the buffer";
         return false;
     }
     return true;

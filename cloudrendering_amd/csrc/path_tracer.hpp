@@ -385,7 +385,10 @@ CRVPT_DEV bool startPath(PathState& st, Walker& W, const SceneParams& P, const S
 // getCloudFiniteDifference (VptUtils.glsl:726-748) at p for the normal feature map: central differences of the
 // normalised grid one texel apart around the (stochastically jittered) texel, scaled by 100 / dim, normalised; the
 // reference returns 0 for NanoVDB grids.
-CRVPT_DEV V3 cloudFiniteDifference(const NanoAccess&, const SceneParams&, const PathState&, V3) { return v3(0.0f); }
+template <bool WIDE>
+CRVPT_DEV V3 cloudFiniteDifference(const NanoAccessT<WIDE>&, const SceneParams&, const PathState&, V3) {
+    return v3(0.0f);
+}
 CRVPT_DEV V3 cloudFiniteDifference(const DenseAccess& grid, const SceneParams& P, const PathState& st, V3 p) {
     V3 u = grid.toTex(P, p);
     if (P.interpolation == INTERP_STOCHASTIC) {

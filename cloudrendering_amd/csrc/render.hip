@@ -558,15 +558,18 @@ __global__ void k_majorant(Grid grid, int svSize, int gx, int gy, int gz, float*
 
 // Flat index of the 8^3 blocks of a NanoVDB grid: per block in [b0, b0 + nb) the leaf offset the tree descent
 // returns, or 0 and the tile value. Built once per uploaded grid; the accessor then replaces each descent by one load.
+// Entries hold the leaf offset / 32; a leaf offset that is not 32-byte aligned or beyond 128 GiB cannot be encoded
+// and sets *bad (the host then renders without the index).
 __global__ void k_blockTable(NanoAccess grid, int bx0, int by0, int bz0, int nbx, int nby, int nbz,
-                             uint32_t* __restrict__ leaf, float* __restrict__ tile) {
+                             uint32_t* __restrict__ leaf, float* __restrict__ tile, uint32_t* __restrict__ bad) {
     const size_t n = size_t(nbx) * nby * nbz;
     for (size_t b = size_t(blockIdx.x) * blockDim.x + threadIdx.x; b < n; b += size_t(gridDim.x) * blockDim.x) {
         const int bi = int(b % size_t(nbx)), bj = int((b / size_t(nbx)) % size_t(nby)),
                   bk = int(b / (size_t(nbx) * nby));
         float t = 0.0f;
-        const uint32_t off = grid.locateBlock((bx0 + bi) * 8, (by0 + bj) * 8, (bz0 + bk) * 8, t);
-        leaf[b] = off;
+        const uint64_t off = grid.locateBlock((bx0 + bi) * 8, (by0 + bj) * 8, (bz0 + bk) * 8, t);
+        if ((off & 31u) != 0u || (off >> 37) != 0u) atomicOr(bad, 1u);
+        leaf[b] = uint32_t(off >> 5);
         tile[b] = off ? 0.0f : t;
     }
 }
@@ -765,7 +768,8 @@ static hipError_t launchRenderT(const SceneParams& P, const Grid& grid, const Su
 hipError_t launchRender(const SceneParams& P, const DenseGrid* dense, const NanoGrid* nano, const SupervoxelGrid& S,
                         float4* paths, uint32_t* counter, const LaunchConfig& cfg, unsigned long long* stats,
                         float4* feat, hipStream_t stream) {
-    if (nano) return launchRenderT(P, NanoAccess{*nano}, S, paths, counter, cfg, stats, feat, stream);
+    if (nano && nano->wide) return launchRenderT(P, NanoAccess{*nano}, S, paths, counter, cfg, stats, feat, stream);
+    if (nano) return launchRenderT(P, NanoAccessNarrow{*nano}, S, paths, counter, cfg, stats, feat, stream);
     return launchRenderT(P, DenseAccess{*dense}, S, paths, counter, cfg, stats, feat, stream);
 }
 
@@ -812,14 +816,14 @@ hipError_t launchTransmittance(const SceneParams& P, const DenseGrid* dense, con
 }
 
 hipError_t launchBlockTable(const NanoGrid& nano, const int b0[3], const int nb[3], uint32_t* leaf, float* tile,
-                            hipStream_t stream) {
+                            uint32_t* bad, hipStream_t stream) {
     NanoGrid g = nano;
     g.blockLeaf = nullptr;  // the table is built from the tree itself
     g.blockTile = nullptr;
     const size_t n = size_t(nb[0]) * nb[1] * nb[2];
     const size_t blocks = std::min<size_t>((n + 255) / 256, 65536);
     hipLaunchKernelGGL(k_blockTable, dim3(unsigned(blocks)), dim3(256), 0, stream, NanoAccess{g}, b0[0], b0[1], b0[2],
-                       nb[0], nb[1], nb[2], leaf, tile);
+                       nb[0], nb[1], nb[2], leaf, tile, bad);
     return hipGetLastError();
 }
 
@@ -903,7 +907,7 @@ static const void* pathPoolKernel(int mode) {
 
 int occupancyBlocksPerCU(int mode, bool sparse) {
     int blocks = 0;
-    const void* k = sparse ? pathPoolKernel<NanoAccess>(mode) : pathPoolKernel<DenseAccess>(mode);
+    const void* k = sparse ? pathPoolKernel<NanoAccessNarrow>(mode) : pathPoolKernel<DenseAccess>(mode);
     const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k, 256, 0);
     if (e != hipSuccess || blocks <= 0) return 4;
     return blocks;

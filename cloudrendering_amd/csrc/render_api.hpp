@@ -102,9 +102,10 @@ hipError_t launchTransmittance(const SceneParams& P, const DenseGrid* dense, con
                                const SupervoxelGrid& S, int estimator, const float* o, const float* d,
                                const float* tmax, int nRays, int nSamples, uint32_t seed, float* mean, float* m2,
                                hipStream_t stream);
-// Flat 8^3-block index of a NanoVDB grid over blocks [b0, b0 + nb) (see NanoGrid::blockLeaf).
+// Flat 8^3-block index of a NanoVDB grid over blocks [b0, b0 + nb) (see NanoGrid::blockLeaf); *bad (cleared by the
+// caller) becomes nonzero when a leaf offset cannot be encoded.
 hipError_t launchBlockTable(const NanoGrid& nano, const int b0[3], const int nb[3], uint32_t* leaf, float* tile,
-                            hipStream_t stream);
+                            uint32_t* bad, hipStream_t stream);
 hipError_t launchMajorant(const DenseGrid* dense, const NanoGrid* nano, const int dims[3], int svSize,
                           const int grid3[3], float* maj, hipStream_t stream);
 hipError_t launchRefSupervoxelStats(const DenseGrid* dense, const NanoGrid* nano, const int dims[3], int svSize,

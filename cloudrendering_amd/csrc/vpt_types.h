@@ -61,16 +61,17 @@ struct DenseGrid {
 // Sparse grid: a NanoVDB 32.3 buffer with FP32 or Fp16 leaves, resident in HBM, read with 32-bit byte offsets.
 struct NanoGrid {
     const uint8_t* buf;
-    uint32_t rootOffset;  // byte offset of the root node
+    uint64_t rootOffset;  // byte offset of the root node
     uint32_t tileCount;
     float invMat[9];      // world -> index map, FP32 copy (Map::mInvMatF, row-major)
     float vec[3];         // Map::mVecF
     float background;
     int32_t indexMin[3];  // index bounding box minimum: texel (0,0,0) of the dense-equivalent grid
     int32_t fp16;         // 1: Fp16 leaves (codes at +96, minimum at +80, quantum at +84); 0: FP32 leaves
+    int32_t wide;         // 1: the grid spans 4 GiB or more (64-bit byte offsets)
     // Optional flat index of the 8^3 blocks in [blockMin, blockMin + blockDim) (block coordinates, x fastest): per
-    // block the leaf's byte offset, or 0 with the block's tile value in blockTile, exactly what the tree descent
-    // returns. Without one (blockDim 0, null pointers) every lookup descends the tree.
+    // block the leaf's byte offset / 32 (nodes are 32-byte aligned), or 0 with the block's tile value in blockTile,
+    // exactly what the tree descent returns. Without one (blockDim 0, null pointers) every lookup descends the tree.
     const uint32_t* blockLeaf;
     const float* blockTile;
     int32_t blockMin[3];

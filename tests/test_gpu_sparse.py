@@ -108,6 +108,32 @@ def test_tile_blocks_through_the_block_index(vpt):
     np.testing.assert_array_equal(got.view(np.uint32), expect.view(np.uint32))
 
 
+def test_grid_beyond_4_gib(vpt):
+    """A NanoVDB grid of 5.3 GiB (1100^3 voxels, all active): leaves beyond the first 4 GiB are read through 64-bit
+    offsets, via the block index and via tree descents outside it; every lookup equals the voxel written."""
+    import ctypes
+
+    print("case: 1100^3 dense-active NanoVDB grid (5.3 GiB)", flush=True)
+    n = 1100
+    vpt.generate_fbm_cloud(n, n, n, seed=3, octaves=5, edge0=-1.0, edge1=2.0)
+    vpt.set_use_sparse_grid(True)
+    rng = np.random.default_rng(5)
+    q = rng.integers(-3, n + 3, size=(500_000, 3)).astype(np.int32)
+    top = np.stack(np.meshgrid(np.arange(n), np.arange(n - 8, n), np.arange(n - 4, n), indexing="ij"), -1)
+    q = np.concatenate([q, top.reshape(-1, 3).astype(np.int32)])
+    got = vpt.grid_lookup(q)  # builds the grid (prepareScene) before the size query below
+    size = ctypes.c_uint64()
+    vpt.call("crvpt_get_nanovdb_buffer", None, 0, ctypes.byref(size))
+    assert size.value > 2 ** 32 + 2 ** 30
+    d = vpt.get_density_field((n, n, n), normalized=True)
+    inside = np.all((q >= 0) & (q < n), 1)
+    expect = np.zeros(len(q), np.float32)
+    expect[inside] = d[q[inside, 2], q[inside, 1], q[inside, 0]]
+    del d
+    np.testing.assert_array_equal(got.view(np.uint32), expect.view(np.uint32))
+    vpt.set_use_sparse_grid(False)
+
+
 @pytest.mark.parametrize("interp", ["nearest", "stochastic"])
 @pytest.mark.parametrize("sparse", [False, True])
 @pytest.mark.parametrize("sv", [8, 16])
