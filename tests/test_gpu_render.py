@@ -136,6 +136,27 @@ def test_render_vs_oracle(vpt, oracle, name, mode, interp):
     _check_equal(m_gpu, s_gpu, m_ref, s_ref, f"{name}/{interp}")
 
 
+@pytest.mark.parametrize("name,mode", MODES)
+def test_converged_image_rel_mse(vpt, oracle, name, mode):
+    """Converged images agree pixel by pixel: a 65536-spp GPU render against the oracle's 65536-spp render of the
+    same scene and camera, relative MSE (mean of (x - r)^2 / (r^2 + 0.01)) at most 1e-3; the two renders' sampling
+    noise (per-pixel variance / spp, 0.5e-4 to 6e-4 here) accounts for it."""
+    print(f"case: converged 24x24 image relMSE mode={name}", flush=True)
+    res, gpu_spp, ref_spp = 24, 65536, 65536
+    d = oracle.fbm_cloud(24, 20, 16, seed=5)
+    _set_volume(vpt, d)
+    _common(vpt, mode, interp="stochastic", ext=96.0, g=0.5)
+    gpu = _cpu(_render(vpt, gpu_spp, res)).transpose(1, 2, 0).astype(np.float64)
+    sc = oracle.Scene(d, mode=name, interpolation="stochastic", extinction=96.0, g=0.5, width=res, height=res)
+    ref, sq = sc.render(ref_spp)
+    ref = ref.astype(np.float64)
+    rel = float(np.mean((gpu - ref) ** 2 / (ref * ref + 0.01)))
+    var = np.maximum(sq / ref_spp - ref ** 2, 0.0)
+    noise = float(np.mean(var / ref_spp / (ref * ref + 0.01)))
+    print(f"relMSE {rel:.3e} (oracle sampling noise {noise:.3e})", flush=True)
+    assert rel <= 1e-3, f"{name}: relMSE {rel}"
+
+
 @pytest.mark.parametrize("name,mode", [("delta_spectral", 1), ("next_event_spectral", 6)])
 @pytest.mark.parametrize("majorant_grid", [True, False])
 def test_spectral_vs_oracle(vpt, oracle, name, mode, majorant_grid):
