@@ -588,14 +588,18 @@ def _time_oracle(sc, res, threads, budget_s, what):
                       f"{threads} threads"}
 
 
-def load_traffic(workload: str):
-    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 PMC summary, when one exists."""
+def load_pmc(workload: str) -> dict:
+    """The dominant kernel's committed rocprofv3 PMC summary (profiles/pmc_<workload>.json), or {}."""
     path = os.path.join(REPO, "profiles", f"pmc_{workload}.json")
     if not os.path.exists(path):
-        return None
+        return {}
     with open(path) as f:
-        pmc = json.load(f)
-    return pmc.get("traffic_bytes_per_launch")
+        return json.load(f)
+
+
+def load_traffic(workload: str):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 PMC summary, when one exists."""
+    return load_pmc(workload).get("traffic_bytes_per_launch")
 
 
 def copy_calibration(torch):
@@ -743,6 +747,9 @@ def main() -> int:
         "config": config,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": load_traffic(args.workload),
+                     # the bound the kernel actually meets: VALU issue (SQ_INSTS_VALU x 4 cycles per wave64
+                     # instruction over 1024 SIMDs at 2.4 GHz for the launch), from the same PMC summary
+                     "valu_issue_frac": load_pmc(args.workload).get("valu_issue_frac"),
                      "kernel": kernel, "launches_per_step": launches,
                      "launch_ms": round(dom_ms / launches, 4), "algorithmic_bytes_per_launch": int(b_alg / launches),
                      "density_taps": int(taps), "supervoxel_reads": int(svr), "tree_descents": int(desc),
