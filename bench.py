@@ -46,9 +46,9 @@ C2 = {"max_bounces": 128, "grid": 512, "edge0": 0.645, "edge1": 0.795, "width": 
       "modes": [(0, "delta tracking"), (2, "ratio tracking"), (4, "residual ratio tracking"),
                 (3, "decomposition tracking")], "gs": [0.0, 0.5, 0.85]}
 C1 = {"max_bounces": 64, "grid": 128, "width": 256, "height": 256, "spp": 64}
-MODE_KERNELS = {0: "k_pathPool<DELTA_TRACKING, NanoAccessT<false>, false>", 2: "k_pathPool<RATIO_TRACKING, NanoAccessT<false>, false>",
-                4: "k_pathPool<RESIDUAL_RATIO_TRACKING, NanoAccessT<false>, false>",
-                3: "k_pathPool<DECOMPOSITION_TRACKING, NanoAccessT<false>, false>"}
+MODE_KERNELS = {0: "k_pathPool<DELTA_TRACKING, NanoAccessT<false, false>, false>", 2: "k_pathPool<RATIO_TRACKING, NanoAccessT<false, false>, false>",
+                4: "k_pathPool<RESIDUAL_RATIO_TRACKING, NanoAccessT<false, false>, false>",
+                3: "k_pathPool<DECOMPOSITION_TRACKING, NanoAccessT<false, false>, false>"}
 
 
 def setup_c1(vpt, torch):
@@ -509,7 +509,7 @@ def run_c5(args, vpt, torch):
                              "denoised_8spp": [round(p["denoised_rel_mse"], 6) for p in per]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": load_traffic("c5_fp16_2048"),
-                     "kernel": "k_pathPool<RESIDUAL_RATIO_TRACKING, NanoAccessT<false>, false>", "launches_per_frame": launches,
+                     "kernel": "k_pathPool<RESIDUAL_RATIO_TRACKING, NanoAccessT<false, false>, false>", "launches_per_frame": launches,
                      "launch_ms": round(kernel_ms / launches, 3), "algorithmic_bytes_per_launch": int(b_alg / launches),
                      "counters_per_frame": {k: int(v) for k, v in c.items()}},
         "cpu_baseline": None,
@@ -725,7 +725,7 @@ def main() -> int:
                   "extinction": 1024.0, "environment": "default sky", "max_bounces": C2["max_bounces"],
                   "majorant": "16^3 supervoxel grid", "steps_cycle": "estimator x g (12 combinations)"}
     else:
-        kernel = "k_pathPool<NEXT_EVENT_TRACKING, DenseAccess, false>"
+        kernel = "k_pathPool<NEXT_EVENT_TRACKING, DenseAccessT<false>, false>"
         config = {"workload": args.workload, "volume": "dense 128^3 fp32 fBm seed 0", "image": f"{W}x{H}",
                   "spp_per_step": spp, "estimator": "next event tracking", "phase_g": 0.85, "albedo": 0.9,
                   "extinction": 1024.0, "environment": "constant white image + directional sun",

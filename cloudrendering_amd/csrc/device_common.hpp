@@ -278,7 +278,10 @@ struct NoCache {
     CRVPT_DEV uint32_t descents() const { return 0u; }
 };
 
-struct DenseAccess {
+// TRI: the trilinear filter is compiled in; the path kernels run TRI = false for the point filters (nearest,
+// stochastic), where the eight-tap code would only hold registers.
+template <bool TRI>
+struct DenseAccessT {
     DenseGrid g;
     using Cache = NoCache;
     CRVPT_DEV float voxel(int i, int j, int k) const {
@@ -292,7 +295,7 @@ struct DenseAccess {
     // u: texel coordinates (grid dim * normalised texture coordinate); (jx, jy, jz) in (-0.5, 0.5) is the
     // stochastic sub-voxel jitter, ignored by the other filters.
     CRVPT_DEV float lookupTex(const SceneParams& P, V3 u, float jx, float jy, float jz) const {
-        if (P.interpolation == INTERP_TRILINEAR) {
+        if (TRI && P.interpolation == INTERP_TRILINEAR) {
             u = u - v3(0.5f);
             const float fx = floorf(u.x), fy = floorf(u.y), fz = floorf(u.z);
             const int i = int(fx), j = int(fy), k = int(fz);
@@ -329,5 +332,7 @@ struct DenseAccess {
         return lookupWorld(P, rng, pos);
     }
 };
+using DenseAccess = DenseAccessT<true>;
+using DenseAccessPoint = DenseAccessT<false>;
 
 }  // namespace crvpt

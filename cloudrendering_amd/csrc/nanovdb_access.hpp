@@ -42,7 +42,8 @@ CRVPT_DEV float mulThenAdd(float a, float b, float c) {
 
 // WIDE: 64-bit byte offsets, for grids of 4 GiB and more (up to the 128 GiB the block index encodes); the path
 // kernels run the 32-bit variant on smaller grids, where the 64-bit arithmetic would cost registers in the hot loop.
-template <bool WIDE>
+// TRI: the trilinear filter is compiled in (see DenseAccessT).
+template <bool WIDE, bool TRI = true>
 struct NanoAccessT {
     using Off = typename std::conditional<WIDE, uint64_t, uint32_t>::type;
     NanoGrid g;
@@ -166,7 +167,7 @@ struct NanoAccessT {
                      pos.x * g.tapMat[3] + pos.y * g.tapMat[4] + pos.z * g.tapMat[5] + g.tapOff[1],
                      pos.x * g.tapMat[6] + pos.y * g.tapMat[7] + pos.z * g.tapMat[8] + g.tapOff[2]);
         }
-        if (P.interpolation == INTERP_TRILINEAR) {
+        if (TRI && P.interpolation == INTERP_TRILINEAR) {
             idx = idx - v3(0.5f);
             const float fx = floorf(idx.x), fy = floorf(idx.y), fz = floorf(idx.z);
             const int i = int(fx), j = int(fy), k = int(fz);
@@ -206,5 +207,6 @@ struct NanoAccessT {
 };
 using NanoAccess = NanoAccessT<true>;        // any grid (lookups, tables, transmittance, batched queue)
 using NanoAccessNarrow = NanoAccessT<false>;  // grids below 4 GiB (the single-frame path kernels)
+using NanoAccessNarrowPoint = NanoAccessT<false, false>;  // ... with a point filter
 
 }  // namespace crvpt

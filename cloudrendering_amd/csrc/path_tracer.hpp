@@ -385,11 +385,12 @@ CRVPT_DEV bool startPath(PathState& st, Walker& W, const SceneParams& P, const S
 // getCloudFiniteDifference (VptUtils.glsl:726-748) at p for the normal feature map: central differences of the
 // normalised grid one texel apart around the (stochastically jittered) texel, scaled by 100 / dim, normalised; the
 // reference returns 0 for NanoVDB grids.
-template <bool WIDE>
-CRVPT_DEV V3 cloudFiniteDifference(const NanoAccessT<WIDE>&, const SceneParams&, const PathState&, V3) {
+template <bool WIDE, bool TRI>
+CRVPT_DEV V3 cloudFiniteDifference(const NanoAccessT<WIDE, TRI>&, const SceneParams&, const PathState&, V3) {
     return v3(0.0f);
 }
-CRVPT_DEV V3 cloudFiniteDifference(const DenseAccess& grid, const SceneParams& P, const PathState& st, V3 p) {
+template <bool TRI>
+CRVPT_DEV V3 cloudFiniteDifference(const DenseAccessT<TRI>& grid, const SceneParams& P, const PathState& st, V3 p) {
     V3 u = grid.toTex(P, p);
     if (P.interpolation == INTERP_STOCHASTIC) {
         const uint4 b = pathBlock(P, st, kFeatureBlock);

@@ -768,8 +768,12 @@ static hipError_t launchRenderT(const SceneParams& P, const Grid& grid, const Su
 hipError_t launchRender(const SceneParams& P, const DenseGrid* dense, const NanoGrid* nano, const SupervoxelGrid& S,
                         float4* paths, uint32_t* counter, const LaunchConfig& cfg, unsigned long long* stats,
                         float4* feat, hipStream_t stream) {
+    // variants: 64-bit offsets only for grids of 4 GiB and more; the trilinear code only for the trilinear filter
+    const bool tri = P.interpolation == INTERP_TRILINEAR;
     if (nano && nano->wide) return launchRenderT(P, NanoAccess{*nano}, S, paths, counter, cfg, stats, feat, stream);
-    if (nano) return launchRenderT(P, NanoAccessNarrow{*nano}, S, paths, counter, cfg, stats, feat, stream);
+    if (nano && tri) return launchRenderT(P, NanoAccessNarrow{*nano}, S, paths, counter, cfg, stats, feat, stream);
+    if (nano) return launchRenderT(P, NanoAccessNarrowPoint{*nano}, S, paths, counter, cfg, stats, feat, stream);
+    if (!tri) return launchRenderT(P, DenseAccessPoint{*dense}, S, paths, counter, cfg, stats, feat, stream);
     return launchRenderT(P, DenseAccess{*dense}, S, paths, counter, cfg, stats, feat, stream);
 }
 
@@ -907,7 +911,7 @@ static const void* pathPoolKernel(int mode) {
 
 int occupancyBlocksPerCU(int mode, bool sparse) {
     int blocks = 0;
-    const void* k = sparse ? pathPoolKernel<NanoAccessNarrow>(mode) : pathPoolKernel<DenseAccess>(mode);
+    const void* k = sparse ? pathPoolKernel<NanoAccessNarrowPoint>(mode) : pathPoolKernel<DenseAccessPoint>(mode);
     const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k, 256, 0);
     if (e != hipSuccess || blocks <= 0) return 4;
     return blocks;
