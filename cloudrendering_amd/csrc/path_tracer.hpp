@@ -33,10 +33,11 @@ constexpr float kInf = 3.0e38f;
 // cells before it returns, so empty space costs cell reads but no random-number blocks.
 // ------------------------------------------------------------------------------------------------------------
 
-// Per-estimator launch tuning, measured on MI355X (1024^2 renders of the 512^3 sparse grid, 64-256 spp): cell
-// crossings per walker call (more crossings per step means fewer steps but more lanes idle while the others
-// cross), and the waves per SIMD the pool kernels are compiled for (128 registers per lane at 4 waves, 102 at 5).
-// Residual ratio and next-event tracking carry the most per-lane state and run best at 2 crossings, 4 waves.
+// Per-estimator launch tuning, measured on MI355X (1024^2 renders of the 512^3 sparse grid, 64-256 spp, and the
+// dense 128^3 next-event workload): cell crossings per walker call (more crossings per step means fewer steps but
+// more lanes idle while the others cross), and the waves per SIMD the pool kernels are compiled for (128 registers
+// per lane at 4 waves, 96 at 5). Next-event tracking carries the most per-lane state (the shadow ray's) and runs
+// best at 4 waves; residual ratio and next-event tracking at 2 crossings.
 template <int MODE>
 constexpr int maxCrossings() {
     return (MODE == VPT_RESIDUAL_RATIO_TRACKING || MODE == VPT_NEXT_EVENT_TRACKING ||
@@ -44,7 +45,7 @@ constexpr int maxCrossings() {
 }
 template <int MODE>
 constexpr int poolWaves() {
-    return (MODE == VPT_DELTA_TRACKING || MODE == VPT_RATIO_TRACKING) ? 5 : 4;
+    return (MODE == VPT_NEXT_EVENT_TRACKING || MODE == VPT_NEXT_EVENT_TRACKING_SPECTRAL) ? 4 : 5;
 }
 
 template <int MODE>
