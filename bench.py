@@ -747,9 +747,7 @@ def main() -> int:
         "config": config,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": load_traffic(args.workload),
-                     # the bound the kernel actually meets: VALU issue (SQ_INSTS_VALU x 4 cycles per wave64
-                     # instruction over 1024 SIMDs at 2.4 GHz for the launch), from the same PMC summary
-                     "valu_issue_frac": load_pmc(args.workload).get("valu_issue_frac"),
+                     "valu_insts_per_launch": load_pmc(args.workload).get("valu_insts_per_launch"),
                      "kernel": kernel, "launches_per_step": launches,
                      "launch_ms": round(dom_ms / launches, 4), "algorithmic_bytes_per_launch": int(b_alg / launches),
                      "density_taps": int(taps), "supervoxel_reads": int(svr), "tree_descents": int(desc),
