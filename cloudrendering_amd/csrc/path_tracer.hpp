@@ -36,8 +36,9 @@ constexpr float kInf = 3.0e38f;
 // Per-estimator launch tuning, measured on MI355X (1024^2 renders of the 512^3 sparse grid, 64-256 spp, and the
 // dense 128^3 next-event workload): cell crossings per walker call (more crossings per step means fewer steps but
 // more lanes idle while the others cross), and the waves per SIMD the pool kernels are compiled for (128 registers
-// per lane at 4 waves, 96 at 5). Next-event tracking carries the most per-lane state (the shadow ray's) and runs
-// best at 4 waves; residual ratio and next-event tracking at 2 crossings.
+// per lane at 4 waves, 96 at 5, 80 at 6). Next-event tracking carries the most per-lane state (the shadow ray's) and
+// runs best at 4 waves, delta and ratio tracking the least (6 waves); residual ratio and next-event tracking at 2
+// crossings.
 template <int MODE>
 constexpr int maxCrossings() {
     return (MODE == VPT_RESIDUAL_RATIO_TRACKING || MODE == VPT_NEXT_EVENT_TRACKING ||
@@ -45,7 +46,8 @@ constexpr int maxCrossings() {
 }
 template <int MODE>
 constexpr int poolWaves() {
-    return (MODE == VPT_NEXT_EVENT_TRACKING || MODE == VPT_NEXT_EVENT_TRACKING_SPECTRAL) ? 4 : 5;
+    if (MODE == VPT_NEXT_EVENT_TRACKING || MODE == VPT_NEXT_EVENT_TRACKING_SPECTRAL) return 4;
+    return (MODE == VPT_DELTA_TRACKING || MODE == VPT_RATIO_TRACKING) ? 6 : 5;
 }
 
 template <int MODE>
