@@ -31,6 +31,7 @@ SIGNATURES = {
     "crvpt_cleanup": (_c_int, []),
     "crvpt_render_frame": (_c_int, [_p, _c_int, _c_int, _c_int, _c_int, _c_int, _p]),
     "crvpt_load_cloud_file": (_c_int, [_s]),
+    "crvpt_read_xyz_file": (_c_int, [_s, _pi, _pd, _p, _c_u64]),
     "crvpt_set_density_field": (_c_int, [_c_int, _c_int, _c_int, _p]),
     "crvpt_set_nanovdb_buffer": (_c_int, [_p, _c_u64]),
     "crvpt_generate_fbm_cloud": (_c_int, [_c_int, _c_int, _c_int, _c_u32, _c_int, _c_f, _c_f, _c_f, _c_f, _c_f,

@@ -323,6 +323,14 @@ int setSparseHost(Renderer& r, std::vector<uint8_t>&& buf, bool fromDense) {
         }
     }
     computeSparseBounds(r, fromDense);
+    if (!fromDense) {
+        // the dense field a dense render decodes from this grid is placed in the grid's box
+        // (CloudData::getDenseDensityField, CloudData.cpp:1190-1191)
+        for (int a = 0; a < 3; a++) {
+            r.boxMinDense[a] = r.boxMinSparse[a];
+            r.boxMaxDense[a] = r.boxMaxSparse[a];
+        }
+    }
     HIP_TRY(r.dNvdb.ensure(r.nvdbHost.size()));
     HIP_TRY(hipMemcpy(r.dNvdb.ptr, r.nvdbHost.data(), r.nvdbHost.size(), hipMemcpyHostToDevice));
     r.volumeVersion++;
@@ -446,6 +454,50 @@ void gridViews(Renderer& r, DenseGrid& dg, NanoGrid& ng, bool& sparse) {
     }
 }
 
+// CloudData::getDenseDensityField (CloudData.cpp:1173-1253): a volume loaded only as NanoVDB renders through a dense
+// field decoded from the grid over its index bounding box and placed in the sparse grid's box. (The reference stores
+// that field as half floats from 1024^3 voxels on; this build keeps FP32.)
+int decodeDenseFromSparse(Renderer& r) {
+    if (r.hasDense || !r.hasSparse) return CRVPT_OK;
+    int dims[3];
+    size_t n = 1;
+    for (int a = 0; a < 3; a++) {
+        dims[a] = r.sparseDims[a];
+        n *= size_t(dims[a]);
+    }
+    HIP_TRY(hipDeviceSynchronize());
+    DenseGrid dg;
+    NanoGrid ng;
+    bool sp;
+    gridViews(r, dg, ng, sp);
+    HIP_TRY(r.dRaw.ensure(n * sizeof(float)));
+    HIP_TRY(launchDecodeDense(ng, dims, r.dRaw.as<float>(), r.stream));
+    r.denseHost.resize(n);
+    HIP_TRY(hipStreamSynchronize(r.stream));
+    HIP_TRY(hipMemcpy(r.denseHost.data(), r.dRaw.ptr, n * sizeof(float), hipMemcpyDeviceToHost));
+    float mn = 3.4e38f, mx = -3.4e38f;
+    for (float v : r.denseHost) {
+        mn = std::min(mn, v);
+        mx = std::max(mx, v);
+    }
+    if (mn == mx) mn = 0.0f;  // DensityField::computeMinMax (DensityField.cpp:149-152)
+    r.vmin = mn;
+    r.vmax = mx;
+    for (int a = 0; a < 3; a++) {
+        r.dims[a] = dims[a];
+        r.boxMinDense[a] = r.boxMinSparse[a];
+        r.boxMaxDense[a] = r.boxMaxSparse[a];
+    }
+    r.denseOnHostValid = true;
+    r.hasDense = true;
+    r.volumeVersion++;
+    r.sceneDirty = r.tablesDirty = true;
+    r.refTablesValid = false;
+    return CRVPT_OK;
+}
+
+int ensureDense(Renderer& r) { return r.useSparse ? CRVPT_OK : decodeDenseFromSparse(r); }
+
 // Builds the flat 8^3-block index over the grid's index bounding box (at most 2^27 blocks: 1 GiB of index; larger
 // sparse domains, and grids whose leaves lie beyond 128 GiB, keep descending the tree).
 int buildBlockTable(Renderer& r) {
@@ -487,6 +539,8 @@ int buildBlockTable(Renderer& r) {
 int prepareScene(Renderer& r) {
     if (!r.hasDense && !r.hasSparse) return fail(CRVPT_ERR_INVALID_ARGUMENT, "crvpt: no volume loaded");
     int rc = ensureSparse(r);
+    if (rc) return rc;
+    rc = ensureDense(r);
     if (rc) return rc;
     rc = uploadEnv(r);
     if (rc) return rc;
@@ -1076,28 +1130,12 @@ int crvpt_load_cloud_file(const char* filename) {
     const std::string ext = fn.size() >= 4 ? fn.substr(fn.size() - 4) : "";
     Renderer& r = *g_r;
     if (ext == ".xyz") {
-        // CloudData::loadFromXyzFile (CloudData.cpp:236-316): u32 x3 sizes, f64 x3 voxel sizes, z-fastest floats.
-        if (bytes.size() < 36) return fail(CRVPT_ERR_IO, "load_cloud_file: truncated .xyz header");
-        uint32_t sz[3];
+        int dims[3];
         double vs[3];
-        std::memcpy(sz, bytes.data(), 12);
-        std::memcpy(vs, bytes.data() + 12, 24);
-        const size_t n = size_t(sz[0]) * sz[1] * sz[2];
-        if (bytes.size() < 36 + n * 4) return fail(CRVPT_ERR_IO, "load_cloud_file: truncated .xyz data");
-        const float* src = reinterpret_cast<const float*>(bytes.data() + 36);
-        std::vector<float> v(n);
-        for (uint32_t z = 0; z < sz[2]; z++)
-            for (uint32_t y = 0; y < sz[1]; y++)
-                for (uint32_t x = 0; x < sz[0]; x++)
-                    v[x + (y + size_t(z) * sz[1]) * sz[0]] = src[z + (y + size_t(x) * sz[1]) * sz[2]];
-        float mn = 0.0f, mx = -3.4e38f;  // sgl::reduceFloatArrayMinMax seeded with (0, lowest)
-        for (float x : v) {
-            mn = std::min(mn, x);
-            mx = std::max(mx, x);
-        }
-        if (mx - mn > 1e-6f)
-            for (float& x : v) x = (x - mn) / (mx - mn);
-        int rc = setDenseHost(r, int(sz[0]), int(sz[1]), int(sz[2]), std::move(v));
+        std::vector<float> v;
+        std::string err;
+        if (!parseXyzFile(bytes, dims, vs, v, err)) return fail(CRVPT_ERR_IO, err);
+        int rc = setDenseHost(r, dims[0], dims[1], dims[2], std::move(v));
         if (rc) return rc;
         r.voxelSize[0] = float(vs[0]);
         r.voxelSize[1] = float(vs[1]);
@@ -1109,10 +1147,29 @@ int crvpt_load_cloud_file(const char* filename) {
         std::vector<uint8_t> grid;
         std::string err;
         if (!extractNanoVdbGridFromFile(bytes, grid, err)) return fail(CRVPT_ERR_IO, err);
-        r.useSparse = true;
-        return setSparseHost(r, std::move(grid), false);
+        const int rc = setSparseHost(r, std::move(grid), false);
+        if (rc == CRVPT_OK) r.useSparse = true;  // only once the grid is in place
+        return rc;
     }
     return fail(CRVPT_ERR_UNSUPPORTED, "load_cloud_file: unsupported file format \"" + fn + "\"");
+}
+
+int crvpt_read_xyz_file(const char* filename, int32_t dims[3], double voxel_size[3], float* out,
+                        uint64_t capacity) {
+    if (!filename || !dims || !voxel_size) return fail(CRVPT_ERR_INVALID_ARGUMENT, "read_xyz_file: null argument");
+    std::ifstream f(filename, std::ios::binary);
+    if (!f) return fail(CRVPT_ERR_IO, std::string("read_xyz_file: cannot open \"") + filename + "\"");
+    std::vector<uint8_t> bytes((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    int d[3];
+    std::vector<float> v;
+    std::string err;
+    if (!parseXyzFile(bytes, d, voxel_size, v, err)) return fail(CRVPT_ERR_IO, err);
+    for (int a = 0; a < 3; a++) dims[a] = d[a];
+    if (out) {
+        if (capacity < v.size()) return fail(CRVPT_ERR_INVALID_ARGUMENT, "read_xyz_file: output too small");
+        std::memcpy(out, v.data(), v.size() * sizeof(float));
+    }
+    return CRVPT_OK;
 }
 
 int crvpt_set_density_field(int nx, int ny, int nz, const float* values) {
@@ -1127,8 +1184,9 @@ int crvpt_set_nanovdb_buffer(const uint8_t* data, uint64_t size) {
     if (int rc = requireInit()) return rc;
     if (!data || size == 0) return fail(CRVPT_ERR_INVALID_ARGUMENT, "set_nanovdb_buffer: empty buffer");
     std::vector<uint8_t> b(data, data + size);
-    g_r->useSparse = true;
-    return setSparseHost(*g_r, std::move(b), false);
+    const int rc = setSparseHost(*g_r, std::move(b), false);
+    if (rc == CRVPT_OK) g_r->useSparse = true;  // a rejected buffer leaves the renderer unchanged
+    return rc;
 }
 
 int crvpt_generate_fbm_cloud(int nx, int ny, int nz, uint32_t seed, int octaves, float lacunarity, float gain,
@@ -1152,6 +1210,7 @@ int crvpt_generate_fbm_cloud(int nx, int ny, int nz, uint32_t seed, int octaves,
 int crvpt_get_density_field(float* out, uint64_t count, int normalized) {
     if (int rc = requireInit()) return rc;
     Renderer& r = *g_r;
+    if (int rc = decodeDenseFromSparse(r)) return rc;  // NanoVDB-only volumes: CloudData::getDenseDensityField
     if (!r.hasDense) return fail(CRVPT_ERR_INVALID_ARGUMENT, "get_density_field: no dense field");
     const size_t n = size_t(r.dims[0]) * r.dims[1] * r.dims[2];
     if (count < n || !out) return fail(CRVPT_ERR_INVALID_ARGUMENT, "get_density_field: output too small");
@@ -1222,9 +1281,8 @@ int crvpt_get_nanovdb_buffer(uint8_t* out, uint64_t capacity, uint64_t* size) {
 
 int crvpt_set_use_sparse_grid(int use_sparse) {
     if (int rc = requireInit()) return rc;
+    // A volume loaded only as NanoVDB gets its dense field decoded from the grid at the next render (ensureDense).
     g_r->useSparse = use_sparse != 0;
-    if (!g_r->useSparse && !g_r->hasDense)
-        return fail(CRVPT_ERR_UNSUPPORTED, "set_use_sparse_grid: the loaded volume has no dense representation");
     g_r->sceneDirty = g_r->tablesDirty = true;
     g_r->refTablesValid = false;
     g_r->resetAccumulation();

@@ -51,7 +51,13 @@ __global__ void k_invIterBase(const float* __restrict__ hiX, const float* __rest
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     const int n = blockIdx.z;
     const int lx = x / 2, ly = y / 2;
-    if (x >= w || y >= h || lx >= loW || ly >= loH) return;
+    if (x >= w || y >= h) return;
+    if (lx >= loW || ly >= loH) {
+        // outside the low-resolution image the reference leaves its zero-initialised result (Kernel.cu:240)
+        for (int c = 0; c < C; c++)
+            out[size_t(n) * size_t(w) * h * C + size_t(c) * size_t(w) * h + size_t(y) * w + x] = 0.0f;
+        return;
+    }
     const int kd = K * K;
     const size_t wPlane = size_t(wW) * wH;
     const float* wb = weights + size_t(n) * wPlane * kd;  // the reference's stride (K*K planes per batch)
@@ -66,7 +72,8 @@ __global__ void k_invIterBase(const float* __restrict__ hiX, const float* __rest
         const size_t lo = size_t(n) * size_t(loW) * loH * C + size_t(c) * size_t(loW) * loH + size_t(ly) * loW + lx;
         const float xHi = hiX[hi], xLo = loX[lo], pLo = loPred[lo];
         float lf = lfW * pLo + (1.0f - lfW) * xLo;
-        if (usePrevious) lf = lf * (1.0f - prW) + prev[hi] * prW;
+        // the reference blends with a double literal: (1.0 - prev_weight) and the sum are evaluated in double
+        if (usePrevious) lf = float(double(lf) * (1.0 - double(prW)) + double(prev[hi] * prW));
         out[size_t(n) * size_t(w) * h * C + size_t(c) * size_t(w) * h + size_t(y) * w + x] =
             hfW * (xHi - xLo + lf) + (1.0f - hfW) * lf;
     }

@@ -264,7 +264,12 @@ __global__ __launch_bounds__(256) void k_gradReduce(const float* __restrict__ wP
 }
 
 // Input features [N, 15, H, W] f32 -> [N, H, W, 32] bf16 (channels 15..31 zero). Radiance (0-2) and radiance variance
-// (11-13) enter as log(1 + max(v, 0)); the other features unchanged.
+// (11-13) enter as log(1 + max(v, 0)). The screen-space depth gradient (9-10) is A / sqrt(1 - A^2) of the averaged
+// view-space normal component A (Clouds.glsl:296): +-inf where the normal lies on a view axis and NaN where rounding
+// puts |A| above 1, so it is clamped to +-kGradientClamp. Any other non-finite input enters as 0: one NaN would
+// otherwise spread through the 3x3 receptive fields of all six layers.
+constexpr float kGradientClamp = 16.0f;
+
 __global__ void k_packFeatures(const float* __restrict__ f, __bf16* __restrict__ out, int N, int H, int W) {
     const size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const size_t plane = size_t(H) * W;
@@ -276,7 +281,12 @@ __global__ void k_packFeatures(const float* __restrict__ f, __bf16* __restrict__
         float x = 0.0f;
         if (c < kKpnInputs) {
             x = f[(n * kKpnInputs + c) * plane + q];
-            if (c <= 2 || (c >= 11 && c <= 13)) x = log1pf(fmaxf(x, 0.0f));
+            if (c == 9 || c == 10) {
+                x = x != x ? 0.0f : fminf(fmaxf(x, -kGradientClamp), kGradientClamp);
+            } else {
+                if (!__builtin_isfinite(x)) x = 0.0f;
+                if (c <= 2 || (c >= 11 && c <= 13)) x = log1pf(fmaxf(x, 0.0f));
+            }
         }
         v[c >> 3][c & 7] = (__bf16)x;
     }

@@ -197,6 +197,46 @@ bool parseNanoVdb(const uint8_t* buf, size_t size, NanoInfo& info, std::string& 
     return true;
 }
 
+bool parseXyzFile(const std::vector<uint8_t>& bytes, int dims[3], double voxelSize[3], std::vector<float>& values,
+                  std::string& err) {
+    if (bytes.size() < 36) {
+        err = "load_cloud_file: truncated .xyz header";
+        return false;
+    }
+    uint32_t sz[3];
+    std::memcpy(sz, bytes.data(), 12);
+    std::memcpy(voxelSize, bytes.data() + 12, 24);
+    // the header is untrusted: empty grids, size products that overflow and sizes beyond int are rejected
+    size_t n = 0, payload = 0;
+    if (sz[0] == 0 || sz[1] == 0 || sz[2] == 0 || sz[0] > 0x7FFFFFFFu || sz[1] > 0x7FFFFFFFu || sz[2] > 0x7FFFFFFFu ||
+        __builtin_mul_overflow(size_t(sz[0]), size_t(sz[1]), &n) || __builtin_mul_overflow(n, size_t(sz[2]), &n) ||
+        __builtin_mul_overflow(n, sizeof(float), &payload) || payload > bytes.size() - 36) {
+        err = "load_cloud_file: truncated or invalid .xyz data";
+        return false;
+    }
+    std::vector<float> src(n);
+    std::memcpy(src.data(), bytes.data() + 36, payload);
+    values.assign(n, 0.0f);
+    const size_t X = sz[0], Y = sz[1], Z = sz[2];
+    parallelFor(Z, [&](size_t z0, size_t z1, size_t) {
+        for (size_t z = z0; z < z1; z++)
+            for (size_t y = 0; y < Y; y++)
+                for (size_t x = 0; x < X; x++) values[x + (y + z * Y) * X] = src[z + (y + x * Y) * Z];
+    }, 1);
+    float mn = 0.0f, mx = -3.402823466e38f;  // sgl::reduceFloatArrayMinMax seeded with (0, lowest)
+    for (float v : values) {
+        mn = std::min(mn, v);
+        mx = std::max(mx, v);
+    }
+    if (mx - mn > 1e-6f) {
+        parallelFor(n, [&](size_t b, size_t e, size_t) {
+            for (size_t i = b; i < e; i++) values[i] = (values[i] - mn) / (mx - mn);
+        }, 1 << 20);
+    }
+    for (int a = 0; a < 3; a++) dims[a] = int(sz[a]);
+    return true;
+}
+
 bool extractNanoVdbGridFromFile(const std::vector<uint8_t>& file, std::vector<uint8_t>& grid, std::string& err) {
     // nanovdb::io segment: Header (16 B: magic, version, grid count u16, codec u16), then per grid a 176-byte
     // MetaData record followed by the grid name, then the grid buffers.

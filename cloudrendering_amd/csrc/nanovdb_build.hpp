@@ -30,6 +30,13 @@ struct NanoInfo {
 // Validates a single-grid FP32 or Fp16 NanoVDB buffer and reads the fields the renderer needs.
 bool parseNanoVdb(const uint8_t* buf, size_t size, NanoInfo& info, std::string& err);
 
+// Parses a .xyz dense grid file (CloudData::loadFromXyzFile, CloudData.cpp:236-316): u32 sizes x, y, z, f64 voxel
+// sizes, then the floats z fastest. Returns the field x fastest, min-max normalised when max - min > 1e-6 (the
+// minimum seeded with 0). Sizes are validated against overflow and the file length; trailing bytes are ignored as
+// the reference's stream reader does.
+bool parseXyzFile(const std::vector<uint8_t>& bytes, int dims[3], double voxelSize[3], std::vector<float>& values,
+                  std::string& err);
+
 // Returns the first grid of an uncompressed .nvdb file (io::writeGrid layout: header, metadata, grid bytes).
 bool extractNanoVdbGridFromFile(const std::vector<uint8_t>& file, std::vector<uint8_t>& grid, std::string& err);
 

@@ -677,6 +677,23 @@ __global__ void k_fbm(float* __restrict__ out, int nx, int ny, int nz, FbmParams
     }
 }
 
+// Dense field of a NanoVDB grid over its index bounding box, x fastest (CloudData::getDenseDensityField,
+// CloudData.cpp:1173-1253: tree.getValue at indexBBox.min + (x, y, z)). Each lane decodes a run of consecutive x
+// through one lookup memo.
+constexpr int kDecodeRun = 8;
+__global__ void k_decodeDense(NanoAccess grid, int nx, int ny, int nz, float* __restrict__ out) {
+    const size_t runsX = (size_t(nx) + kDecodeRun - 1) / kDecodeRun;
+    const size_t runs = runsX * size_t(ny) * nz;
+    for (size_t r = size_t(blockIdx.x) * blockDim.x + threadIdx.x; r < runs; r += size_t(gridDim.x) * blockDim.x) {
+        const int x0 = int(r % runsX) * kDecodeRun;
+        const size_t yz = r / runsX;
+        const int y = int(yz % size_t(ny)), z = int(yz / size_t(ny));
+        LeafCache c;
+        const int x1 = min(x0 + kDecodeRun, nx);
+        for (int x = x0; x < x1; x++) out[(size_t(z) * ny + y) * nx + x] = grid.voxelAt(x, y, z, c);
+    }
+}
+
 // Dense grid normalisation: (raw - vmin) / (vmax - vmin) per voxel, as sampleGridImage applies it per lookup.
 __global__ void k_normalize(const float* __restrict__ raw, float* __restrict__ out, size_t n, float vmin,
                             float vmax) {
@@ -865,6 +882,14 @@ hipError_t launchFbm(float* out, int nx, int ny, int nz, const FbmParams& F, hip
     const size_t total = size_t(nx) * ny * nz;
     const size_t blocks = std::min<size_t>((total + 255) / 256, 65536);
     hipLaunchKernelGGL(k_fbm, dim3(unsigned(blocks)), dim3(256), 0, stream, out, nx, ny, nz, F);
+    return hipGetLastError();
+}
+
+hipError_t launchDecodeDense(const NanoGrid& nano, const int dims[3], float* out, hipStream_t stream) {
+    const size_t runs = (size_t(dims[0]) + kDecodeRun - 1) / kDecodeRun * size_t(dims[1]) * size_t(dims[2]);
+    const size_t blocks = std::min<size_t>((runs + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_decodeDense, dim3(unsigned(blocks)), dim3(256), 0, stream, NanoAccess{nano}, dims[0], dims[1],
+                       dims[2], out);
     return hipGetLastError();
 }
 

@@ -112,6 +112,8 @@ hipError_t launchRefSupervoxelStats(const DenseGrid* dense, const NanoGrid* nano
                                     const int grid3[3], int halo, float* mn, float* mx, float* avg,
                                     hipStream_t stream);
 hipError_t launchFbm(float* out, int nx, int ny, int nz, const FbmParams& F, hipStream_t stream);
+// Dense FP32 field of a NanoVDB grid over its index bounding box (dims = its size), x fastest.
+hipError_t launchDecodeDense(const NanoGrid& nano, const int dims[3], float* out, hipStream_t stream);
 hipError_t launchNormalize(const float* raw, float* out, size_t n, float vmin, float vmax, hipStream_t stream);
 int occupancyBlocksPerCU(int mode, bool sparse);
 hipError_t launchGridLookup(const DenseGrid* dense, const NanoGrid* nano, int n, const int32_t* ijk, float* out,

@@ -47,6 +47,10 @@ int crvpt_render_frame(float* out, int channels, int height, int width, int fram
 /* Module.cpp:815-824 load_cloud_file / load_volume_file (.xyz dense float grids, CloudData.cpp:236-316;
  * .nvdb NanoVDB float grids, CloudData.cpp:1360-1365). */
 int crvpt_load_cloud_file(const char* filename);
+/* No reference counterpart, host only (no GPU needed): the .xyz parse that load_cloud_file applies
+ * (CloudData::loadFromXyzFile, CloudData.cpp:236-316): grid sizes, voxel sizes, and with out != NULL the field x
+ * fastest (transposed from the file's z-fastest order) and min-max normalised when max - min > 1e-6. */
+int crvpt_read_xyz_file(const char* filename, int32_t dims[3], double voxel_size[3], float* out, uint64_t capacity);
 /* CloudData::setDensityField (CloudData.cpp:147-167): dense grid, x fastest, size nx*ny*nz (copied). */
 int crvpt_set_density_field(int nx, int ny, int nz, const float* values);
 /* CloudData::setNanoVdbGridHandle (CloudData.cpp:169-172): a NanoVDB 32.3 FP32 grid buffer (copied). */

@@ -61,7 +61,9 @@ def inv_iter_base(x, lo_pred, lo_x, prev, fused, k, use_previous):
                     xh, xl, pl = x[b, ch, yy, xx], lo_x[b, ch, ly, lx], lo_pred[b, ch, ly, lx]
                     lf = lfw * pl + (F(1.0) - lfw) * xl
                     if use_previous:
-                        lf = lf * (F(1.0) - prw) + np.asarray(prev, F)[b, ch, yy, xx] * prw
+                        # Kernel.cu:176 blends with the double literal 1.0: the product and the sum are in double
+                        pv = F(np.asarray(prev, F)[b, ch, yy, xx] * prw)
+                        lf = F(np.float64(lf) * (1.0 - np.float64(prw)) + np.float64(pv))
                     out[b, ch, yy, xx] = hfw * (xh - xl + lf) + (F(1.0) - hfw) * lf
     return out
 

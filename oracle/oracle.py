@@ -89,6 +89,14 @@ def load():
                                            ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p, ctypes.c_int]
     lib.oracle_render_features.restype = ctypes.c_int
     lib.oracle_hybrid_taus.argtypes = [ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p]
+    lib.oracle_trace_rays.argtypes = [P(OracleScene), ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+    lib.oracle_trace_rays.restype = ctypes.c_int
+    lib.oracle_sky.argtypes = [P(OracleScene), ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    lib.oracle_sky.restype = ctypes.c_int
+    lib.oracle_light_radiance.argtypes = [P(OracleScene), ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                          ctypes.c_void_p]
+    lib.oracle_light_radiance.restype = ctypes.c_int
     lib.oracle_camera_matrices.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_double] * 4 + [ctypes.c_void_p] * 3
     _lib = lib
     return lib
@@ -250,6 +258,37 @@ class Scene:
                                       m2.ctypes.data, threads)
         assert rc == 0
         return mean, m2
+
+    def trace_rays(self, origins, directions, seed=1, rrt_max_iterations=-1, threads=0):
+        """One path of the scene's tracker per ray from (origin, direction): radiance [n, 3] float32 and, for
+        residual ratio tracking, the in-scattering iterations made [n] (see oracle_trace_rays)."""
+        lib = load()
+        o = np.ascontiguousarray(np.asarray(origins, np.float32).reshape(-1, 3))
+        d = np.ascontiguousarray(np.asarray(directions, np.float32).reshape(-1, 3))
+        n = o.shape[0]
+        rgb = np.zeros((n, 3), np.float32)
+        it = np.zeros(n, np.int32)
+        rc = lib.oracle_trace_rays(ctypes.byref(self.sc), n, o.ctypes.data, d.ctypes.data, seed & 0xFFFFFFFF,
+                                   int(rrt_max_iterations), rgb.ctypes.data, it.ctypes.data, threads)
+        assert rc == 0
+        return rgb, it
+
+    def sky(self, directions):
+        """sampleSkybox + sampleLight for unit directions [n, 3] -> [n, 3] float32."""
+        lib = load()
+        d = np.ascontiguousarray(np.asarray(directions, np.float32).reshape(-1, 3))
+        out = np.zeros_like(d)
+        assert lib.oracle_sky(ctypes.byref(self.sc), d.shape[0], d.ctypes.data, out.ctypes.data) == 0
+        return out
+
+    def light_radiance(self, light_idx, positions):
+        """sampleHeadlight: radiance of light light_idx towards positions [n, 3] -> [n, 3] float32."""
+        lib = load()
+        p = np.ascontiguousarray(np.asarray(positions, np.float32).reshape(-1, 3))
+        out = np.zeros_like(p)
+        assert lib.oracle_light_radiance(ctypes.byref(self.sc), int(light_idx), p.shape[0], p.ctypes.data,
+                                         out.ctypes.data) == 0
+        return out
 
     def optical_depth(self, origins, directions, t_max=None, threads=0):
         lib = load()

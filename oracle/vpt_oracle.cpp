@@ -115,7 +115,12 @@ struct Ctx {
     std::vector<float> rrt;     // (mu_c, mu_r_bar)
     std::vector<float> decomp;  // (min, max)
     std::vector<uint8_t> occ;
+    // in-scattering iterations of residualRatioTracking before it stops (ResidualRatioTracking.glsl:203 fixes 10;
+    // oracle_trace_rays lowers it to pin the first traversal on its own)
+    int rrtMaxIterations = 10;
 };
+// in-scattering iterations of the calling thread's last residualRatioTracking path (read by oracle_trace_rays)
+thread_local int t_lastRrtIterations = 0;
 
 inline float rawVoxel(const Ctx& c, int x, int y, int z) {
     if (x < 0 || y < 0 || z < 0 || x >= c.dim[0] || y >= c.dim[1] || z >= c.dim[2]) return 0.0f;  // border black
@@ -884,7 +889,7 @@ vec3 residualRatioTracking(const Ctx& c, Rng& rng, vec3 x, vec3 w) {  // Residua
             break;
         }
         float xi = rng.random();
-        if (xi > reservoirWeightSum || iteration >= 10) break;
+        if (xi > reservoirWeightSum || iteration >= c.rrtMaxIterations) break;
         accumulatedColor += T * (sampleSkybox(c, w) + sampleLight(c, w));
         iteration++;
         T = reservoirT;
@@ -896,6 +901,7 @@ vec3 residualRatioTracking(const Ctx& c, Rng& rng, vec3 x, vec3 w) {  // Residua
         reservoirDist = 0.0f;
     }
     accumulatedColor += T * (sampleSkybox(c, w) + sampleLight(c, w));
+    t_lastRrtIterations = iteration;
     return accumulatedColor;
 }
 
@@ -1585,4 +1591,56 @@ extern "C" int oracle_supervoxel_tables(const OracleScene* scene, float extincti
     if (decomp_pairs) std::memcpy(decomp_pairs, c.decomp.data(), 2 * n * sizeof(float));
     if (occupancy) std::memcpy(occupancy, c.occ.data(), n);
     return c.svSize;
+}
+
+extern "C" int oracle_trace_rays(const OracleScene* scene, int n_rays, const float* o, const float* d, uint32_t seed,
+                                 int rrt_max_iterations, float* out_rgb, int* out_iterations, int num_threads) {
+    if (!scene || n_rays < 0 || !o || !d || !out_rgb) return 1;
+    Ctx base;
+    initCtx(base, scene);
+    if (rrt_max_iterations >= 0) base.rrtMaxIterations = rrt_max_iterations;
+#ifdef _OPENMP
+    if (num_threads > 0) omp_set_num_threads(num_threads);
+#pragma omp parallel for schedule(dynamic, 64)
+#endif
+    for (int i = 0; i < n_rays; i++) {
+        t_lastRrtIterations = 0;
+        Rng rng;
+        rng.init(seed + uint32_t(i));
+        const vec3 r = tracePath(base, rng, vec3(o + 3 * size_t(i)), vec3(d + 3 * size_t(i)));
+        const int iterations = t_lastRrtIterations;
+        out_rgb[3 * size_t(i)] = r.x;
+        out_rgb[3 * size_t(i) + 1] = r.y;
+        out_rgb[3 * size_t(i) + 2] = r.z;
+        if (out_iterations) out_iterations[i] = iterations;
+    }
+    return 0;
+}
+
+extern "C" int oracle_sky(const OracleScene* scene, int n, const float* dirs, float* out_rgb) {
+    if (!scene || n < 0 || !dirs || !out_rgb) return 1;
+    Ctx c;
+    c.sc = scene;
+    for (int i = 0; i < n; i++) {
+        const vec3 w(dirs + 3 * size_t(i));
+        const vec3 r = sampleSkybox(c, w) + sampleLight(c, w);
+        out_rgb[3 * size_t(i)] = r.x;
+        out_rgb[3 * size_t(i) + 1] = r.y;
+        out_rgb[3 * size_t(i) + 2] = r.z;
+    }
+    return 0;
+}
+
+extern "C" int oracle_light_radiance(const OracleScene* scene, int light_idx, int n, const float* positions,
+                                     float* out_rgb) {
+    if (!scene || light_idx < 0 || light_idx >= scene->num_lights || n < 0 || !positions || !out_rgb) return 1;
+    Ctx c;
+    c.sc = scene;
+    for (int i = 0; i < n; i++) {
+        const vec3 r = sampleHeadlight(c, vec3(positions + 3 * size_t(i)), uint32_t(light_idx));
+        out_rgb[3 * size_t(i)] = r.x;
+        out_rgb[3 * size_t(i) + 1] = r.y;
+        out_rgb[3 * size_t(i) + 2] = r.z;
+    }
+    return 0;
 }

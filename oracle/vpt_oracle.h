@@ -136,6 +136,20 @@ int oracle_optical_depth_batch(const OracleScene* scene, int n_rays, const float
 int oracle_supervoxel_tables(const OracleScene* scene, float extinction_x, int* grid_size3, float* rrt_pairs,
                              float* decomp_pairs, uint8_t* occupancy);
 
+/* One path of the scene's tracker per ray, started at o[3i..] in unit direction d[3i..] instead of at the camera
+ * (hybrid-Taus stream seeded with seed + i), radiance to out_rgb[3i..]. rrt_max_iterations >= 0 replaces the
+ * residual-ratio tracker's fixed limit of 10 in-scattering iterations (0: the first traversal alone); out_iterations
+ * (nullable) receives the iterations made. Lets tests pin the radiance loops against closed forms per ray. */
+int oracle_trace_rays(const OracleScene* scene, int n_rays, const float* o, const float* d, uint32_t seed,
+                      int rrt_max_iterations, float* out_rgb, int* out_iterations, int num_threads);
+
+/* sampleSkybox(dir) + sampleLight(dir) (VptUtils.glsl:301-391) for n unit directions. */
+int oracle_sky(const OracleScene* scene, int n, const float* dirs, float* out_rgb);
+
+/* sampleHeadlight(pos, light_idx) (VptUtils.glsl:394-453): the radiance light light_idx sends towards each
+ * position. */
+int oracle_light_radiance(const OracleScene* scene, int light_idx, int n, const float* positions, float* out_rgb);
+
 /* Hybrid Taus stream (VptUtils.glsl:46-65): n values after initializeRandom(seed). */
 void oracle_hybrid_taus(uint32_t seed, int n, float* out);
 

@@ -53,13 +53,25 @@ def _inputs(seed, n, h, w):
     return feats.cuda(), noisy.cuda(), target.cuda()
 
 
+def _sanitise(feats):
+    """The packing's input guard: the depth-gradient channels 9-10 clamped to +-16 (NaN -> 0), any other
+    non-finite feature -> 0."""
+    torch = _torch()
+    x = feats.clone()
+    grad = x[:, 9:11]
+    x[:, 9:11] = torch.clamp(torch.nan_to_num(grad, nan=0.0, posinf=16.0, neginf=-16.0), -16.0, 16.0)
+    rest = [c for c in range(15) if c not in (9, 10)]
+    x[:, rest] = torch.nan_to_num(x[:, rest], nan=0.0, posinf=0.0, neginf=0.0)
+    return x
+
+
 def _reference(params, feats, noisy, target=None):
     """Float32 torch forward (with the GPU's bf16 rounding points), the 5x5 softmax filter and the relative MSE."""
     torch = _torch()
     F = torch.nn.functional
     kpn = _kpn()
     w_np, b_np = kpn.KpnDenoiser.split(params)
-    x = feats.clone()
+    x = _sanitise(feats)
     x[:, LOG_CHANNELS] = torch.log1p(torch.clamp(x[:, LOG_CHANNELS], min=0.0))
     x = _bf16(x)
     x = torch.cat([x, torch.zeros((x.shape[0], 17) + tuple(x.shape[2:]), device=x.device)], 1)
@@ -99,6 +111,31 @@ def test_forward_matches_torch(vpt):
     taps = torch.nn.functional.unfold(noisy, 5, padding=2).view(2, 3, 25, 37, 45)
     pred = (taps * k.unsqueeze(1)).sum(2)
     np.testing.assert_allclose(out.cpu().numpy(), pred.cpu().numpy(), rtol=1e-5, atol=1e-6)
+
+
+def test_non_finite_features_are_sanitised(vpt):
+    """The depth-gradient map is +-inf where an averaged normal lies on a view axis (and NaN where rounding puts the
+    normal component above 1); such pixels must not turn the network's logits into NaN."""
+    print("case: kpn non-finite features 1x24x40", flush=True)
+    torch = _torch()
+    kpn = _kpn()
+    params = _random_params(5)
+    net = kpn.KpnDenoiser(parameters=params)
+    feats, noisy, _ = _inputs(6, 1, 24, 40)
+    bad = feats.clone()
+    bad[0, 9, 3, 4] = float("inf")
+    bad[0, 10, 3, 4] = float("-inf")
+    bad[0, 9, 10, 20] = float("nan")
+    bad[0, 10, 17, 33] = 1e9
+    bad[0, 4, 12, 12] = float("nan")
+    bad[0, 0, 20, 2] = float("inf")
+    out_bad = net.denoise(bad, noisy).clone()
+    out_clean = net.denoise(_sanitise(bad), noisy).clone()
+    torch.cuda.synchronize()
+    assert torch.isfinite(out_bad).all(), "non-finite features leaked into the denoised image"
+    assert torch.equal(out_bad, out_clean)
+    ref_logits, _, _, _, _ = _reference(params, bad, noisy)
+    assert torch.isfinite(ref_logits).all()
 
 
 def test_one_hot_kernel_is_a_shift(vpt):
