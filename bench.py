@@ -119,12 +119,9 @@ def c3_requests():
     return reqs
 
 
-def setup_c3(vpt, torch):
-    """Queues the batched-queue workload: one seeded fBm volume per kind (dense: smoothstep(0.45, 0.6), ~30 %
-    nonzero; sparse: smoothstep(0.645, 0.795), ~12 % active in 8^3 leaves), next-event tracking with the default
+def c3_settings(vpt):
+    """Scene settings shared by every request of the batched-queue workload: next-event tracking with the default
     sky and one directional sun, extinction 1024, albedo 0.9, g 0.5, at most 128 scattering events per path."""
-    reqs = c3_requests()
-    vpt.batch_clear()
     vpt.set_vpt_mode(5)
     vpt.set_grid_interpolation_type(1)
     vpt.set_extinction_base([1.0, 1.0, 1.0])
@@ -140,23 +137,47 @@ def setup_c3(vpt, torch):
     vpt.set_light_property(0, "space", "World")
     vpt.set_light_property(0, "color", "1 0.961538462 0.884615385")
     vpt.set_light_property(0, "intensity", "2.6")
-    for kind, (rep, n) in enumerate(C3["volumes"]):
+
+
+def c3_volume(vpt, kind):
+    """The volume of request kind `kind`: one seeded fBm volume per kind (dense: smoothstep(0.45, 0.6), ~30 %
+    nonzero; sparse: smoothstep(0.645, 0.795), ~12 % active in 8^3 leaves)."""
+    rep, n = C3["volumes"][kind]
+    sparse = rep == "sparse"
+    e0, e1 = (0.645, 0.795) if sparse else (0.45, 0.6)
+    vpt.generate_fbm_cloud(n, n, n, seed=100 + kind, octaves=5, lacunarity=2.0, gain=0.5, base_frequency=4.0,
+                           edge0=e0, edge1=e1)
+    vpt.set_use_sparse_grid(sparse)
+    vpt.set_super_voxel_size(16 if sparse else 8)
+
+
+def c3_view(vpt, q):
+    """Camera, sun direction and seed offset of request q."""
+    vpt.set_camera_position(q["camera"])
+    vpt.set_camera_target([0.0, 0.0, 0.0])
+    vpt.set_light_property(0, "position", " ".join(f"{100.0 * v:.6f}" for v in q["sun"]))
+    vpt.set_seed_offset(q["index"])
+
+
+def c3_spp(q, spp_divisor=1):
+    return max(1, q["spp"] // spp_divisor)
+
+
+def setup_c3(vpt, torch, spp_divisor=1):
+    """Queues the batched-queue workload (c3_requests), volume by volume; spp_divisor > 1 queues every request with
+    its spp divided (at least 1)."""
+    reqs = c3_requests()
+    vpt.batch_clear()
+    c3_settings(vpt)
+    for kind in range(len(C3["volumes"])):
         mine = [q for q in reqs if q["kind"] == kind]
         if not mine:
             continue
-        sparse = rep == "sparse"
-        e0, e1 = (0.645, 0.795) if sparse else (0.45, 0.6)
-        vpt.generate_fbm_cloud(n, n, n, seed=100 + kind, octaves=5, lacunarity=2.0, gain=0.5, base_frequency=4.0,
-                               edge0=e0, edge1=e1)
-        vpt.set_use_sparse_grid(sparse)
-        vpt.set_super_voxel_size(16 if sparse else 8)
+        c3_volume(vpt, kind)
         for q in mine:
-            vpt.set_camera_position(q["camera"])
-            vpt.set_camera_target([0.0, 0.0, 0.0])
-            vpt.set_light_property(0, "position", " ".join(f"{100.0 * v:.6f}" for v in q["sun"]))
-            vpt.set_seed_offset(q["index"])
+            c3_view(vpt, q)
             vpt.batch_add_request(torch.zeros((3, q["side"], q["side"]), dtype=torch.float32, device="cuda"),
-                                  q["spp"])
+                                  c3_spp(q, spp_divisor))
     return reqs
 
 

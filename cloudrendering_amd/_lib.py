@@ -32,6 +32,7 @@ SIGNATURES = {
     "crvpt_render_frame": (_c_int, [_p, _c_int, _c_int, _c_int, _c_int, _c_int, _p]),
     "crvpt_load_cloud_file": (_c_int, [_s]),
     "crvpt_read_xyz_file": (_c_int, [_s, _pi, _pd, _p, _c_u64]),
+    "crvpt_read_nvdb_file": (_c_int, [_s, _p, _c_u64, _pu64]),
     "crvpt_set_density_field": (_c_int, [_c_int, _c_int, _c_int, _p]),
     "crvpt_set_nanovdb_buffer": (_c_int, [_p, _c_u64]),
     "crvpt_generate_fbm_cloud": (_c_int, [_c_int, _c_int, _c_int, _c_u32, _c_int, _c_f, _c_f, _c_f, _c_f, _c_f,
@@ -41,7 +42,9 @@ SIGNATURES = {
     "crvpt_build_nanovdb_from_dense": (_c_int, [_p, _c_int, _c_int, _c_int, _c_d, _pd, _p, _c_u64, _pu64]),
     "crvpt_build_nanovdb_from_dense_typed": (_c_int, [_p, _c_int, _c_int, _c_int, _c_d, _pd, _c_int, _p, _c_u64,
                                                       _pu64]),
+    "crvpt_build_fog_volume_sphere": (_c_int, [_c_f, _p, _c_d, _c_d, _p, _c_u64, _pu64]),
     "crvpt_set_sparse_value_type": (_c_int, [_c_int]),
+    "crvpt_set_output_foreground_map": (_c_int, [_c_int]),
     "crvpt_set_use_sparse_grid": (_c_int, [_c_int]),
     "crvpt_set_grid_interpolation_type": (_c_int, [_c_int]),
     "crvpt_set_vpt_mode": (_c_int, [_c_i64]),

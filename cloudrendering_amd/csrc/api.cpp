@@ -148,6 +148,7 @@ struct Renderer {
     double znear = 1.0 / 512.0, zfar = 80.0;
     double maxGridExtent = 0.25;
     bool useSparse = false;
+    bool outputForeground = false;  // OUTPUT_FOREGROUND_MAP (VolumetricPathTracingPass.cpp:1669-1671)
     int sparseValueType = 0;  // leaf type of grids built from the dense field: 0 FP32, 1 Fp16
     int denoiser = 0;  // 0 none, 1 EAW, 2 SVGF
     // feature maps (VolumetricPathTracingPass::setUseFeatureMaps / setFeatureMapType)
@@ -338,26 +339,6 @@ int setSparseHost(Renderer& r, std::vector<uint8_t>&& buf, bool fromDense) {
     r.refTablesValid = false;
     r.resetAccumulation();
     return CRVPT_OK;
-}
-
-// NanoVDB fog-volume sphere of the default scene (Module.cpp:793-795: radius 0.25, centre 0, voxel 0.01),
-// produced as a dense field: density 1 deeper than three voxels inside, a linear ramp to 0 across the last three
-// voxels, 0 outside.
-std::vector<float> fogSphere(int& n) {
-    const double radius = 0.25, voxel = 0.01, halfWidth = 3.0;
-    n = int(std::ceil(2.0 * radius / voxel)) + 2;
-    std::vector<float> v(size_t(n) * n * n);
-    const double c = 0.5 * n;
-    for (int z = 0; z < n; z++)
-        for (int y = 0; y < n; y++)
-            for (int x = 0; x < n; x++) {
-                const double dx = (x + 0.5 - c) * voxel, dy = (y + 0.5 - c) * voxel, dz = (z + 0.5 - c) * voxel;
-                const double sdf = std::sqrt(dx * dx + dy * dy + dz * dz) - radius;
-                double d = -sdf / (halfWidth * voxel);
-                d = std::min(std::max(d, 0.0), 1.0);
-                v[(size_t(z) * n + y) * n + x] = float(d);
-            }
-    return v;
 }
 
 bool sparseActive(const Renderer& r) { return r.useSparse && r.hasSparse; }
@@ -808,9 +789,11 @@ int crvpt_initialize(void) {
         l0.lightSpace = 2;
         r->lights.push_back(l0);
         g_r = r;
-        int n = 0;
-        std::vector<float> sphere = fogSphere(n);
-        int rc = setDenseHost(*r, n, n, n, std::move(sphere));
+        // Default scene (Module.cpp:793-795): createFogVolumeSphere<float>(0.25, 0, 0.01) as the only volume; the
+        // pass keeps rendering a dense grid (useSparseGrid defaults to false, VolumetricPathTracingPass.hpp:260),
+        // which is decoded from the sphere's NanoVDB grid.
+        const float centre[3] = {0.0f, 0.0f, 0.0f};
+        int rc = setSparseHost(*r, buildFogVolumeSphere(0.25f, centre, double(0.01f), 3.0, "sphere_fog"), false);
         if (rc) return rc;
     }
     g_refCount++;
@@ -885,6 +868,8 @@ int crvpt_render_frame(float* out, int channels, int height, int width, int fram
     if (svgf) r.frameCount = 0;
     const bool denoise = r.denoiser == 1 && r.eawIterations > 0;
     const bool features = featuresActive(r);
+    const int foreground = !r.outputForeground ? 0
+                           : (r.mode == VPT_RESIDUAL_RATIO_TRACKING || r.mode == VPT_DECOMPOSITION_TRACKING) ? 2 : 1;
     // Paths per launch: every launch ends with a drain in which the pool's lanes run out of work, so fewer, larger
     // launches trace faster (16 M -> 256 M paths: +13 % on a 1024^2 x 256-spp sparse render). The records of one
     // launch take 16 bytes per path (4 GiB at 2^28), plus 48 bytes per path with feature maps (2^26 paths then).
@@ -924,7 +909,7 @@ int crvpt_render_frame(float* out, int channels, int height, int width, int fram
         const bool last = done + f == uint32_t(traceFrames);
         if (last) HIP_TRY(hipEventRecord(r.ev1, s));
         HIP_TRY(launchAccumulate(r.dPaths.as<float4>(), npix, f, r.dAcc.as<float4>(), r.frameCount + done,
-                                 last ? directOut : nullptr, channels, s));
+                                 last ? directOut : nullptr, channels, foreground, s));
         if (features)
             HIP_TRY(launchAccumulateFeatures(P, r.dPaths.as<float4>(), feat, npix, f, r.frameCount + done, svgf,
                                              facc, s));
@@ -1148,7 +1133,12 @@ int crvpt_load_cloud_file(const char* filename) {
         std::string err;
         if (!extractNanoVdbGridFromFile(bytes, grid, err)) return fail(CRVPT_ERR_IO, err);
         const int rc = setSparseHost(r, std::move(grid), false);
-        if (rc == CRVPT_OK) r.useSparse = true;  // only once the grid is in place
+        // VolumetricPathTracingPass::setCloudData (VolumetricPathTracingPass.cpp:780-795): grids of 1024^3 index
+        // entries and more switch the pass to the sparse grid, smaller ones keep the current choice (by default the
+        // dense field decoded from the grid). The reference also requires the grid to stay below 2 GiB, a limit of
+        // its GLSL accessor that this renderer does not have.
+        if (rc == CRVPT_OK && uint64_t(r.dims[0]) * uint64_t(r.dims[1]) * uint64_t(r.dims[2]) >= (1ull << 30))
+            r.useSparse = true;
         return rc;
     }
     return fail(CRVPT_ERR_UNSUPPORTED, "load_cloud_file: unsupported file format \"" + fn + "\"");
@@ -1168,6 +1158,24 @@ int crvpt_read_xyz_file(const char* filename, int32_t dims[3], double voxel_size
     if (out) {
         if (capacity < v.size()) return fail(CRVPT_ERR_INVALID_ARGUMENT, "read_xyz_file: output too small");
         std::memcpy(out, v.data(), v.size() * sizeof(float));
+    }
+    return CRVPT_OK;
+}
+
+int crvpt_read_nvdb_file(const char* filename, uint8_t* out, uint64_t capacity, uint64_t* size) {
+    if (!filename || !size) return fail(CRVPT_ERR_INVALID_ARGUMENT, "read_nvdb_file: null argument");
+    std::ifstream f(filename, std::ios::binary);
+    if (!f) return fail(CRVPT_ERR_IO, std::string("read_nvdb_file: cannot open \"") + filename + "\"");
+    std::vector<uint8_t> bytes((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    std::vector<uint8_t> grid;
+    std::string err;
+    if (!extractNanoVdbGridFromFile(bytes, grid, err)) return fail(CRVPT_ERR_IO, err);
+    NanoInfo info;
+    if (!parseNanoVdb(grid.data(), grid.size(), info, err)) return fail(CRVPT_ERR_INVALID_ARGUMENT, err);
+    *size = grid.size();
+    if (out) {
+        if (capacity < grid.size()) return fail(CRVPT_ERR_INVALID_ARGUMENT, "read_nvdb_file: output too small");
+        std::memcpy(out, grid.data(), grid.size());
     }
     return CRVPT_OK;
 }
@@ -1244,6 +1252,19 @@ int crvpt_build_nanovdb_from_dense_typed(const float* values, int nx, int ny, in
     return CRVPT_OK;
 }
 
+int crvpt_build_fog_volume_sphere(float radius, const float center[3], double voxel_size, double half_width,
+                                  uint8_t* out, uint64_t capacity, uint64_t* size) {
+    if (!center || !size || !(radius > 0.0f) || !(voxel_size > 0.0) || !(half_width > 0.0))
+        return fail(CRVPT_ERR_INVALID_ARGUMENT, "build_fog_volume_sphere: invalid arguments");
+    std::vector<uint8_t> b = buildFogVolumeSphere(radius, center, voxel_size, half_width, "sphere_fog");
+    *size = b.size();
+    if (out) {
+        if (capacity < b.size()) return fail(CRVPT_ERR_INVALID_ARGUMENT, "build_fog_volume_sphere: too small");
+        std::memcpy(out, b.data(), b.size());
+    }
+    return CRVPT_OK;
+}
+
 int crvpt_set_sparse_value_type(int value_type) {
     if (int rc = requireInit()) return rc;
     if (value_type != 0 && value_type != 1)
@@ -1275,6 +1296,16 @@ int crvpt_get_nanovdb_buffer(uint8_t* out, uint64_t capacity, uint64_t* size) {
     if (out) {
         if (capacity < r.nvdbHost.size()) return fail(CRVPT_ERR_INVALID_ARGUMENT, "get_nanovdb_buffer: too small");
         std::memcpy(out, r.nvdbHost.data(), r.nvdbHost.size());
+    }
+    return CRVPT_OK;
+}
+
+int crvpt_set_output_foreground_map(int output_foreground_map) {
+    if (int rc = requireInit()) return rc;
+    const bool v = output_foreground_map != 0;
+    if (v != g_r->outputForeground) {
+        g_r->outputForeground = v;
+        g_r->resetAccumulation();
     }
     return CRVPT_OK;
 }
@@ -1993,6 +2024,8 @@ int crvpt_batch_add_request(float* out, int channels, int height, int width, int
     Batch& b = *g_batch;
     if (b.mode >= 0 && b.mode != r.mode)
         return fail(CRVPT_ERR_INVALID_ARGUMENT, "batch_add_request: all requests of a batch use one estimator");
+    if (r.outputForeground)
+        return fail(CRVPT_ERR_UNSUPPORTED, "batch_add_request: the foreground map is written by render_frame only");
     if (int rc = prepareScene(r)) return rc;
     HIP_TRY(hipStreamSynchronize(r.stream));
     VolumeKey key;

@@ -275,6 +275,11 @@ bool extractNanoVdbGridFromFile(const std::vector<uint8_t>& file, std::vector<ui
 
 std::vector<uint8_t> buildNanoVdbFromDense(const float* values, const int dims[3], double dx, const double p0[3],
                                            const std::string& name, int valueType) {
+    return buildNanoVdb(values, dims, dx, p0, name, valueType, DenseBuildOptions());
+}
+
+std::vector<uint8_t> buildNanoVdb(const float* values, const int dims[3], double dx, const double p0[3],
+                                  const std::string& name, int valueType, const DenseBuildOptions& opt) {
     const bool fp16 = valueType == 1;
     const uint32_t leafSize = fp16 ? kLeafSizeFp16 : kLeafSize;
     // 1. Leaves: every 8^3 block with at least one non-background voxel (GridBuilder.h:251-300); a fully active
@@ -293,7 +298,7 @@ std::vector<uint8_t> buildNanoVdbFromDense(const float* values, const int dims[3
                 for (int by = 0; by < ly; by++)
                     for (int bx = 0; bx < lx; bx++) {
                         Leaf L;
-                        L.origin = {bx * 8, by * 8, bz * 8};
+                        L.origin = {opt.ijk0[0] + bx * 8, opt.ijk0[1] + by * 8, opt.ijk0[2] + bz * 8};
                         std::fill(std::begin(L.values), std::end(L.values), 0.0f);
                         std::fill(std::begin(L.mask), std::end(L.mask), 0ull);
                         int active = 0;
@@ -302,8 +307,9 @@ std::vector<uint8_t> buildNanoVdbFromDense(const float* values, const int dims[3
                                 for (int z = 0; z < 8; z++) {
                                     const int gx = bx * 8 + x, gy = by * 8 + y, gz = bz * 8 + z;
                                     if (gx >= dims[0] || gy >= dims[1] || gz >= dims[2]) continue;
-                                    const float v = values[size_t(gx) + (size_t(gy) + size_t(gz) * dims[1]) * dims[0]];
-                                    if (v == 0.0f) continue;
+                                    const size_t idx = size_t(gx) + (size_t(gy) + size_t(gz) * dims[1]) * dims[0];
+                                    const float v = values[idx];
+                                    if (opt.active ? !opt.active[idx] : v == 0.0f) continue;
                                     const int n = x << 6 | y << 3 | z;
                                     L.values[n] = v;
                                     L.mask[n >> 6] |= 1ull << (n & 63);
@@ -634,7 +640,7 @@ std::vector<uint8_t> buildNanoVdbFromDense(const float* values, const int dims[3
         putCoord(b, rootOff + 12, rootStats.bmax);
     }
     put<uint32_t>(b, rootOff + 24, uint32_t(tiles));
-    put<float>(b, rootOff + 28, 0.0f);
+    put<float>(b, rootOff + 28, opt.background);
     if (tiles == 0) {
         put<float>(b, rootOff + 32, 0.0f);
         put<float>(b, rootOff + 36, 0.0f);
@@ -657,8 +663,8 @@ std::vector<uint8_t> buildNanoVdbFromDense(const float* values, const int dims[3
     put<uint32_t>(b, tree + 48, tileCount[1]);
     put<uint32_t>(b, tree + 52, tileCount[2]);
     put<uint64_t>(b, tree + 56, voxelCount);
-    // 8. Grid (GridBuilder.h processGrid; GridStats.h:531-570). The reference passes GridClass::FogVolume in the
-    //    oracle slot of getHandle (CloudData.cpp:1398-1400), so the class stays the builder default, Unknown.
+    // 8. Grid (GridBuilder.h processGrid; GridStats.h:531-570). CloudData passes GridClass::FogVolume in the
+    //    oracle slot of getHandle (CloudData.cpp:1398-1400), so its grids keep the builder default class, Unknown.
     put<uint64_t>(b, 0, kMagic);
     put<uint32_t>(b, 16, kVersion);
     put<uint32_t>(b, 20, rootEmpty ? (4u | 8u | 16u | 32u) : (2u | 4u | 8u | 16u | 32u));
@@ -711,7 +717,7 @@ std::vector<uint8_t> buildNanoVdbFromDense(const float* values, const int dims[3
         applyMap(dx, p0, zero, m0);
         for (int a = 0; a < 3; a++) put<double>(b, 608 + 8 * a, m1[a] - m0[a]);
     }
-    put<uint32_t>(b, 632, 0u);  // GridClass::Unknown
+    put<uint32_t>(b, 632, opt.gridClass);
     put<uint32_t>(b, 636, fp16 ? kGridTypeFp16 : kGridTypeFloat);
     put<int64_t>(b, 640, 0);
     put<uint32_t>(b, 648, 0);
@@ -720,6 +726,81 @@ std::vector<uint8_t> buildNanoVdbFromDense(const float* values, const int dims[3
     const uint32_t head = crc32(b.data() + 16, size_t(rootOff + kRootSize + kTileSize * tiles) - 16);
     put<uint64_t>(b, 8, (uint64_t(0xFFFFFFFFu) << 32) | head);
     return b;
+}
+
+std::vector<uint8_t> buildFogVolumeSphere(float radius, const float center[3], double voxelSize, double halfWidth,
+                                          const std::string& name) {
+    // initSphere (Primitives.h:494-541): radius and centre in voxel units, float arithmetic as the reference's
+    // ValueT = float template; the origin of the index space is world 0.
+    const float background = float(halfWidth * voxelSize);
+    DenseBuildOptions opt;
+    opt.background = background;
+    opt.gridClass = 2;  // sdfToFog leaves the grid a FogVolume
+    const float r0 = float(double(radius) / voxelSize);
+    const float rmax = float(double(r0) + halfWidth);
+    float c[3];
+    for (int a = 0; a < 3; a++) c[a] = float(double(center[a]) / voxelSize);
+    int lo[3], hi[3];
+    for (int a = 0; a < 3; a++) {
+        lo[a] = int(std::floor(c[a] - rmax));
+        hi[a] = int(std::ceil(c[a] + rmax));
+    }
+    const double p0[3] = {0.0, 0.0, 0.0};
+    if (r0 < 1.5f) {  // below the Nyquist frequency: an empty grid
+        const int dims[3] = {0, 0, 0};
+        return buildNanoVdb(nullptr, dims, voxelSize, p0, name, 0, opt);
+    }
+    // Dense staging over the 8-aligned index box that holds [lo, hi].
+    int dims[3];
+    for (int a = 0; a < 3; a++) {
+        opt.ijk0[a] = lo[a] >= 0 ? lo[a] & ~7 : -((-lo[a] + 7) & ~7);
+        dims[a] = ((hi[a] + 1 - opt.ijk0[a]) + 7) & ~7;
+    }
+    const size_t n = size_t(dims[0]) * dims[1] * dims[2];
+    std::vector<float> values(n, 0.0f);
+    std::vector<uint8_t> active(n, 0);
+    std::vector<uint8_t> band(n, 0);
+    auto at = [&](int i, int j, int k) {
+        return size_t(i - opt.ijk0[0]) + (size_t(j - opt.ijk0[1]) + size_t(k - opt.ijk0[2]) * dims[1]) * dims[0];
+    };
+    auto dist = [&](int i, int j, int k) {
+        const float x2 = (float(i) - c[0]) * (float(i) - c[0]);
+        const float x2y2 = (float(j) - c[1]) * (float(j) - c[1]) + x2;
+        return std::sqrt(x2y2 + (float(k) - c[2]) * (float(k) - c[2])) - r0;
+    };
+    // Narrow band with the reference's leapfrogging along k (voxels it skips lie outside the band).
+    for (int i = lo[0]; i <= hi[0]; i++)
+        for (int j = lo[1]; j <= hi[1]; j++)
+            for (int k = lo[2], m = 1; k <= hi[2]; k += m) {
+                m = 1;
+                const float v = dist(i, j, k);
+                const float d = v < 0.0f ? -v : v;
+                if (double(d) < halfWidth) {
+                    const size_t q = at(i, j, k);
+                    values[q] = float(voxelSize * double(v));  // distance in world units
+                    band[q] = 1;
+                } else {
+                    m += int(std::floor(double(d) - halfWidth));
+                }
+            }
+    // sdfToFog (GridBuilder.h:613-626): v > 0 -> 0 (inactive); else v > -background ? v / -background : 1. Voxels
+    // off the band take the inside (-background) or outside (+background) value of the signed flood fill, which for
+    // the sphere is the sign of the distance.
+    const float dneg = -background, w = 1.0f / dneg;
+    for (int k = opt.ijk0[2]; k < opt.ijk0[2] + dims[2]; k++)
+        for (int j = opt.ijk0[1]; j < opt.ijk0[1] + dims[1]; j++)
+            for (int i = opt.ijk0[0]; i < opt.ijk0[0] + dims[0]; i++) {
+                const size_t q = at(i, j, k);
+                float v = band[q] ? values[q] : (dist(i, j, k) < 0.0f ? dneg : background);
+                if (v > 0.0f) {
+                    values[q] = 0.0f;
+                    continue;
+                }
+                values[q] = v > dneg ? v * w : 1.0f;
+                active[q] = 1;
+            }
+    opt.active = active.data();
+    return buildNanoVdb(values.data(), dims, voxelSize, p0, name, 0, opt);
 }
 
 }  // namespace crvpt

@@ -47,4 +47,23 @@ bool extractNanoVdbGridFromFile(const std::vector<uint8_t>& file, std::vector<ui
 std::vector<uint8_t> buildNanoVdbFromDense(const float* values, const int dims[3], double dx, const double p0[3],
                                            const std::string& name, int valueType = 0);
 
+// Options of the general writer behind buildNanoVdbFromDense.
+struct DenseBuildOptions {
+    const uint8_t* active = nullptr;  // per-voxel active flags (nullptr: a voxel is active where its value != 0)
+    int32_t ijk0[3] = {0, 0, 0};      // index coordinate of values[0]; multiples of 8
+    float background = 0.0f;          // root background value
+    uint32_t gridClass = 0;           // nanovdb::GridClass (0 Unknown, 2 FogVolume)
+};
+std::vector<uint8_t> buildNanoVdb(const float* values, const int dims[3], double dx, const double p0[3],
+                                  const std::string& name, int valueType, const DenseBuildOptions& opt);
+
+// nanovdb::createFogVolumeSphere<float>(radius, center, voxelSize, halfWidth) (util/Primitives.h:84-95, 893-914):
+// the level-set sphere of initSphere (distances in voxel units from the centre, a narrow band of halfWidth voxels,
+// leapfrogging outside it) converted to a fog volume by GridBuilder::sdfToFog (GridBuilder.h:613-680): band voxels
+// at distance v <= 0 hold v * voxelSize / (-halfWidth * voxelSize), voxels beyond the band inside the sphere 1
+// (active), everything outside 0 (inactive). The root background stays the level set's, halfWidth * voxelSize.
+// The reference module's default scene is createFogVolumeSphere<float>(0.25, 0, 0.01) (Module.cpp:793-795).
+std::vector<uint8_t> buildFogVolumeSphere(float radius, const float center[3], double voxelSize, double halfWidth,
+                                          const std::string& name);
+
 }  // namespace crvpt

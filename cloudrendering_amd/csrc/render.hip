@@ -208,30 +208,36 @@ CRVPT_DEV float rrtEstimator(const SceneParams& P, const Grid& grid, Rng& rng, V
 // Accumulation: running mean over all frames since the last reset (Clouds.glsl:172-179), summed per pixel in
 // sample order, then written as [C, H, W] float32 (alpha = 1) when requested.
 // ------------------------------------------------------------------------------------------------------------
+// foreground: 0 plain running mean (alpha 1); otherwise the OUTPUT_FOREGROUND_MAP store of Clouds.glsl:181-191,
+// the running mean of (radiance, 1) over samples whose first event has a value and (0, 0, 0, 0) over the others:
+// 1 reads that flag from the record's w, 2 is the trackers that never record a first event (residual ratio and
+// decomposition tracking), whose every sample stores 0.
 __global__ void k_accumulate(const float4* __restrict__ paths, uint32_t npix, uint32_t frames,
                              float4* __restrict__ acc, uint32_t prevFrames, float* __restrict__ outCHW,
-                             int channels) {
+                             int channels, int foreground) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npix) return;
-    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f, sw = 0.0f;
     for (uint32_t f = 0; f < frames; f++) {
         const float4 v = paths[size_t(f) * npix + p];
-        sx += v.x;
-        sy += v.y;
-        sz += v.z;
+        const float keep = foreground == 0 ? 1.0f : (foreground == 1 ? v.w : 0.0f);
+        sx += v.x * keep;
+        sy += v.y * keep;
+        sz += v.z * keep;
+        sw += keep;
     }
-    float4 a = prevFrames == 0 ? make_float4(0.0f, 0.0f, 0.0f, 1.0f) : acc[p];
+    float4 a = prevFrames == 0 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : acc[p];
     const float inv = 1.0f / float(prevFrames + frames);
     a.x = (a.x * float(prevFrames) + sx) * inv;
     a.y = (a.y * float(prevFrames) + sy) * inv;
     a.z = (a.z * float(prevFrames) + sz) * inv;
-    a.w = 1.0f;
+    a.w = foreground == 0 ? 1.0f : (a.w * float(prevFrames) + sw) * inv;
     acc[p] = a;
     if (outCHW) {
         outCHW[p] = a.x;
         outCHW[npix + p] = a.y;
         outCHW[2 * size_t(npix) + p] = a.z;
-        if (channels == 4) outCHW[3 * size_t(npix) + p] = 1.0f;
+        if (channels == 4) outCHW[3 * size_t(npix) + p] = a.w;
     }
 }
 
@@ -795,9 +801,9 @@ hipError_t launchRender(const SceneParams& P, const DenseGrid* dense, const Nano
 }
 
 hipError_t launchAccumulate(const float4* paths, uint32_t npix, uint32_t frames, float4* acc, uint32_t prevFrames,
-                            float* outCHW, int channels, hipStream_t stream) {
+                            float* outCHW, int channels, int foreground, hipStream_t stream) {
     hipLaunchKernelGGL(k_accumulate, dim3((npix + 255) / 256), dim3(256), 0, stream, paths, npix, frames, acc,
-                       prevFrames, outCHW, channels);
+                       prevFrames, outCHW, channels, foreground);
     return hipGetLastError();
 }
 

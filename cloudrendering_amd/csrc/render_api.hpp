@@ -88,8 +88,10 @@ hipError_t launchKpnAdam(float* param, const float* grad, float* m1, float* m2, 
 hipError_t launchKpnSwizzle(const float* w, void* fwd, void* bwd, int layers, hipStream_t stream);
 hipError_t launchKpnZero(float* p, size_t n, hipStream_t stream);
 int batchOccupancyBlocksPerCU(int mode, int blockSize);
+// foreground: 0 plain running mean, 1 foreground map from the records' first-event flags, 2 foreground map of a
+// tracker without first events (all zero).
 hipError_t launchAccumulate(const float4* paths, uint32_t npix, uint32_t frames, float4* acc, uint32_t prevFrames,
-                            float* outCHW, int channels, hipStream_t stream);
+                            float* outCHW, int channels, int foreground, hipStream_t stream);
 hipError_t launchWriteCHW(const float4* img, uint32_t npix, float* outCHW, int channels, hipStream_t stream);
 // noAccumulation: the reference's DISABLE_ACCUMULATION maps (the depth of a ray without an event is the far plane).
 hipError_t launchAccumulateFeatures(const SceneParams& P, const float4* paths, const float4* feat, uint32_t npix,

@@ -281,6 +281,10 @@ def set_denoiser_property(key: str, value: str) -> None:
     call("crvpt_set_denoiser_property", key.encode(), str(value).encode())
 
 
+def set_output_foreground_map(_shall_output_foreground_map: bool) -> None:
+    call("crvpt_set_output_foreground_map", int(bool(_shall_output_foreground_map)))
+
+
 def set_use_lights(_use_lights: bool) -> None:
     call("crvpt_set_use_lights", int(bool(_use_lights)))
 
@@ -568,6 +572,19 @@ def build_nanovdb_from_dense(density_zyx, dx: float, origin=(0.0, 0.0, 0.0), val
     out = np.empty(size.value, dtype=np.uint8)
     call("crvpt_build_nanovdb_from_dense_typed", a.ctypes.data_as(ctypes.c_void_p), nx, ny, nz, float(dx), o,
          int(value_type), out.ctypes.data_as(ctypes.c_void_p), out.size, ctypes.byref(size))
+    return out.tobytes()
+
+
+def build_fog_volume_sphere(radius: float = 0.25, center=(0.0, 0.0, 0.0), voxel_size: float = 0.01,
+                            half_width: float = 3.0) -> bytes:
+    """nanovdb::createFogVolumeSphere<float> (the default scene's grid, Module.cpp:793-795); no GPU needed."""
+    c = (ctypes.c_float * 3)(*[float(v) for v in center])
+    vs = float(np.float32(voxel_size))  # the reference passes the voxel size as a float
+    size = ctypes.c_uint64()
+    call("crvpt_build_fog_volume_sphere", float(radius), c, vs, float(half_width), None, 0, ctypes.byref(size))
+    out = np.empty(size.value, dtype=np.uint8)
+    call("crvpt_build_fog_volume_sphere", float(radius), c, vs, float(half_width), out.ctypes.data_as(ctypes.c_void_p),
+         out.size, ctypes.byref(size))
     return out.tobytes()
 
 

@@ -51,6 +51,10 @@ int crvpt_load_cloud_file(const char* filename);
  * (CloudData::loadFromXyzFile, CloudData.cpp:236-316): grid sizes, voxel sizes, and with out != NULL the field x
  * fastest (transposed from the file's z-fastest order) and min-max normalised when max - min > 1e-6. */
 int crvpt_read_xyz_file(const char* filename, int32_t dims[3], double voxel_size[3], float* out, uint64_t capacity);
+/* No reference counterpart, host only (no GPU needed): the .nvdb read that load_cloud_file applies
+ * (CloudData::loadFromNvdbFile, CloudData.cpp:1360-1365: io::readGrid of the file's first grid, uncompressed): the
+ * grid's bytes, validated as a NanoVDB 32.3 FP32 or Fp16 grid. With out == NULL only *size is written. */
+int crvpt_read_nvdb_file(const char* filename, uint8_t* out, uint64_t capacity, uint64_t* size);
 /* CloudData::setDensityField (CloudData.cpp:147-167): dense grid, x fastest, size nx*ny*nz (copied). */
 int crvpt_set_density_field(int nx, int ny, int nz, const float* values);
 /* CloudData::setNanoVdbGridHandle (CloudData.cpp:169-172): a NanoVDB 32.3 FP32 grid buffer (copied). */
@@ -71,6 +75,11 @@ int crvpt_build_nanovdb_from_dense(const float* values, int nx, int ny, int nz, 
 int crvpt_build_nanovdb_from_dense_typed(const float* values, int nx, int ny, int nz, double dx,
                                          const double origin[3], int value_type, uint8_t* out, uint64_t capacity,
                                          uint64_t* size);
+/* Host only (no GPU needed): nanovdb::createFogVolumeSphere<float>(radius, center, voxel_size, half_width)
+ * (src/nanovdb/util/Primitives.h:84-95, 893-914), the grid of the module's default scene
+ * (Module.cpp:793-795: radius 0.25, centre 0, voxel 0.01, half width 3). With out == NULL only *size is written. */
+int crvpt_build_fog_volume_sphere(float radius, const float center[3], double voxel_size, double half_width,
+                                  uint8_t* out, uint64_t capacity, uint64_t* size);
 /* No reference counterpart: leaf value type of the NanoVDB grid that set_use_sparse_grid builds from a dense field
  * (0 FP32, the reference's; 1 Fp16: half the leaf memory, values quantised per leaf). Loaded .nvdb files and
  * crvpt_set_nanovdb_buffer grids may be either type. */
@@ -82,6 +91,12 @@ int crvpt_get_nanovdb_buffer(uint8_t* out, uint64_t capacity, uint64_t* size);
 /* VolumetricPathTracingPass::setUseSparseGrid / setSparseGridInterpolationType (VolumetricPathTracingPass.cpp:
  * 863-875); interpolation: 0 nearest, 1 stochastic, 2 trilinear (RenderSettings.hpp:52-56). */
 int crvpt_set_use_sparse_grid(int use_sparse);
+/* Module.cpp:1074-1076 set_output_foreground_map (VolumetricPathTracingPass::setOutputForegroundMap,
+ * VolumetricPathTracingPass.cpp:1111-1126; the OUTPUT_FOREGROUND_MAP store of Clouds.glsl:181-191): render_frame
+ * returns the running mean of (radiance, 1) over samples whose first scattering or absorption event has a value and
+ * (0, 0, 0, 0) over the others, so alpha is the foreground coverage. Residual ratio and decomposition tracking record
+ * no first event, so their foreground map is 0. Changing it resets the accumulation. */
+int crvpt_set_output_foreground_map(int output_foreground_map);
 int crvpt_set_grid_interpolation_type(int type);
 
 /* Module.cpp set_vpt_mode / set_vpt_mode_from_name (VptMode order and names of RenderSettings.hpp:32-45). */

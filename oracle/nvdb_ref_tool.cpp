@@ -8,6 +8,10 @@
  *   nvdb_ref ijk    <in.grid> <ijk.i32> <out.f32>                PNanoVDB lookups at integer coordinates
  *   nvdb_ref ijk16  <in.grid> <ijk.i32> <out.f32>                the same on an Fp16 grid
  *   nvdb_ref world  <in.grid> <xyz.f32> <out.f32> <out_ijk.i32>  nearest world-space lookups
+ *   nvdb_ref nvdbfile <in.grid> <out.nvdb>                      io::writeGrid of a grid (the .nvdb file format that
+ *                                                                CloudData::loadFromNvdbFile reads, CloudData.cpp:1360)
+ *   nvdb_ref fogsphere <radius> <voxel> <out.grid>               createFogVolumeSphere<float>(radius, 0, voxel), the
+ *                                                                default scene of Module.cpp:793-795
  */
 #include <cstdint>
 #include <cstdio>
@@ -17,6 +21,8 @@
 #include <vector>
 
 #include "nanovdb/util/GridBuilder.h"
+#include "nanovdb/util/IO.h"
+#include "nanovdb/util/Primitives.h"
 
 extern "C" void ref_lookup_ijk(const uint32_t* words, int n, const int32_t* ijk, float* out);
 extern "C" void ref_lookup_ijk16(const uint32_t* words, int n, const int32_t* ijk, float* out);
@@ -92,6 +98,20 @@ int main(int argc, char** argv) {
         writeAll(argv[5], ijk.data(), ijk.size());
         return 0;
     }
-    std::fprintf(stderr, "usage: nvdb_ref build|ijk|world ...\n");
+    if (cmd == "nvdbfile" && argc == 4) {
+        std::vector<uint8_t> grid = readAll<uint8_t>(argv[2]);
+        auto buffer = nanovdb::HostBuffer::create(grid.size());
+        std::memcpy(buffer.data(), grid.data(), grid.size());
+        nanovdb::GridHandle<nanovdb::HostBuffer> handle(std::move(buffer));
+        nanovdb::io::writeGrid(argv[3], handle);
+        return 0;
+    }
+    if (cmd == "fogsphere" && argc == 5) {
+        const float radius = float(std::atof(argv[2])), voxel = float(std::atof(argv[3]));
+        auto handle = nanovdb::createFogVolumeSphere<float>(radius, nanovdb::Vec3<float>(0), voxel);
+        writeAll(argv[4], handle.data(), handle.size());
+        return 0;
+    }
+    std::fprintf(stderr, "usage: nvdb_ref build|build16|ijk|ijk16|world|nvdbfile|fogsphere ...\n");
     return 2;
 }

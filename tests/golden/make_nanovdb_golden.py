@@ -4,7 +4,12 @@ tree). Each fixture holds the dense input, the reference grid bytes, integer-coo
 world-space lookups (with the voxel each one selected). The nanovdb_fp16_*.npz fixtures hold grids of
 GridBuilder<float, Fp16> (16-bit quantised leaves) and their integer-coordinate lookups.
 
-Run from the repository root: python tests/golden/make_nanovdb_golden.py
+The fog_sphere_default.npz fixture holds the reference module's default scene grid (createFogVolumeSphere) and its
+lookups.
+
+nvdb_file_block20.npz holds the block20 grid as a .nvdb file written by the reference's io::writeGrid.
+
+Run from the repository root: python tests/golden/make_nanovdb_golden.py [fog_sphere | nvdb_file]
 """
 import os
 import subprocess
@@ -89,5 +94,53 @@ def main():
         print("fp16", name, "grid bytes", grid.size, "active lookups", int((vals != 0).sum()))
 
 
+def fog_sphere():
+    """fog_sphere_default.npz: the reference module's default scene grid, nanovdb::createFogVolumeSphere<float>(0.25,
+    0, 0.01) (Module.cpp:793-795), and reference lookups over its index box and a margin (all voxels on the three
+    axes and the diagonal, plus random voxels)."""
+    if not os.path.exists(TOOL):
+        subprocess.run(["make", "-s", "-C", os.path.join(REPO, "oracle"), "ref"], check=True)
+    rng = np.random.default_rng(2026)
+    out_dir = os.path.dirname(os.path.abspath(__file__))
+    with tempfile.TemporaryDirectory() as tmp:
+        f = lambda n: os.path.join(tmp, n)  # noqa: E731
+        run("fogsphere", "0.25", "0.01", f("g.bin"))
+        grid = np.fromfile(f("g.bin"), np.uint8)
+        line = np.arange(-31, 32)
+        zero = np.zeros_like(line)
+        ijk = np.concatenate([
+            np.stack([line, zero, zero], 1), np.stack([zero, line, zero], 1), np.stack([zero, zero, line], 1),
+            np.stack([line, line, line], 1), rng.integers(-31, 32, size=(30000, 3)),
+        ]).astype(np.int32)
+        ijk.tofile(f("ijk.i32"))
+        run("ijk", f("g.bin"), f("ijk.i32"), f("v.f32"))
+        vals = np.fromfile(f("v.f32"), np.float32)
+    np.savez_compressed(os.path.join(out_dir, "fog_sphere_default.npz"), grid=grid, ijk=ijk, values=vals)
+    print("fog sphere grid bytes", grid.size, "nonzero lookups", int((vals != 0).sum()))
+
+
+def nvdb_file():
+    """nvdb_file_block20.npz: the block20 fixture's grid written as a .nvdb file by the reference's
+    nanovdb::io::writeGrid (uncompressed), the format CloudData::loadFromNvdbFile reads (CloudData.cpp:1360-1365)."""
+    if not os.path.exists(TOOL):
+        subprocess.run(["make", "-s", "-C", os.path.join(REPO, "oracle"), "ref"], check=True)
+    out_dir = os.path.dirname(os.path.abspath(__file__))
+    grid = np.load(os.path.join(out_dir, "nanovdb_block20.npz"))["grid"]
+    with tempfile.TemporaryDirectory() as tmp:
+        g, f = os.path.join(tmp, "g.bin"), os.path.join(tmp, "g.nvdb")
+        grid.tofile(g)
+        run("nvdbfile", g, f)
+        file_bytes = np.fromfile(f, np.uint8)
+    np.savez_compressed(os.path.join(out_dir, "nvdb_file_block20.npz"), nvdb=file_bytes)
+    print(".nvdb file bytes", file_bytes.size)
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["fog_sphere"]:
+        fog_sphere()
+    elif sys.argv[1:] == ["nvdb_file"]:
+        nvdb_file()
+    else:
+        main()
+        fog_sphere()
+        nvdb_file()

@@ -101,3 +101,44 @@ def test_xyz_truncated_header(tmp_path):
     L = _reader()
     with pytest.raises(L.CrvptError, match="header"):
         L.call("crvpt_read_xyz_file", p.encode(), (ctypes.c_int * 3)(), (ctypes.c_double * 3)(), None, 0)
+
+
+# .nvdb (CloudData::loadFromNvdbFile, CloudData.cpp:1360-1365: nanovdb::io::readGrid of the file's first grid). The
+# fixture is a file the reference's own nanovdb::io::writeGrid wrote (tests/golden/make_nanovdb_golden.py).
+GOLDEN_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _read_nvdb(path):
+    L = _reader()
+    size = ctypes.c_uint64()
+    L.call("crvpt_read_nvdb_file", path.encode(), None, 0, ctypes.byref(size))
+    out = np.zeros(size.value, np.uint8)
+    L.call("crvpt_read_nvdb_file", path.encode(), out.ctypes.data, out.size, ctypes.byref(size))
+    return out
+
+
+def test_nvdb_file_yields_the_reference_grid(tmp_path):
+    f = np.load(os.path.join(GOLDEN_DIR, "nvdb_file_block20.npz"))["nvdb"]
+    grid = np.load(os.path.join(GOLDEN_DIR, "nanovdb_block20.npz"))["grid"]
+    p = str(tmp_path / "b.nvdb")
+    f.tofile(p)
+    got = _read_nvdb(p)
+    assert got.size == grid.size
+    np.testing.assert_array_equal(got, grid)
+
+
+@pytest.mark.parametrize("damage", ["truncated_grid", "truncated_meta", "codec", "magic"])
+def test_nvdb_file_rejects_damaged_files(tmp_path, damage):
+    f = np.load(os.path.join(GOLDEN_DIR, "nvdb_file_block20.npz"))["nvdb"].copy()
+    if damage == "truncated_grid":
+        f = f[:-100]
+    elif damage == "truncated_meta":
+        f = f[:100]
+    elif damage == "codec":
+        f[14] = 1  # ZIP
+    else:
+        f[0] ^= 0xFF
+    p = str(tmp_path / "d.nvdb")
+    f.tofile(p)
+    with pytest.raises(RuntimeError):
+        _read_nvdb(p)

@@ -4,12 +4,15 @@ Volume: NanoVDB FP32 grid of a 512^3 seeded fBm (seed 0, smoothstep(0.645, 0.795
 extinction 1024, stochastic filtering (its expectation is the trilinear field, which the FP64 integrator evaluates
 exactly per voxel with 2-point Gauss-Legendre). Rays: origins uniform on a sphere of radius 1.5 x the box's
 half-diagonal around the box centre, directions towards uniform points in the box, seed 1. One estimate per ray:
-the mean error over all rays, and within bins of the exact transmittance, must be within 4.5 standard errors.
+the mean error over all rays, and within each of the six bins of the exact transmittance, must be within zcrit(7) =
+3.78 standard errors (the family-wise 3-sigma level of the seven comparisons).
 """
 import math
 
 import numpy as np
 import pytest
+
+from _stats import zcrit
 
 pytestmark = pytest.mark.gpu
 
@@ -65,7 +68,9 @@ def test_c2_transmittance_1e6_rays_vs_fp64(vpt, c2, estimator):
     diff = mean.astype(np.float64) - exact
     se = diff.std() / math.sqrt(len(diff))
     z = diff.mean() / max(se, 1e-12)
-    assert abs(z) < 4.5, f"{estimator}: mean error {diff.mean():.3e}, z {z:.2f}"
+    zc = zcrit(7)
+    print(f"c2 {estimator}: mean error {diff.mean():.3e} z {z:.2f} (zcrit {zc:.2f})", flush=True)
+    assert abs(z) <= zc, f"{estimator}: mean error {diff.mean():.3e}, z {z:.2f}"
     edges = [0.0, 0.05, 0.25, 0.5, 0.75, 0.95, 1.0 + 1e-9]
     for lo, hi in zip(edges[:-1], edges[1:]):
         sel = (exact >= lo) & (exact < hi)
@@ -73,4 +78,5 @@ def test_c2_transmittance_1e6_rays_vs_fp64(vpt, c2, estimator):
             continue
         db = diff[sel]
         zb = db.mean() / max(db.std() / math.sqrt(len(db)), 1e-12)
-        assert abs(zb) < 4.5, f"{estimator}: bin [{lo}, {hi}) n={sel.sum()} mean error {db.mean():.3e} z {zb:.2f}"
+        print(f"  bin [{lo}, {hi}) n={sel.sum()}: z {zb:.2f}", flush=True)
+        assert abs(zb) <= zc, f"{estimator}: bin [{lo}, {hi}) n={sel.sum()} mean error {db.mean():.3e} z {zb:.2f}"

@@ -1,10 +1,12 @@
 """GPU checks of the feature maps (Clouds.glsl:146-383) against the oracle's restatement: exact relations between the
-maps of one first event, an opaque cube whose answers are geometric, and image means of every map within 4 sigma of
-the oracle's (eight seed-offset batches on each side)."""
+maps of one first event, an opaque cube whose answers are geometric, and image means of every map against the
+oracle's (24 seed-offset batches on each side; Student's t at the Bonferroni-corrected 3-sigma level)."""
 import math
 
 import numpy as np
 import pytest
+
+from _stats import tcrit
 
 pytestmark = pytest.mark.gpu
 
@@ -101,7 +103,7 @@ def test_map_means_vs_oracle(vpt, oracle, name, mode):
     d = oracle.fbm_cloud(24, 20, 16, seed=33)
     _setup(vpt, d, mode)
     gpu = {k: [] for k in ORACLE}
-    nb = 8
+    nb = 24
     for b in range(nb):
         vpt.set_seed_offset(1000003 * (b + 1))
         m = _maps(vpt, 32, names=list(ORACLE))
@@ -114,6 +116,8 @@ def test_map_means_vs_oracle(vpt, oracle, name, mode):
         o = sc.render_features(32, names=list(ORACLE.values()))
         for k, on in ORACLE.items():
             ref[k].append(o[on].reshape(-1, o[on].shape[-1]).mean(0))
+    # one family over every channel of every map; batch standard errors -> Student's t with Welch's dof
+    comps = sum(np.array(ref[k]).shape[1] for k in ORACLE)
     for k in ORACLE:
         g = np.array(gpu[k], np.float64)
         r = np.array(ref[k], np.float64)
@@ -121,7 +125,12 @@ def test_map_means_vs_oracle(vpt, oracle, name, mode):
         g = g[:, :c]
         mg, sg = g.mean(0), g.std(0, ddof=1) / math.sqrt(nb)
         mr, sr = r.mean(0), r.std(0, ddof=1) / math.sqrt(nb)
-        tol = 4.0 * np.sqrt(sg ** 2 + sr ** 2) + 1e-4 * (1 + np.abs(mr))
+        se2 = sg ** 2 + sr ** 2
+        dof = se2 ** 2 / np.maximum(sg ** 4 / (nb - 1) + sr ** 4 / (nb - 1), 1e-300)
+        crit = np.array([tcrit(comps, max(float(v), 1.0)) for v in dof])
+        z = (mg - mr) / np.maximum(np.sqrt(se2), 1e-30)
+        print(f"{name}/{k}: z {np.round(z, 2)} (t crit {np.round(crit, 2)})", flush=True)
+        tol = crit * np.sqrt(se2) + 1e-6 * (1 + np.abs(mr))
         assert np.all(np.abs(mg - mr) <= tol), f"{name}/{k}: gpu {mg} oracle {mr} tol {tol}"
 
 

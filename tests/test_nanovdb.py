@@ -117,3 +117,41 @@ def test_fp16_writer_bit_exact_vs_reference_tool_random(seed):
     mine = _build(dense, dx, origin, value_type=1)
     assert mine.size == ref.size
     assert np.array_equal(mine, ref), np.nonzero(mine != ref)[0][:16]
+
+
+def test_fog_sphere_bit_exact_vs_reference_default_scene():
+    """The default scene's grid (Module.cpp:793-795, nanovdb::createFogVolumeSphere<float>(0.25, 0, 0.01)) equals the
+    reference's own createFogVolumeSphere output byte for byte (tests/golden/fog_sphere_default.npz)."""
+    from cloudrendering_amd import vpt
+
+    g = np.load(os.path.join(REPO, "tests", "golden", "fog_sphere_default.npz"))
+    mine = np.frombuffer(vpt.build_fog_volume_sphere(0.25, (0.0, 0.0, 0.0), 0.01), np.uint8)
+    ref = g["grid"]
+    assert mine.size == ref.size
+    diff = np.nonzero(mine != ref)[0]
+    assert diff.size == 0, f"first differing byte offsets {diff[:16]}"
+
+
+def test_fog_sphere_golden_lookups_are_the_fog_profile():
+    """The golden lookups describe the fog sphere: 1 deeper than three voxels inside radius 25 voxels, the linear
+    ramp -d / 3 across the band, 0 outside (pins that the fixture's lookups and the geometry agree)."""
+    g = np.load(os.path.join(REPO, "tests", "golden", "fog_sphere_default.npz"))
+    ijk, vals = g["ijk"].astype(np.float64), g["values"].astype(np.float64)
+    d = np.sqrt((ijk ** 2).sum(1)) - 25.0
+    expect = np.clip(-d / 3.0, 0.0, 1.0)
+    np.testing.assert_allclose(vals, expect, atol=2e-6)
+    assert np.any(vals == 1.0) and np.any((vals > 0) & (vals < 1))
+
+
+@pytest.mark.skipif(not os.path.exists(TOOL), reason="reference NanoVDB tool not built (needs the reference tree)")
+@pytest.mark.parametrize("radius,voxel", [(0.25, 0.02), (0.1, 0.005), (0.03, 0.01)])
+def test_fog_sphere_bit_exact_vs_reference_tool(radius, voxel):
+    from cloudrendering_amd import vpt
+
+    mine = np.frombuffer(vpt.build_fog_volume_sphere(radius, (0.0, 0.0, 0.0), voxel), np.uint8)
+    with tempfile.TemporaryDirectory() as tmp:
+        out = os.path.join(tmp, "g.bin")
+        subprocess.run([TOOL, "fogsphere", repr(radius), repr(voxel), out], check=True)
+        ref = np.fromfile(out, np.uint8)
+    assert mine.size == ref.size
+    assert np.array_equal(mine, ref)

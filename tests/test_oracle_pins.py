@@ -272,3 +272,12 @@ def test_mu_c_closed_form(oracle, ext):
     np.testing.assert_array_equal(dec[7], [1.0, 1.0])
     # an empty supervoxel: mu_c = 0 and the floor mu_r_bar = 0.1
     np.testing.assert_array_equal(rrt[1], np.float32([0.0, 0.1]))
+
+
+def test_zcrit_family_correction():
+    """The tests' critical values: one comparison at 3 sigma, Bonferroni for families."""
+    from _stats import zcrit
+
+    assert abs(zcrit(1) - 3.0) < 1e-3
+    assert abs(zcrit(3) - 3.32) < 0.01
+    assert zcrit(512) > zcrit(12) > zcrit(3)
