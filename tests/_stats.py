@@ -73,8 +73,8 @@ def assert_means_equal(m_a, s_a, m_b, s_b, what, m=None, rel_floor=1e-6):
 
 
 def z_scores_vs_exact(mean, m2, exact, n):
-    """Per-ray z of a transmittance estimate (mean and mean of squares over n samples) against the exact value;
-    the variance is floored at 1/n^2 so rays without a single nonzero sample stay finite."""
+    """Per-ray z of a continuous transmittance estimate (mean and mean of squares over n samples) against the exact
+    value; the variance is floored at 1/n^2 so rays without a single nonzero sample stay finite."""
     mean = np.asarray(mean, np.float64)
     var = np.maximum(np.asarray(m2, np.float64) - mean ** 2, 0.0) * (n / max(n - 1, 1))
     se = np.sqrt(np.maximum(var / n, 1.0 / (n * n)))
@@ -82,10 +82,54 @@ def z_scores_vs_exact(mean, m2, exact, n):
 
 
 def assert_unbiased_rays(z, what):
-    """A family of per-ray z-scores: each within zcrit(rays) and their mean within zcrit(1) / sqrt(rays)."""
+    """A family of per-ray z-scores of continuous estimators: each within zcrit(rays), and their mean within
+    zcrit(1) / sqrt(rays)."""
     z = np.asarray(z, np.float64)
     zc = zcrit(z.size)
     zm = float(z.mean()) * math.sqrt(z.size)
     print(f"{what}: max|z| {np.max(np.abs(z)):.2f} (zcrit {zc:.2f}), mean z * sqrt(n) {zm:.2f}", flush=True)
     assert np.max(np.abs(z)) <= zc, (what, int(np.argmax(np.abs(z))), float(np.max(np.abs(z))))
     assert abs(zm) <= zcrit(1), (what, zm)
+
+
+def assert_unbiased_binary_rays(mean, m2, exact, n, what):
+    """Per-ray estimates of a binary (0 / 1 per sample) transmittance estimator: the count of ones over n samples is
+    Binomial(n, T) with T the exact transmittance, so each ray gets an exact two-sided binomial p-value (no normal
+    approximation, which fails for the rays with n T of a few), Bonferroni-corrected over the rays; the mean of the
+    exactly standardised errors (variance T (1 - T) / n) stays within zcrit(1) / sqrt(rays)."""
+    from scipy.stats import binom
+
+    mean = np.asarray(mean, np.float64)
+    T = np.clip(np.asarray(exact, np.float64), 0.0, 1.0)
+    np.testing.assert_allclose(np.asarray(m2, np.float64), mean, rtol=1e-6, atol=1e-7)  # binary samples
+    k = np.rint(mean * n).astype(np.int64)
+    p = 2.0 * np.minimum(binom.cdf(k, n, T), binom.sf(k - 1, n, T))
+    p = np.minimum(p, 1.0)
+    z = (mean - T) / np.sqrt(np.maximum(T * (1.0 - T), 1e-300) / n)
+    zm = float(z.mean()) * math.sqrt(z.size)
+    print(f"{what}: min p {p.min():.2e} (limit {FAMILY_ALPHA / p.size:.2e}), mean z * sqrt(n) {zm:.2f}", flush=True)
+    assert p.min() >= FAMILY_ALPHA / p.size, (what, int(np.argmin(p)), float(p.min()), int(k[np.argmin(p)]),
+                                               float(T[np.argmin(p)]))
+    assert abs(zm) <= zcrit(1), (what, zm)
+
+
+def assert_converged(gpu, var_gpu, n_gpu, ref, var_ref, n_ref, what, rel_max=None):
+    """Two converged renders [H, W, 3] with their per-pixel sample variances and sample counts. Per pixel and channel
+    z = (gpu - ref) / sqrt(var_gpu / n_gpu + var_ref / n_ref). A bias shows as a shifted mean of z: per channel,
+    mean z * sqrt(pixels) stays within zcrit(3). mean(z^2) stays near 1 (at most 1.25: the per-pixel variances are
+    themselves estimates, heavy-tailed for next-event estimation); a misregistered view or a pixel-dependent error
+    lifts it far above. Returns (relMSE, z)."""
+    gpu, ref = np.asarray(gpu, np.float64), np.asarray(ref, np.float64)
+    se = np.sqrt(np.asarray(var_gpu, np.float64) / n_gpu + np.asarray(var_ref, np.float64) / n_ref)
+    z = (gpu - ref) / np.maximum(se, 1e-12)
+    npix = z.shape[0] * z.shape[1]
+    zbar = z.reshape(-1, 3).mean(0) * math.sqrt(npix)
+    z2 = float(np.mean(z * z))
+    rel = float(np.mean((gpu - ref) ** 2 / (ref * ref + 0.01)))
+    print(f"{what}: relMSE {rel:.3e}, mean z * sqrt(pixels) {np.round(zbar, 2)} (zcrit {zcrit(3):.2f}), "
+          f"mean z^2 {z2:.3f}", flush=True)
+    assert np.all(np.abs(zbar) <= zcrit(3)), (what, zbar)
+    assert z2 <= 1.25, (what, z2)
+    if rel_max is not None:
+        assert rel <= rel_max, (what, rel)
+    return rel, z

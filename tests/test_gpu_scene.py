@@ -37,7 +37,11 @@ def test_default_scene_is_the_reference_fog_sphere(vpt):
     g = np.load(os.path.join(GOLDEN, "fog_sphere_default.npz"))
     buf = np.frombuffer(vpt.get_nanovdb_buffer(), np.uint8)
     np.testing.assert_array_equal(buf, g["grid"])
-    got = vpt.grid_lookup(g["ijk"])
+    vpt.set_use_sparse_grid(True)  # grid_lookup reads the active representation; the default one is dense
+    try:
+        got = vpt.grid_lookup(g["ijk"])
+    finally:
+        vpt.set_use_sparse_grid(False)
     np.testing.assert_array_equal(got.view(np.uint32), g["values"].view(np.uint32))
     assert vpt.get_volume_voxel_size() == [51, 51, 51]  # index box [-25, 25]^3
     dense = vpt.get_density_field((51, 51, 51))
