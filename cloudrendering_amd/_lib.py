@@ -62,6 +62,7 @@ SIGNATURES = {
     "crvpt_set_view_projection_matrix_as_previous": (_c_int, []),
     "crvpt_kpn_per_pixel_kernel": (_c_int, [ctypes.c_void_p] * 3 + [_c_int] * 7 + [ctypes.c_void_p]),
     "crvpt_kpn_inv_iter_kernel": (_c_int, [ctypes.c_void_p] * 6 + [_c_int] * 11 + [ctypes.c_void_p]),
+    "crvpt_kpn_inv_iter_kernel_noalloc": (_c_int, [ctypes.c_void_p] * 6 + [_c_int] * 11 + [ctypes.c_void_p]),
     "crvpt_get_camera_position": (_c_int, [_pd]),
     "crvpt_get_camera_view_matrix": (_c_int, [_pd]),
     "crvpt_get_camera_fovy": (_c_int, [_pd]),

@@ -789,6 +789,7 @@ int crvpt_initialize(void) {
         l0.lightSpace = 2;
         r->lights.push_back(l0);
         g_r = r;
+        HIP_TRY(kpnScratchSetup(nullptr));  // address of the no-allocation entry point's scratch on this device
         // Default scene (Module.cpp:793-795): createFogVolumeSphere<float>(0.25, 0, 0.01) as the only volume; the
         // pass keeps rendering a dense grid (useSparseGrid defaults to false, VolumetricPathTracingPass.hpp:260),
         // which is decoded from the sphere's NanoVDB grid.
@@ -1069,6 +1070,31 @@ int crvpt_kpn_inv_iter_kernel(const float* x, const float* low_res_pred, const f
                               img_w, lo_h, lo_w, weight_h, weight_w, kernel_size, s));
     HIP_TRY(launchSoftmaxChannels(fused_weights, soft, n, weight_c, kk, weight_h, weight_w, s));
     HIP_TRY(launchPerPixelKernel(base, soft, out, n, c, h, w, weight_h, weight_w, kernel_size, s));
+    return CRVPT_OK;
+}
+
+int crvpt_kpn_inv_iter_kernel_noalloc(const float* x, const float* low_res_pred, const float* low_res_x,
+                                      const float* prev, const float* fused_weights, float* out, int n, int c,
+                                      int img_h, int img_w, int lo_h, int lo_w, int weight_c, int weight_h,
+                                      int weight_w, int kernel_size, int use_previous, void* stream) {
+    if (int rc = requireInit()) return rc;
+    const int kk = kernel_size * kernel_size;
+    if (!x || !low_res_pred || !low_res_x || !fused_weights || !out || (use_previous && !prev) || n <= 0 || c <= 0 ||
+        img_h <= 0 || img_w <= 0 || weight_h <= 0 || weight_w <= 0 || kernel_size <= 0 || kernel_size % 2 == 0 ||
+        weight_c < kk + 3)
+        return fail(CRVPT_ERR_INVALID_ARGUMENT, "kpn_inv_iter_kernel_noalloc: invalid arguments");
+    if (c > 4) return fail(CRVPT_ERR_INVALID_ARGUMENT, "Error, must have <= 4 channels");
+    const int h = std::min(img_h, weight_h), w = std::min(img_w, weight_w);
+    const uint64_t need = uint64_t(n) * c * h * w + uint64_t(n) * kk * weight_h * weight_w;
+    if (need > kKpnNoAllocScratchFloats)
+        return fail(CRVPT_ERR_INVALID_ARGUMENT,
+                    "kpn_inv_iter_kernel_noalloc: the call needs " + std::to_string(need) +
+                        " scratch floats (n*c*min(img_h, weight_h)*min(img_w, weight_w) + n*k*k*weight_h*weight_w), "
+                        "more than the fixed " + std::to_string(kKpnNoAllocScratchFloats) +
+                        "; use kpn_inv_iter_kernel");
+    HIP_TRY(launchInvIterNoAlloc(x, low_res_pred, low_res_x, prev, fused_weights, out, use_previous != 0, n, c, img_h,
+                                 img_w, lo_h, lo_w, weight_c, weight_h, weight_w, kernel_size,
+                                 reinterpret_cast<hipStream_t>(stream)));
     return CRVPT_OK;
 }
 

@@ -8,6 +8,8 @@
 //   softmax         over the first K*K channels of the fused weights, per pixel (torch::softmax(l, 1)).
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "render_api.hpp"
 
 namespace crvpt {
@@ -119,6 +121,53 @@ hipError_t launchSoftmaxChannels(const float* in, float* out, int N, int Ct, int
     const dim3 block(16, 16, 1), grid((W + 15) / 16, (H + 15) / 16, N);
     hipLaunchKernelGGL(k_softmaxChannels, grid, block, 0, stream, in, out, Ct, kk, H, W);
     return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// Inverse-iteration apply without allocation: the blended base image and the normalised weights live in this file's
+// static device array instead of renderer-owned buffers. Each GPU holds its own copy of the array; its address on
+// a device is taken once (hipGetSymbolAddress with that device current) and kept per device. Calls on one GPU share
+// the array, so calls that overlap in time on one GPU race; every call overwrites the part it uses completely, so
+// nothing needs clearing between calls.
+// ------------------------------------------------------------------------------------------------------------
+static __device__ __align__(256) float scratch[kKpnNoAllocScratchFloats];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratchMutex;
+float* g_scratchPtr[kMaxScratchDevices] = {};
+}  // namespace
+
+hipError_t kpnScratchSetup(float** ptr) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    if (!g_scratchPtr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratchPtr[dev] = static_cast<float*>(p);
+    }
+    if (ptr) *ptr = g_scratchPtr[dev];
+    return hipSuccess;
+}
+
+hipError_t launchInvIterNoAlloc(const float* hiX, const float* loPred, const float* loX, const float* prev,
+                                const float* weights, float* out, bool usePrevious, int N, int C, int imgH, int imgW,
+                                int loH, int loW, int Ct, int wH, int wW, int K, hipStream_t stream) {
+    float* base = nullptr;
+    hipError_t e = kpnScratchSetup(&base);
+    if (e != hipSuccess) return e;
+    const int h = imgH < wH ? imgH : wH, w = imgW < wW ? imgW : wW;
+    float* soft = base + size_t(N) * C * h * w;
+    e = launchInvIterBase(hiX, loPred, loX, prev, weights, base, usePrevious, N, C, imgH, imgW, loH, loW, wH, wW, K,
+                          stream);
+    if (e != hipSuccess) return e;
+    e = launchSoftmaxChannels(weights, soft, N, Ct, K * K, wH, wW, stream);
+    if (e != hipSuccess) return e;
+    return launchPerPixelKernel(base, soft, out, N, C, h, w, wH, wW, K, stream);
 }
 
 }  // namespace crvpt

@@ -160,6 +160,15 @@ hipError_t launchInvIterBase(const float* hiX, const float* loPred, const float*
                              const float* weights, float* out, bool usePrevious, int N, int C, int imgH, int imgW,
                              int loH, int loW, int wH, int wW, int K, hipStream_t stream);
 hipError_t launchSoftmaxChannels(const float* in, float* out, int N, int Ct, int kk, int H, int W, hipStream_t stream);
+// Floats of the inverse-iteration apply's static scratch array (128 MiB): the base image (N*C*h*w) and the
+// normalised weights (N*K*K*wH*wW) of one call.
+constexpr size_t kKpnNoAllocScratchFloats = size_t(32) << 20;
+// Address of the scratch array on the current device (looked up once per device).
+hipError_t kpnScratchSetup(float** ptr);
+// crvpt_kpn_inv_iter_kernel's kernels on the scratch array: no device allocation.
+hipError_t launchInvIterNoAlloc(const float* hiX, const float* loPred, const float* loX, const float* prev,
+                                const float* weights, float* out, bool usePrevious, int N, int C, int imgH, int imgW,
+                                int loH, int loW, int Ct, int wH, int wW, int K, hipStream_t stream);
 hipError_t launchSvgf(const SvgfInputs& in, const SvgfState& st, int iterations, float4** result, hipStream_t stream);
 
 }  // namespace crvpt

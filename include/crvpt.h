@@ -217,6 +217,15 @@ int crvpt_kpn_inv_iter_kernel(const float* x, const float* low_res_pred, const f
                               const float* fused_weights, float* out, int n, int c, int img_h, int img_w, int lo_h,
                               int lo_w, int weight_c, int weight_h, int weight_w, int kernel_size, int use_previous,
                               void* stream);
+/* No reference counterpart: crvpt_kpn_inv_iter_kernel without device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, serving loops). The blended base image and the normalised weights live in
+ * a fixed 128 MiB device array per GPU (32 Mi floats; n*c*min(img_h, weight_h)*min(img_w, weight_w) +
+ * n*k*k*weight_h*weight_w must fit, else CRVPT_ERR_INVALID_ARGUMENT); the array is shared by every call on that GPU,
+ * so calls that overlap on one GPU race. Same arguments and results as crvpt_kpn_inv_iter_kernel. */
+int crvpt_kpn_inv_iter_kernel_noalloc(const float* x, const float* low_res_pred, const float* low_res_x,
+                                      const float* prev, const float* fused_weights, float* out, int n, int c,
+                                      int img_h, int img_w, int lo_h, int lo_w, int weight_c, int weight_h,
+                                      int weight_w, int kernel_size, int use_previous, void* stream);
 
 /* No reference counterpart: statistics of the last crvpt_render_frame: kernel milliseconds of the path kernels
  * (device events on the render stream), number of paths, number of path-kernel launches. */
