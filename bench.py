@@ -4,8 +4,8 @@ Default workload "c2_sparse512" (one GPU): a NanoVDB FP32 sparse grid of a 512^3
 (5 octaves, seed 0, lacunarity 2, gain 0.5, base frequency 4) thresholded by smoothstep(0.645, 0.795) so that about
 12 % of the voxels are active in 8^3 leaves, 16^3-voxel majorant supervoxels, extinction 1024, albedo 0.9, the
 reference's default sky, 1024 x 1024 pinhole camera at the reference module's default pose, 256 samples per pixel,
-at most 128 scattering events per path. Step k renders the image once with estimator
-and phase g taken from the 12 combinations (delta, ratio, residual ratio, decomposition) x g in {0, 0.5, 0.85}.
+at most 128 scattering events per path. One step renders the image once with every one of the 12 combinations
+(delta, ratio, residual ratio, decomposition tracking) x g in {0, 0.5, 0.85}: 12 x 268 M paths.
 
 Workload "c1_dense128": dense 128^3 FP32 fBm (smoothstep(0.45, 0.6)) filling the unit cube, next-event tracking with
 a constant white environment image and one directional sun, g = 0.85, 256 x 256, 64 spp per step, at most 64
@@ -15,9 +15,10 @@ Workload "c3_batch": the batched render queue, 64 requests (dense 64^3 / 128^3 /
 volumes, image side and spp log-uniform in 128-1024 and 16-1024, random camera orbits and sun directions) traced by
 one persistent launch per step with the autotuned launch configuration; prints its own JSON line.
 
-Workload "c4_kpn": the kernel-prediction denoiser: training pairs rendered by the tracer (4 spp + feature maps,
-1024-spp targets in one batched launch), training on the GPU, and the equal-quality wall-time comparison with plain
-tracking against a 16k-spp reference on an unseen cloud; prints its own JSON line.
+Workload "c4_kpn": the kernel-prediction denoiser at the C4 scale: 4096 training crops rendered by the tracer (4-spp
+and 4-256-spp inputs with feature maps, 1024-spp targets in one batched launch), training on the GPU, the
+equal-time comparison with EAW, SVGF and plain tracking at the 4-spp budget, and the wall time to relMSE 1e-3
+against plain tracking, all judged by a 16k-spp render of an unseen cloud; prints its own JSON line.
 
 Workload "c5_fp16_2048": the large sparse production frame: NanoVDB with Fp16 leaves over a 2048^3 domain (6-octave
 anvil fBm, about 5 % active), 1920 x 1080, 1024 spp, residual ratio tracking, at most 256 scattering events, plus in
@@ -216,10 +217,10 @@ def run_c3(args, vpt, torch):
     print(json.dumps(line), flush=True)
 
 
-C4 = {"crops": 384, "crop": 128, "clouds": 16, "input_spp": [4, 16, 64, 256], "eval_input_spp": [4, 16, 64, 96, 128, 192, 256],
-      "target_spp": 1024, "batch": 16,
-      "train_steps": 3000, "lr": 1e-3, "eval_res": 1024, "ref_spp": 16384, "plain_spp": [4, 16, 64, 256, 1024, 4096],
-      "max_bounces": 128}
+C4 = {"crops": 4096, "crop": 128, "clouds": 16, "input_spp": 4, "mixed_input_spp": [4, 16, 64, 256],
+      "eval_input_spp": [4, 16, 64, 128, 256], "target_spp": 1024, "batch": 16, "train_steps": 6000, "lr": 1e-3,
+      "eval_res": 1024, "ref_spp": 16384, "plain_spp": [4, 16, 64, 256, 1024, 4096], "max_bounces": 128,
+      "eaw_iterations": 4, "svgf_max_frames": 256}
 
 
 def _c4_scene(vpt, seed):
@@ -247,19 +248,39 @@ def _c4_camera(vpt, rng):
     vpt.set_camera_target([0.0, 0.0, 0.0])
 
 
+def _train(torch, net, feats, noisy, target, steps, gen, log):
+    """`steps` Adam steps on random batches of the pairs (learning rate / 10 for the last fifth)."""
+    losses = []
+    n = feats.shape[0]
+    for s_ in range(steps):
+        idx = torch.randint(0, n, (C4["batch"],), generator=gen).cuda()
+        lr = C4["lr"] * (0.1 if s_ > 0.8 * steps else 1.0)
+        losses.append(net.train_step(feats[idx], noisy[idx], target[idx], lr))
+        if s_ % 1000 == 0:
+            print(f"{log} step {s_}: loss {losses[-1]:.5f}", file=sys.stderr, flush=True)
+    return losses
+
+
 def run_c4(args, vpt, torch):
-    """Kernel-prediction denoiser: training pairs rendered by the tracer (inputs: 4, 16, 64 or 256 spp with the feature
-    maps; targets: 1024 spp, all crops in one batched launch), training with the hand-written bf16 MFMA network, and
-    the comparison on an unseen cloud at 1024^2 against a 16k-spp reference: the wall time and relative MSE of plain
-    tracking at increasing spp and of the denoised renders at 4 to 256 input spp (render + features + inference)."""
+    """Kernel-prediction denoiser at the C4 scale: 4096 seeded 128^2 crops of 16 fBm clouds, each with a 1024-spp
+    target (all targets in one batched launch) and two inputs with the tracer's feature maps: 4 spp (network A, the
+    4-spp -> 1024-spp training of the configuration) and 4, 16, 64 or 256 spp (network B, the same crops and
+    targets). Evaluation on an unseen cloud and camera at 1024^2, judged against a 16k-spp render:
+      * equal time: network A on a 4-spp render against EAW (4 a-trous iterations on the same 4-spp render), SVGF (1-spp
+        frames of the static view accumulated by its temporal filter for the same wall time) and plain tracking at
+        the same wall time (interpolated on the plain curve);
+      * the headline: wall time to relMSE 1e-3, network B (render at 4-256 input spp + features + inference) against
+        plain tracking at increasing spp."""
     import numpy as np
 
     from cloudrendering_amd import kpn
 
     crops = args.kpn_crops or C4["crops"]
     cs = C4["crop"]
-    feats = torch.zeros((crops, 15, cs, cs), dtype=torch.float32, device="cuda")
-    noisy = torch.zeros((crops, 3, cs, cs), dtype=torch.float32, device="cuda")
+    featsA = torch.zeros((crops, 15, cs, cs), dtype=torch.float32, device="cuda")
+    noisyA = torch.zeros((crops, 3, cs, cs), dtype=torch.float32, device="cuda")
+    featsB = torch.zeros((crops, 15, cs, cs), dtype=torch.float32, device="cuda")
+    noisyB = torch.zeros((crops, 3, cs, cs), dtype=torch.float32, device="cuda")
     t0 = time.perf_counter()
     vpt.batch_clear()
     per_cloud = (crops + C4["clouds"] - 1) // C4["clouds"]
@@ -267,36 +288,42 @@ def run_c4(args, vpt, torch):
     for c in range(crops):
         if c % per_cloud == 0:
             _c4_scene(vpt, 1 + c // per_cloud)
+            print(f"pairs: cloud {1 + c // per_cloud}, crop {c}", file=sys.stderr, flush=True)
         rng = np.random.default_rng(10_000 + c)
         _c4_camera(vpt, rng)
-        vpt.set_seed_offset(2 * c + 1)
+        vpt.set_use_feature_maps([])
+        vpt.set_seed_offset(3 * c + 2)
         targets.append(vpt.batch_add_request(torch.zeros((3, cs, cs), device="cuda"), C4["target_spp"])[1])
-        vpt.set_seed_offset(2 * c)
-        f, x = kpn.render_inputs(vpt, cs, cs, C4["input_spp"][c % len(C4["input_spp"])])
-        feats[c] = f
-        noisy[c] = x
+        vpt.set_seed_offset(3 * c)
+        f, x = kpn.render_inputs(vpt, cs, cs, C4["input_spp"])
+        featsA[c] = f
+        noisyA[c] = x
+        vpt.set_seed_offset(3 * c + 1)
+        f, x = kpn.render_inputs(vpt, cs, cs, C4["mixed_input_spp"][c % len(C4["mixed_input_spp"])])
+        featsB[c] = f
+        noisyB[c] = x
+    vpt.set_use_feature_maps([])
     vpt.batch_set_launch_config(0, 256, 16, 64)
     vpt.batch_render()
     torch.cuda.synchronize()
     target = torch.stack(targets)
+    del targets
     vpt.batch_clear()
     gen_s = time.perf_counter() - t0
     print(f"pairs: {crops} crops in {gen_s:.1f} s", file=sys.stderr, flush=True)
 
-    net = kpn.KpnDenoiser(seed=0)
     steps = args.kpn_steps or C4["train_steps"]
     g = torch.Generator(device="cpu").manual_seed(0)
+    netA = kpn.KpnDenoiser(seed=0)
+    netB = kpn.KpnDenoiser(seed=1)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    losses = []
-    for s_ in range(steps):
-        idx = torch.randint(0, crops, (C4["batch"],), generator=g).cuda()
-        lr = C4["lr"] * (0.1 if s_ > 0.8 * steps else 1.0)
-        losses.append(net.train_step(feats[idx], noisy[idx], target[idx], lr))
-        if s_ % 250 == 0:
-            print(f"train step {s_}: loss {losses[-1]:.5f}", file=sys.stderr, flush=True)
+    lossA = _train(torch, netA, featsA, noisyA, target, steps, g, "train A (4 spp)")
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
+    lossB = _train(torch, netB, featsB, noisyB, target, steps, g, "train B (4-256 spp)")
+    torch.cuda.synchronize()
+    del featsA, noisyA, featsB, noisyB, target
 
     # evaluation on an unseen cloud and camera
     res = args.kpn_eval_res or C4["eval_res"]
@@ -310,26 +337,28 @@ def run_c4(args, vpt, torch):
     vpt.batch_render()
     torch.cuda.synchronize()
     vpt.batch_clear()
-    plain = []
     probe = torch.zeros((3, res, res), dtype=torch.float32, device="cuda")
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        return out, time.perf_counter() - t
+
+    plain = []
     for spp in C4["plain_spp"]:
         vpt.set_seed_offset(100 + spp)
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        img = vpt.render_frame(probe, spp)
-        torch.cuda.synchronize()
-        plain.append({"spp": spp, "seconds": round(time.perf_counter() - t, 4),
-                      "rel_mse": kpn.rel_mse(img, ref)})
-    denoised = []
-    for spp in C4["eval_input_spp"]:
-        vpt.set_seed_offset(55 + spp)
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        f, x = kpn.render_inputs(vpt, res, res, spp)
-        den = net.denoise(f.unsqueeze(0), x.unsqueeze(0))[0]
-        torch.cuda.synchronize()
-        denoised.append({"input_spp": spp, "seconds": round(time.perf_counter() - t, 4),
-                         "rel_mse": kpn.rel_mse(den, ref), "noisy_rel_mse": kpn.rel_mse(x, ref)})
+        img, sec = timed(lambda: vpt.render_frame(probe, spp))
+        plain.append({"spp": spp, "seconds": round(sec, 4), "rel_mse": kpn.rel_mse(img, ref)})
+
+    def plain_at(seconds):
+        """Plain-tracking relMSE after `seconds`, interpolated log-log on the measured curve."""
+        for a, b in zip(plain, plain[1:]):
+            if a["seconds"] <= seconds <= b["seconds"]:
+                u = (math.log(seconds) - math.log(a["seconds"])) / (math.log(b["seconds"]) - math.log(a["seconds"]))
+                return math.exp(math.log(a["rel_mse"]) + u * (math.log(b["rel_mse"]) - math.log(a["rel_mse"])))
+        return None
 
     def time_to(err):
         """Plain-tracking seconds to reach err, interpolated log-log between the measured spp."""
@@ -339,16 +368,63 @@ def run_c4(args, vpt, torch):
                 return math.exp(math.log(a["seconds"]) + u * (math.log(b["seconds"]) - math.log(a["seconds"])))
         return None
 
+    def kpn_run(net, spp, seed):
+        vpt.set_seed_offset(seed)
+
+        def go():
+            f, x = kpn.render_inputs(vpt, res, res, spp)
+            return net.denoise(f.unsqueeze(0), x.unsqueeze(0))[0], x
+        (den, x), sec = timed(go)
+        vpt.set_use_feature_maps([])
+        return den, x, sec
+
+    # equal time at the 4-spp budget: network A, EAW, SVGF, plain
+    denA, xA, secA = kpn_run(netA, C4["input_spp"], 55)
+    vpt.set_denoiser("EAW")
+    vpt.set_denoiser_property("maxNumIterations", str(C4["eaw_iterations"]))
+    vpt.set_seed_offset(56)
+    eaw, sec_eaw = timed(lambda: vpt.render_frame(probe, C4["input_spp"]))
+    vpt.set_denoiser("SVGF")
+    vpt.set_denoiser_property("maxNumIterations", "5")
+    vpt.set_seed_offset(57)
+    frames, sec_svgf = 0, 0.0
+    while sec_svgf < secA and frames < C4["svgf_max_frames"]:
+        svgf, sec = timed(lambda: vpt.render_frame(probe, 1))
+        sec_svgf += sec
+        frames += 1
+    vpt.set_denoiser("None")
+    vpt.set_use_feature_maps([])
+    equal_time = {
+        "budget_s": round(secA, 4),
+        "kpn_4spp": {"seconds": round(secA, 4), "rel_mse": kpn.rel_mse(denA, ref), "noisy_rel_mse": kpn.rel_mse(xA, ref)},
+        "eaw_4spp": {"seconds": round(sec_eaw, 4), "iterations": C4["eaw_iterations"], "rel_mse": kpn.rel_mse(eaw, ref)},
+        "svgf": {"seconds": round(sec_svgf, 4), "frames_1spp": frames, "rel_mse": kpn.rel_mse(svgf, ref)},
+        "plain": {"seconds": round(secA, 4), "rel_mse": plain_at(secA)},
+    }
+    print(f"equal time: {equal_time}", file=sys.stderr, flush=True)
+
+    # headline: wall time to relMSE 1e-3, network B at increasing input spp against plain tracking
+    denoised = []
+    for spp in C4["eval_input_spp"]:
+        den, x, sec = kpn_run(netB, spp, 500 + spp)
+        d = {"input_spp": spp, "seconds": round(sec, 4), "rel_mse": kpn.rel_mse(den, ref),
+             "noisy_rel_mse": kpn.rel_mse(x, ref)}
+        d["plain_seconds_at_same_rel_mse"] = time_to(d["rel_mse"])
+        d["speedup_at_same_rel_mse"] = (d["plain_seconds_at_same_rel_mse"] / d["seconds"]
+                                        if d["plain_seconds_at_same_rel_mse"] else None)
+        denoised.append(d)
+
     # Denoiser inference throughput on the full evaluation frame (HIP events on the stream the network runs on):
     # algorithmic FLOPs per pixel = 2 * 9 * (15 * 32 + 4 * 32 * 32 + 32 * 25) for the six 3x3 convolutions.
     f, x = kpn.render_inputs(vpt, res, res, 4)
+    vpt.set_use_feature_maps([])
     f, x = f.unsqueeze(0), x.unsqueeze(0)
-    net.denoise(f, x)
+    netB.denoise(f, x)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     reps = 20
     e0.record()
     for _ in range(reps):
-        net.denoise(f, x)
+        netB.denoise(f, x)
     e1.record()
     torch.cuda.synchronize()
     inf_ms = e0.elapsed_time(e1) / reps
@@ -361,22 +437,20 @@ def run_c4(args, vpt, torch):
                                       "5x5 filter) against the dense bf16 MFMA peak"}}
 
     plain_1e3 = time_to(1e-3)
-    for d in denoised:
-        d["plain_seconds_at_same_rel_mse"] = time_to(d["rel_mse"])
-        d["speedup_at_same_rel_mse"] = (d["plain_seconds_at_same_rel_mse"] / d["seconds"]
-                                        if d["plain_seconds_at_same_rel_mse"] else None)
     reached = [d for d in denoised if d["rel_mse"] <= 1e-3]
     kpn_1e3 = min(d["seconds"] for d in reached) if reached else None
     line = {
         "metric": "wall time to relMSE 1e-3: KPN-denoised vs plain tracking", "unit": "s",
         "value": round(kpn_1e3, 4) if kpn_1e3 else None, "higher_is_better": False, "n_gpus": 1,
         "dtype": "bf16 (network, FP32 accumulation) / f32 (tracer)", "data": "synthetic (seeded fBm clouds)",
-        "config": {"workload": "c4_kpn", "crops": crops, "crop": cs, "input_spp": C4["input_spp"],
-                   "eval_input_spp": C4["eval_input_spp"],
-                   "target_spp": C4["target_spp"], "train_steps": steps, "batch": C4["batch"],
-                   "eval_res": res, "ref_spp": ref_spp, "estimator": "delta tracking"},
-        "pair_generation_s": round(gen_s, 2), "train_s": round(train_s, 2), "final_train_loss": losses[-1],
-        "train_steps_per_s": round(steps / train_s, 1), "inference": inference,
+        "config": {"workload": "c4_kpn", "crops": crops, "crop": cs, "clouds": C4["clouds"],
+                   "network_a_input_spp": C4["input_spp"], "network_b_input_spp": C4["mixed_input_spp"],
+                   "eval_input_spp": C4["eval_input_spp"], "target_spp": C4["target_spp"], "train_steps": steps,
+                   "batch": C4["batch"], "eval_res": res, "ref_spp": ref_spp, "estimator": "delta tracking",
+                   "volume": "dense 128^3 fBm per cloud (seeds 1-16 training, 999 evaluation)"},
+        "pair_generation_s": round(gen_s, 2), "train_s_per_network": round(train_s, 2),
+        "final_train_loss": {"a": lossA[-1], "b": lossB[-1]}, "train_steps_per_s": round(steps / train_s, 1),
+        "equal_time_4spp": equal_time, "inference": inference,
         "kpn": denoised, "plain": plain, "plain_seconds_to_1e-3": plain_1e3,
         "speedup_to_1e-3": (plain_1e3 / kpn_1e3) if (plain_1e3 and kpn_1e3) else None,
     }
@@ -538,13 +612,39 @@ def run_c5(args, vpt, torch):
     print(json.dumps(line), flush=True)
 
 
+def host_threads():
+    """The host threads this process may use: OMP_NUM_THREADS when set (on a shared GPU host it is the CPU share of
+    one GPU), else every CPU in the process's affinity mask."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def host_cpu():
+    """Host CPU model (lscpu's "Model name", read from /proc/cpuinfo) and the logical CPU count."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count(), "threads_used": host_threads()}
+
+
 def cpu_baseline_c1(budget_s: float = 12.0):
     """The oracle (FP32 restatement of the reference shaders, hybrid-Taus RNG) on the C1 scene and camera."""
     import numpy as np
 
     from oracle import oracle as O
 
-    threads = max(1, min(os.cpu_count() or 1, 16))
+    threads = host_threads()
     d = O.fbm_cloud(C1["grid"], C1["grid"], C1["grid"], seed=0)
     sun = np.array([0.5826, 0.7660, 0.2717])
     sun = 100.0 * sun / np.linalg.norm(sun)
@@ -564,7 +664,7 @@ def cpu_baseline_c2(vpt, budget_s: float = 15.0):
 
     from oracle import oracle as O
 
-    threads = max(1, min(os.cpu_count() or 1, 16))
+    threads = host_threads()
     g = C2["grid"]
     d = vpt.get_density_field((g, g, g), normalized=True)
     # The sparse grid spans the active voxels only: crop the dense field to them and use the same world box.
@@ -592,7 +692,8 @@ def cpu_baseline_c2(vpt, budget_s: float = 15.0):
         frames += 1
     return {"value": paths / elapsed / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
             "sample": f"{res}x{res} px x {frames} spp ({paths} paths) of the C2 volume and camera, estimator/g "
-                      f"cycling like the timed steps, {elapsed:.1f} s on {threads} threads"}
+                      f"cycling like the timed steps, {elapsed:.1f} s on {threads} threads",
+            "cpu": host_cpu()}
 
 
 def _time_oracle(sc, res, threads, budget_s, what):
@@ -606,7 +707,7 @@ def _time_oracle(sc, res, threads, budget_s, what):
         frames += 1
     return {"value": paths / elapsed / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
             "sample": f"{res}x{res} px x {frames} spp ({paths} paths) of the {what}, {elapsed:.1f} s on "
-                      f"{threads} threads"}
+                      f"{threads} threads", "cpu": host_cpu()}
 
 
 def load_pmc(workload: str) -> dict:
@@ -638,7 +739,9 @@ def copy_calibration(torch):
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=0, help="timed steps (0: 12, or 3 for c3_batch and c5_fp16_2048)")
+    ap.add_argument("--steps", type=int, default=0,
+                    help="timed steps (0: 2 sweeps of the 12 estimator x g renders for c2_sparse512, 12 for "
+                         "c1_dense128, 3 otherwise)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["c2_sparse512", "c1_dense128", "c3_batch", "c4_kpn", "c5_fp16_2048"],
                     default="c2_sparse512")
@@ -651,7 +754,7 @@ def main() -> int:
     ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.steps <= 0:
-        args.steps = 3 if args.workload in ("c3_batch", "c5_fp16_2048") else 12
+        args.steps = {"c2_sparse512": 2, "c1_dense128": 12}.get(args.workload, 3)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path tracer is single-GPU here",
               file=sys.stderr)
@@ -688,24 +791,26 @@ def main() -> int:
         vpt.set_phase_g(g)
         return m
 
+    # One step renders every combination once (c2_sparse512: the 12 estimator x g renders; c1_dense128: one).
     for k in range(args.warmup):
-        apply(k)
-        vpt.render_frame(target, spp)
+        for c in range(len(combos)):
+            apply(c)
+            vpt.render_frame(target, spp)
     torch.cuda.synchronize()
     per_combo_ms = {}
     t0 = time.perf_counter()
     for k in range(args.steps):
-        apply(k)
-        vpt.render_frame(target, spp)
-        st = vpt.get_last_render_stats()
-        per_combo_ms.setdefault(combos[k % len(combos)], []).append(st["path_kernel_ms"])
-        launches = st["launches"]
-        print(f"step {k}: {combos[k % len(combos)]} path kernel {st['path_kernel_ms']:.1f} ms", file=sys.stderr,
-              flush=True)
+        for c in range(len(combos)):
+            apply(c)
+            vpt.render_frame(target, spp)
+            st = vpt.get_last_render_stats()
+            per_combo_ms.setdefault(combos[c], []).append(st["path_kernel_ms"])
+            launches = st["launches"]
+            print(f"step {k}: {combos[c]} path kernel {st['path_kernel_ms']:.1f} ms", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1e3 / args.steps
-    paths_per_step = W * H * spp
+    paths_per_step = W * H * spp * len(combos)
     value = paths_per_step / (wall / args.steps) / 1e6
 
     # Dominant kernel: the estimator whose path kernel takes the most time over the timed steps.
@@ -744,7 +849,8 @@ def main() -> int:
                   "image": f"{W}x{H}", "spp_per_step": spp,
                   "estimators": [n for _, n in C2["modes"]], "phase_g": C2["gs"], "albedo": 0.9,
                   "extinction": 1024.0, "environment": "default sky", "max_bounces": C2["max_bounces"],
-                  "majorant": "16^3 supervoxel grid", "steps_cycle": "estimator x g (12 combinations)"}
+                  "majorant": "16^3 supervoxel grid",
+                  "step": "one render of each of the 12 estimator x g combinations (12 x 268,435,456 paths)"}
     else:
         kernel = "k_pathPool<NEXT_EVENT_TRACKING, DenseAccessT<false>, false>"
         config = {"workload": args.workload, "volume": "dense 128^3 fp32 fBm seed 0", "image": f"{W}x{H}",
@@ -752,6 +858,11 @@ def main() -> int:
                   "extinction": 1024.0, "environment": "constant white image + directional sun",
                   "max_bounces": C1["max_bounces"], "majorant": "8^3 supervoxel grid"}
     per_combo = {f"{m}/{g}": round(sum(v) / len(v), 3) for (m, g), v in per_combo_ms.items()}
+    names = dict(C2["modes"]) if c2 else {5: "next event tracking"}
+    # path-kernel rate of each estimator over its timed renders (all g)
+    per_mode = {names[m]: {"path_kernel_ms": round(sum(ms) / len(ms), 3),
+                           "mpaths_per_s": round(W * H * spp / (sum(ms) / len(ms) * 1e-3) / 1e6, 2)}
+                for m, ms in by_mode.items()}
     line = {
         "metric": "Mpaths/s on 1 MI355X",
         "value": round(value, 3),
@@ -774,6 +885,7 @@ def main() -> int:
                      "density_taps": int(taps), "supervoxel_reads": int(svr), "tree_descents": int(desc),
                      "paths": int(paths)},
         "kernel_ms_per_mode_g": per_combo,
+        "per_estimator": per_mode,
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline_c2(vpt) if c2 else cpu_baseline_c1()

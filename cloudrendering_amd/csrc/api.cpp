@@ -854,7 +854,7 @@ int crvpt_render_frame(float* out, int channels, int height, int width, int fram
     NanoGrid ng;
     bool sp;
     gridViews(r, dg, ng, sp);
-    HIP_TRY(r.dCounter.ensure(64));
+    HIP_TRY(r.dCounter.ensure(64 * 1024));  // claim counters of the path pool (one per claim range)
     unsigned long long* stats = nullptr;
     if (r.collectStats) {
         HIP_TRY(r.dStats.ensure(4 * sizeof(unsigned long long)));

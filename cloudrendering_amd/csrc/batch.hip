@@ -44,6 +44,18 @@ CRVPT_DEV int findRequest(const uint64_t* __restrict__ begins, int n, uint64_t q
     return lo;
 }
 
+#ifndef CRVPT_BATCH_CLAIM_PARTS
+#define CRVPT_BATCH_CLAIM_PARTS 8
+#endif
+// Unit partitions: the batch's units are split into this many contiguous ranges (consecutive requests), each with its
+// own claim counter; a workgroup first drains range blockIdx.x % parts (its XCD: workgroups are dispatched to the 8
+// XCDs round robin), then the following ones, so one XCD's paths in flight read few volumes and image regions.
+constexpr uint32_t kBatchParts = CRVPT_BATCH_CLAIM_PARTS;
+
+CRVPT_DEV uint32_t unitPartBegin(uint32_t totalUnits, uint32_t part) {
+    return uint32_t(uint64_t(totalUnits) * part / kBatchParts);
+}
+
 template <int MODE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(poolWaves<MODE>()))) void k_batchPool(
         const BatchRequest* __restrict__ reqs, const uint64_t* __restrict__ unitBegins, int nreq, uint32_t totalUnits,
@@ -69,20 +81,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(poolWaves<M
         st.flags = 0;
         if (left == 0u) records[st.unit] = make_float4(sum.x, sum.y, sum.z, 0.0f);
     };
+    uint32_t part = kBatchParts == 1u ? 0u : blockIdx.x % kBatchParts;  // wave-uniform
+    uint32_t partsLeft = kBatchParts;
+    uint32_t begin = unitPartBegin(totalUnits, part), count = unitPartBegin(totalUnits, part + 1u) - begin;
     while (true) {
         while (true) {
             const bool claim = !(st.flags & PF_ALIVE) && left == 0u && !exhausted;
             const uint64_t claimMask = __ballot(claim);
             if (claimMask != 0ull) {
                 const int leader = __ffsll((unsigned long long)claimMask) - 1;
+                const uint32_t want = uint32_t(__popcll(claimMask));
                 uint32_t base = 0;
-                if (lane == leader) base = atomicAdd(counter, uint32_t(__popcll(claimMask)));
+                if (lane == leader) base = atomicAdd(&counter[part], want);
                 base = __shfl(base, leader);
-                if (claim) {
-                    const uint32_t u = base + uint32_t(__popcll(claimMask & ((1ull << lane) - 1ull)));
-                    if (u >= totalUnits) {
+                const bool drained = base + want >= count;
+                const uint32_t partStart = begin, partCount = count;
+                if (drained) {
+                    // this range is drained: the next one, until every range is
+                    if (--partsLeft == 0u) {
                         exhausted = true;
                     } else {
+                        part = part + 1u == kBatchParts ? 0u : part + 1u;
+                        begin = unitPartBegin(totalUnits, part);
+                        count = unitPartBegin(totalUnits, part + 1u) - begin;
+                    }
+                }
+                if (claim) {
+                    const uint32_t c = base + uint32_t(__popcll(claimMask & ((1ull << lane) - 1ull)));
+                    const uint32_t u = partStart + c;
+                    if (c < partCount) {
                         const int r = findRequest(unitBegins, nreq, u);
                         if (r != req) {
                             gc = LeafCache();  // the memo holds byte offsets into the previous request's grid
@@ -190,7 +217,7 @@ hipError_t launchBatch(int mode, const BatchRequest* reqs, const uint64_t* unitB
     if (cfg.blockSize <= 0 || cfg.blockSize > 256 || cfg.blockSize % 64 != 0 || cfg.numBlocks == 0 ||
         cfg.samplesPerUnit == 0 || cfg.stepsPerRefill <= 0)
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_setU32, dim3(1), dim3(64), 0, stream, counter, 0u);
+    for (uint32_t p = 0; p < kBatchParts; p++) hipLaunchKernelGGL(k_setU32, dim3(1), dim3(64), 0, stream, counter + p, 0u);
     void* args[] = {(void*)&reqs,    (void*)&unitBegins, (void*)&nreq,    (void*)&totalUnits,
                     (void*)&cfg.samplesPerUnit, (void*)&frameOffset, (void*)&records, (void*)&counter,
                     (void*)&cfg.stepsPerRefill};

@@ -55,6 +55,29 @@ CRVPT_DEV bool claimToSlot(const SceneParams& P, uint32_t c, uint32_t& slot) {
     return true;
 }
 
+#ifndef CRVPT_CLAIM_PARTS
+#define CRVPT_CLAIM_PARTS 32
+#endif
+// Claim partitions: the tiles are split into this many contiguous ranges, each with its own claim counter, and a
+// workgroup first drains range blockIdx.x % parts, then the following ones. Workgroups are dispatched to the 8 XCDs
+// round robin, so with a multiple of 8 ranges the paths in flight on one XCD come from a few bands of the image,
+// which keeps the volume region they read, and so the XCD's L2 working set, small; the claims also spread over many
+// counters instead of one.
+constexpr uint32_t kClaimParts = CRVPT_CLAIM_PARTS;
+#ifndef CRVPT_COUNTER_STRIDE
+#define CRVPT_COUNTER_STRIDE 1
+#endif
+// Distance in 32-bit words between the claim counters of consecutive ranges (32: one 128-byte line each).
+constexpr uint32_t kCounterStride = CRVPT_COUNTER_STRIDE;
+
+CRVPT_DEV uint32_t partBegin(const SceneParams& P, uint32_t part) {
+    if (kClaimParts == 1u) return part == 0u ? 0u : claimCount(P);
+    const uint32_t tx = (uint32_t(P.width) + kClaimTile - 1u) / kClaimTile;
+    const uint32_t ty = (uint32_t(P.height) + kClaimTile - 1u) / kClaimTile;
+    const uint32_t tiles = tx * ty;
+    return uint32_t(uint64_t(tiles) * part / kClaimParts) * (kClaimTile * kClaimTile * P.numFrames);
+}
+
 // STATS: instrumented variant that keeps the per-lane work counters and adds them to `stats` at exit; in the timed
 // variant the counters are dead code and take no registers.
 template <int MODE, class Grid, bool STATS>
@@ -71,7 +94,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(poolWaves<M
     Walker W;
     W.svReads = 0;
     typename Grid::Cache gc;
-    const uint32_t claims = claimCount(P);
+    uint32_t part = kClaimParts == 1u ? 0u : blockIdx.x % kClaimParts;  // wave-uniform
+    uint32_t partsLeft = kClaimParts;
+    uint32_t begin = partBegin(P, part), count = partBegin(P, part + 1u) - begin;
     while (true) {
         // Refill until every lane holds a live path or the queue is drained: a claimed path whose camera ray misses
         // the volume finishes inside startPath, and its lane claims again before the wave steps.
@@ -80,16 +105,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(poolWaves<M
             const uint64_t needMask = __ballot(needs);
             if (needMask == 0) break;
             const int leader = __ffsll((unsigned long long)needMask) - 1;
+            const uint32_t want = uint32_t(__popcll(needMask));
             uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(counter, uint32_t(__popcll(needMask)));
+            if (lane == leader) base = atomicAdd(&counter[part * kCounterStride], want);
             base = __shfl(base, leader);
             if (needs) {
                 const uint32_t c = base + uint32_t(__popcll(needMask & ((1ull << lane) - 1ull)));
-                if (c >= claims) {
-                    exhausted = true;
-                } else if (claimToSlot(P, c, st.unit)) {
+                if (c < count && claimToSlot(P, begin + c, st.unit)) {
                     paths++;
                     if (!startPath<MODE>(st, W, P, S, feat)) writePath(out, st);
+                }
+            }
+            if (base + want >= count) {
+                // this range is drained: the next one, until every range is
+                if (--partsLeft == 0u) {
+                    exhausted = true;
+                } else {
+                    part = part + 1u == kClaimParts ? 0u : part + 1u;
+                    begin = partBegin(P, part);
+                    count = partBegin(P, part + 1u) - begin;
                 }
             }
         }
@@ -754,7 +788,8 @@ template <class Grid>
 static hipError_t launchRenderT(const SceneParams& P, const Grid& grid, const SupervoxelGrid& S, float4* paths,
                                 uint32_t* counter, const LaunchConfig& cfg, unsigned long long* stats, float4* feat,
                                 hipStream_t stream) {
-    hipLaunchKernelGGL(k_zeroU32, dim3(1), dim3(64), 0, stream, counter, 1);
+    hipLaunchKernelGGL(k_zeroU32, dim3((kClaimParts * kCounterStride + 255) / 256), dim3(256), 0, stream, counter,
+                       int(kClaimParts * kCounterStride));
     const dim3 block(256);
     const dim3 gridDim(cfg.numBlocks);
     const int st = cfg.stepsPerRefill;

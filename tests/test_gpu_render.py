@@ -297,6 +297,62 @@ def test_next_event_lights_vs_oracle(vpt, oracle, case):
     _check_equal(m_gpu, s_gpu, m_ref, s_ref, f"light {case}")
 
 
+def _sparse_blobs(n=64, seed=23):
+    """A mostly empty n^3 volume: two fBm blobs in opposite corners, empty space between them (long runs of empty
+    supervoxels for the walker)."""
+    from oracle import oracle as O
+
+    d = np.zeros((n, n, n), np.float32)
+    b = O.fbm_cloud(20, 18, 16, seed=seed)
+    d[4:20, 6:24, 5:25] = b
+    d[n - 22:n - 6, n - 20:n - 2, n - 24:n - 4] = b
+    return d
+
+
+@pytest.mark.parametrize("estimator", ["delta", "ratio"])
+@pytest.mark.parametrize("sv", [2, 4])
+def test_mostly_empty_volume_transmittance(vpt, oracle, estimator, sv):
+    """Transmittance through a mostly empty volume with small majorant supervoxels (the walker crosses long runs of
+    empty cells) against the FP64 optical depth, on rays that cross the empty middle of the volume."""
+    print(f"case: mostly empty volume transmittance {estimator} edge {sv}", flush=True)
+    d = _sparse_blobs()
+    _set_volume(vpt, d)
+    _common(vpt, 0, interp="stochastic", ext=80.0)
+    vpt.set_launch_config(sv, 0, 0)
+    try:
+        o, w = _rays(512, 31, 0.25)
+        n = 8192
+        mean, m2 = vpt.transmittance_batch(estimator, o, w, n_samples=n, seed=7)
+    finally:
+        vpt.set_launch_config(0, 0, 0)
+    exact = np.exp(-oracle.Scene(d, interpolation="stochastic", extinction=80.0).optical_depth(o, w))
+    assert np.ptp(exact) > 0.5
+    if estimator == "delta":
+        assert_unbiased_binary_rays(mean, m2, exact, n, f"mostly empty transmittance {estimator}/{sv}")
+    else:
+        assert_unbiased_rays(z_scores_vs_exact(mean, m2, exact, n), f"mostly empty transmittance {estimator}/{sv}")
+
+
+@pytest.mark.parametrize("name,mode", [("delta", 0), ("next_event", 5), ("residual_ratio", 4), ("decomposition", 3)])
+def test_mostly_empty_volume_render(vpt, oracle, name, mode):
+    """Renders of the mostly empty volume with 2-voxel supervoxels (majorant grid and reference tables) against the
+    oracle."""
+    print(f"case: mostly empty volume render {name}", flush=True)
+    d = _sparse_blobs()
+    _set_volume(vpt, d)
+    _common(vpt, mode, ext=150.0, g=0.5)
+    vpt.set_launch_config(2, 0, 0)
+    vpt.set_super_voxel_size(2)
+    try:
+        m_gpu, s_gpu, _ = _render_stats(vpt, GPU_SPP)
+    finally:
+        vpt.set_launch_config(0, 0, 0)
+        vpt.set_super_voxel_size(8)
+    sc = oracle.Scene(d, mode=name, extinction=150.0, g=0.5, width=RES, height=RES, super_voxel_size=2)
+    m_ref, s_ref = _oracle_stats(sc, SPP)
+    _check_equal(m_gpu, s_gpu, m_ref, s_ref, f"mostly empty {name}")
+
+
 def test_majorant_grid_matches_global_majorant(vpt, oracle):
     print("case: majorant grid on/off next-event", flush=True)
     d = oracle.fbm_cloud(32, 32, 32, seed=7)
