@@ -65,7 +65,7 @@ CRVPT_DEV bool claimToSlot(const SceneParams& P, uint32_t c, uint32_t& slot) {
 // counters instead of one.
 constexpr uint32_t kClaimParts = CRVPT_CLAIM_PARTS;
 #ifndef CRVPT_COUNTER_STRIDE
-#define CRVPT_COUNTER_STRIDE 1
+#define CRVPT_COUNTER_STRIDE 32
 #endif
 // Distance in 32-bit words between the claim counters of consecutive ranges (32: one 128-byte line each).
 constexpr uint32_t kCounterStride = CRVPT_COUNTER_STRIDE;
@@ -251,21 +251,24 @@ __global__ void k_accumulate(const float4* __restrict__ paths, uint32_t npix, ui
                              int channels, int foreground) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npix) return;
-    float sx = 0.0f, sy = 0.0f, sz = 0.0f, sw = 0.0f;
+    // FP64 sums: a float32 sum of thousands of samples drifts systematically (tracking estimators return a few
+    // distinct values, e.g. the sky radiance or 0, and adding the same value to a growing float sum rounds the same
+    // way every time: about 5e-4 relative after 65536 samples); the doubles keep the mean exact to float precision.
+    double sx = 0.0, sy = 0.0, sz = 0.0, sw = 0.0;
     for (uint32_t f = 0; f < frames; f++) {
         const float4 v = paths[size_t(f) * npix + p];
         const float keep = foreground == 0 ? 1.0f : (foreground == 1 ? v.w : 0.0f);
-        sx += v.x * keep;
-        sy += v.y * keep;
-        sz += v.z * keep;
-        sw += keep;
+        sx += double(v.x * keep);
+        sy += double(v.y * keep);
+        sz += double(v.z * keep);
+        sw += double(keep);
     }
     float4 a = prevFrames == 0 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : acc[p];
-    const float inv = 1.0f / float(prevFrames + frames);
-    a.x = (a.x * float(prevFrames) + sx) * inv;
-    a.y = (a.y * float(prevFrames) + sy) * inv;
-    a.z = (a.z * float(prevFrames) + sz) * inv;
-    a.w = foreground == 0 ? 1.0f : (a.w * float(prevFrames) + sw) * inv;
+    const double prev = double(prevFrames), inv = 1.0 / double(prevFrames + frames);
+    a.x = float((double(a.x) * prev + sx) * inv);
+    a.y = float((double(a.y) * prev + sy) * inv);
+    a.z = float((double(a.z) * prev + sz) * inv);
+    a.w = foreground == 0 ? 1.0f : float((double(a.w) * prev + sw) * inv);
     acc[p] = a;
     if (outCHW) {
         outCHW[p] = a.x;

@@ -156,6 +156,35 @@ def test_converged_image_rel_mse(vpt, oracle, name, mode):
     assert_converged(gpu, var_gpu, gpu_spp, ref, var_ref, ref_spp, f"converged {name}", rel_max=1e-3)
 
 
+@pytest.mark.parametrize("env", ["Default", "Single Color"])
+def test_sky_seen_directly_vs_oracle(vpt, oracle, env):
+    """With extinction 0 every camera ray escapes, so the image is the sky (and the DEFAULT sky's sun lobe) averaged
+    over each pixel's footprint: sampleSkybox + sampleLight (VptUtils.glsl:301-391) on the GPU against the oracle,
+    pixel by pixel (per-pixel z-scores of the jittered footprint averages, tests/_stats.py assert_converged)."""
+    print(f"case: sky seen directly, {env}", flush=True)
+    torch = _torch()
+    d = oracle.fbm_cloud(16, 16, 16, seed=2)
+    _set_volume(vpt, d)
+    _common(vpt, 0, ext=0.0)
+    vpt.set_use_builtin_environment_map(env)
+    res, spp = 48, 16384
+    vpt.set_use_feature_maps(["Radiance Variance"])
+    try:
+        t = torch.zeros((4, res, res), dtype=torch.float32, device="cuda")
+        gpu = _cpu(vpt.render_frame(t, spp))[:3].transpose(1, 2, 0).astype(np.float64)
+        var_gpu = _cpu(vpt.get_feature_map(t, 21))[:3].transpose(1, 2, 0).astype(np.float64)
+    finally:
+        vpt.set_use_feature_maps([])
+        vpt.set_use_builtin_environment_map("Default")
+    sc = oracle.Scene(d, mode="delta", extinction=0.0, g=0.5, width=res, height=res,
+                      env="default" if env == "Default" else "single_color")
+    ref, sq = sc.render(spp)
+    ref = ref.astype(np.float64)
+    var_ref = np.maximum(sq / spp - ref ** 2, 0.0)
+    print(f"max relative difference {np.max(np.abs(gpu - ref) / np.maximum(np.abs(ref), 1e-6)):.2e}", flush=True)
+    assert_converged(gpu, var_gpu, spp, ref, var_ref, spp, f"sky {env}")
+
+
 @pytest.mark.parametrize("name,mode", [("delta_spectral", 1), ("next_event_spectral", 6)])
 @pytest.mark.parametrize("majorant_grid", [True, False])
 def test_spectral_vs_oracle(vpt, oracle, name, mode, majorant_grid):
