@@ -75,6 +75,9 @@ def load():
     lib.oracle_render.argtypes = [P(OracleScene), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
     lib.oracle_render.restype = ctypes.c_int
+    lib.oracle_render_sums.argtypes = [P(OracleScene), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+    lib.oracle_render_sums.restype = ctypes.c_int
     lib.oracle_transmittance.argtypes = [P(OracleScene), ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32,
                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
@@ -221,6 +224,19 @@ class Scene:
                                threads)
         assert rc == 0
         return acc, sq
+
+    def render_mean(self, frames, frame_start=0, threads=0):
+        """Exact FP64 sample mean [H, W, 3] of frames [frame_start, frame_start + frames) (the samples of render),
+        plus the per-pixel sum of squared path radiance. Converged comparisons use this: the reference's FP32
+        running mean drifts by up to a few 1e-4 relative over 10^4-10^5 frames."""
+        lib = load()
+        h, w = self.height, self.width
+        s1 = np.zeros((h, w, 3), np.float64)
+        sq = np.zeros((h, w, 3), np.float64)
+        rc = lib.oracle_render_sums(ctypes.byref(self.sc), w, h, frame_start, frames, s1.ctypes.data,
+                                    sq.ctypes.data, threads)
+        assert rc == 0
+        return s1 / max(frames, 1), sq
 
     def render_features(self, frames, names=FEATURE_MAPS, frame_start=0, state=None, prev_view_proj=None,
                         far_distance=80.0, threads=0):

@@ -1273,6 +1273,24 @@ extern "C" void oracle_camera_matrices(const double pos[3], const double target[
     }
 }
 
+namespace {
+/* One sample of pixel (x, y) in frame f: the seed and camera jitter of Clouds.glsl:73-90, then the estimator. */
+vec3 renderSample(const Ctx& c, const OracleScene* scene, int width, int height, int x, int y, int f) {
+    uint32_t frameGlobal = uint32_t(f);
+    uint32_t seed = frameGlobal * uint32_t(width) * uint32_t(height) + uint32_t(x) + uint32_t(y) * uint32_t(width);
+    seed += scene->custom_seed_offset;
+    Rng rng;
+    rng.init(seed);
+    float r0 = rng.random();
+    float r1 = rng.random();
+    float sx = 2.0f * (float(x) + r0) / float(width) - 1;
+    float sy = 2.0f * (float(y) + r1) / float(height) - 1;
+    vec3 xo, w;
+    createCameraRay(c, sx, sy, xo, w);
+    return tracePath(c, rng, xo, w);
+}
+}  // namespace
+
 extern "C" int oracle_render(const OracleScene* scene, int width, int height, int frame_start, int num_frames,
                              float* acc_rgb, double* sum_sq, int num_threads) {
     if (!scene || !acc_rgb || width <= 0 || height <= 0 || num_frames < 0) return 1;
@@ -1286,20 +1304,8 @@ extern "C" int oracle_render(const OracleScene* scene, int width, int height, in
         for (int x = 0; x < width; x++) {
             float* acc = acc_rgb + (size_t(y) * width + x) * 3;
             for (int f = frame_start; f < frame_start + num_frames; f++) {
-                uint32_t frame = uint32_t(f);
-                uint32_t frameGlobal = uint32_t(f);
-                uint32_t seed = frameGlobal * uint32_t(width) * uint32_t(height) + uint32_t(x) +
-                                uint32_t(y) * uint32_t(width);
-                seed += scene->custom_seed_offset;
-                Rng rng;
-                rng.init(seed);
-                float r0 = rng.random();
-                float r1 = rng.random();
-                float sx = 2.0f * (float(x) + r0) / float(width) - 1;
-                float sy = 2.0f * (float(y) + r1) / float(height) - 1;
-                vec3 xo, w;
-                createCameraRay(c, sx, sy, xo, w);
-                vec3 result = tracePath(c, rng, xo, w);
+                const uint32_t frame = uint32_t(f);
+                vec3 result = renderSample(c, scene, width, height, x, y, f);
                 if (sum_sq) {
                     double* q = sum_sq + (size_t(y) * width + x) * 3;
                     q[0] += double(result.x) * result.x;
@@ -1311,6 +1317,33 @@ extern "C" int oracle_render(const OracleScene* scene, int width, int height, in
                 acc[0] = result.x;
                 acc[1] = result.y;
                 acc[2] = result.z;
+            }
+        }
+    }
+    return 0;
+}
+
+extern "C" int oracle_render_sums(const OracleScene* scene, int width, int height, int frame_start, int num_frames,
+                                  double* sum_rgb, double* sum_sq, int num_threads) {
+    if (!scene || !sum_rgb || !sum_sq || width <= 0 || height <= 0 || num_frames < 0) return 1;
+    Ctx c;
+    initCtx(c, scene);
+#ifdef _OPENMP
+    if (num_threads > 0) omp_set_num_threads(num_threads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+    for (int y = 0; y < height; y++) {
+        for (int x = 0; x < width; x++) {
+            double* m = sum_rgb + (size_t(y) * width + x) * 3;
+            double* q = sum_sq + (size_t(y) * width + x) * 3;
+            for (int f = frame_start; f < frame_start + num_frames; f++) {
+                const vec3 r = renderSample(c, scene, width, height, x, y, f);
+                m[0] += double(r.x);
+                m[1] += double(r.y);
+                m[2] += double(r.z);
+                q[0] += double(r.x) * r.x;
+                q[1] += double(r.y) * r.y;
+                q[2] += double(r.z) * r.z;
             }
         }
     }

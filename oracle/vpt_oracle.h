@@ -89,6 +89,14 @@ int oracle_render(const OracleScene* scene, int width, int height, int frame_sta
                   float* acc_rgb, double* sum_sq, int num_threads);
 
 /*
+ * The same samples as oracle_render, summed in FP64 into sum_rgb and sum_sq ([h][w][3] doubles each, added to):
+ * the exact sample mean, free of the FP32 running mean's rounding drift (up to 5e-4 relative after 65536 frames),
+ * for comparisons of converged images. Returns 0 on success.
+ */
+int oracle_render_sums(const OracleScene* scene, int width, int height, int frame_start, int num_frames,
+                       double* sum_rgb, double* sum_sq, int num_threads);
+
+/*
  * Feature maps of Clouds.glsl:146-383 (WRITE_*_MAP): every pointer that is not null receives its map as height x
  * width pixels of the reference image format's channels (RGBA: result, first_x, first_w, normal, cloud_only,
  * background, albedo; RG: depth (mean, std), density (mean, std), reproj_uv, flow, flow_reverse, depth_nabla;

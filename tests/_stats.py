@@ -52,12 +52,12 @@ def gpu_image_stats(vpt, frames, res, device="cuda"):
 
 
 def oracle_image_stats(scene, frames, frame_start=0, threads=0):
-    """The oracle's image after `frames` samples: (per-channel mean, standard error, image [H, W, 3])."""
-    img, sq = scene.render(frames, frame_start=frame_start, threads=threads)
-    npix = img.shape[0] * img.shape[1]
-    img64 = img.astype(np.float64)
+    """The oracle's image after `frames` samples (the exact FP64 sample mean of each pixel): (per-channel mean,
+    standard error, image [H, W, 3])."""
+    img64, sq = scene.render_mean(frames, frame_start=frame_start, threads=threads)
+    npix = img64.shape[0] * img64.shape[1]
     var = np.maximum(sq / frames - img64 ** 2, 0.0) * (frames / max(frames - 1, 1))
-    return img64.reshape(-1, 3).mean(0), np.sqrt(var.reshape(-1, 3).sum(0) / frames) / npix, img
+    return img64.reshape(-1, 3).mean(0), np.sqrt(var.reshape(-1, 3).sum(0) / frames) / npix, img64
 
 
 def assert_means_equal(m_a, s_a, m_b, s_b, what, m=None, rel_floor=1e-6):

@@ -70,8 +70,7 @@ def _converged_crop(vpt, oracle, scene_kwargs, density, target, what, res=6, gpu
         vpt.set_camera_target([pos[0], pos[1], pos[2] - 1.0])
     cam = oracle.camera_matrices(pos=tuple(pos), target=tuple(float(v) for v in target), fovy=fovy, aspect=1.0)
     sc = oracle.Scene(density, camera=cam, width=res, height=res, **scene_kwargs)
-    ref, sq = sc.render(ref_spp, threads=THREADS)
-    ref = ref.astype(np.float64)
+    ref, sq = sc.render_mean(ref_spp, threads=THREADS)
     var_ref = np.maximum(sq / ref_spp - ref ** 2, 0.0)
     assert ref.mean() > 0.01, "the crop must see the cloud"
     return assert_converged(gpu, var_gpu, gpu_spp, ref, var_ref, ref_spp, what)

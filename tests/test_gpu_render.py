@@ -150,22 +150,24 @@ def test_converged_image_rel_mse(vpt, oracle, name, mode):
     finally:
         vpt.set_use_feature_maps([])
     sc = oracle.Scene(d, mode=name, interpolation="stochastic", extinction=96.0, g=0.5, width=res, height=res)
-    ref, sq = sc.render(ref_spp)
-    ref = ref.astype(np.float64)
+    ref, sq = sc.render_mean(ref_spp)
     var_ref = np.maximum(sq / ref_spp - ref ** 2, 0.0)
     assert_converged(gpu, var_gpu, gpu_spp, ref, var_ref, ref_spp, f"converged {name}", rel_max=1e-3)
 
 
 @pytest.mark.parametrize("env", ["Default", "Single Color"])
 def test_sky_seen_directly_vs_oracle(vpt, oracle, env):
-    """With extinction 0 every camera ray escapes, so the image is the sky (and the DEFAULT sky's sun lobe) averaged
-    over each pixel's footprint: sampleSkybox + sampleLight (VptUtils.glsl:301-391) on the GPU against the oracle,
-    pixel by pixel (per-pixel z-scores of the jittered footprint averages, tests/_stats.py assert_converged)."""
+    """With a vanishing extinction (1e-4: under 1e-4 collisions per camera ray) the camera rays escape, so the image
+    is the sky (and the DEFAULT sky's sun lobe) averaged over each pixel's footprint: sampleSkybox + sampleLight
+    (VptUtils.glsl:301-391) on the GPU against the oracle, pixel by pixel (per-pixel z-scores of the jittered
+    footprint averages, tests/_stats.py assert_converged). Extinction exactly 0 is not usable: the reference's
+    delta tracking draws the free path -log(1 - u) / majorant, which is NaN for u = 0 and a zero majorant, and its
+    loop `if (t > d) break` then never ends (the oracle restates it and does the same)."""
     print(f"case: sky seen directly, {env}", flush=True)
     torch = _torch()
     d = oracle.fbm_cloud(16, 16, 16, seed=2)
     _set_volume(vpt, d)
-    _common(vpt, 0, ext=0.0)
+    _common(vpt, 0, ext=1e-4)
     vpt.set_use_builtin_environment_map(env)
     res, spp = 48, 16384
     vpt.set_use_feature_maps(["Radiance Variance"])
@@ -176,10 +178,9 @@ def test_sky_seen_directly_vs_oracle(vpt, oracle, env):
     finally:
         vpt.set_use_feature_maps([])
         vpt.set_use_builtin_environment_map("Default")
-    sc = oracle.Scene(d, mode="delta", extinction=0.0, g=0.5, width=res, height=res,
+    sc = oracle.Scene(d, mode="delta", extinction=1e-4, g=0.5, width=res, height=res,
                       env="default" if env == "Default" else "single_color")
-    ref, sq = sc.render(spp)
-    ref = ref.astype(np.float64)
+    ref, sq = sc.render_mean(spp)
     var_ref = np.maximum(sq / spp - ref ** 2, 0.0)
     print(f"max relative difference {np.max(np.abs(gpu - ref) / np.maximum(np.abs(ref), 1e-6)):.2e}", flush=True)
     assert_converged(gpu, var_gpu, spp, ref, var_ref, spp, f"sky {env}")

@@ -170,6 +170,25 @@ def test_white_furnace(oracle):
     np.testing.assert_allclose(img, np.broadcast_to(sky, img.shape), rtol=1e-6)
 
 
+def test_render_mean_sums_the_render_samples(oracle):
+    """render_mean draws the samples render draws: one frame is the sample itself, the squared sums agree exactly,
+    frame ranges add up, and the FP32 running mean stays within its rounding drift of the exact mean."""
+    from oracle.oracle import fbm_cloud
+
+    sc = oracle.Scene(fbm_cloud(12, 10, 8, seed=4), mode="delta", extinction=48.0, width=12, height=10)
+    one, sq1 = sc.render(1)
+    m1, q1 = sc.render_mean(1)
+    np.testing.assert_array_equal(m1, one.astype(np.float64))
+    np.testing.assert_array_equal(q1, sq1)
+    acc, sq = sc.render(64)
+    m, q = sc.render_mean(64)
+    np.testing.assert_array_equal(q, sq)
+    np.testing.assert_allclose(acc, m, rtol=1e-5, atol=1e-6)
+    ma, _ = sc.render_mean(24)
+    mb, _ = sc.render_mean(40, frame_start=24)
+    np.testing.assert_allclose((24 * ma + 40 * mb) / 64, m, rtol=1e-12, atol=1e-15)
+
+
 def _mean_and_sigma(img, sq, frames):
     """Image mean per channel and its standard error from the per-pixel sample variances."""
     npix = img.shape[0] * img.shape[1]
