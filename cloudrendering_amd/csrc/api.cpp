@@ -627,6 +627,18 @@ FeatureAcc featureAcc(Renderer& r, uint32_t npix) {
     return a;
 }
 
+// The walker's per-axis constants of the majorant and reference cell grids (cell edge in world units, cells per
+// world unit, the reference grid's upper corner), from the box and texel scale of P.
+void setWalkerConstants(const SceneParams& P, SupervoxelGrid& S) {
+    for (int a = 0; a < 3; a++) {
+        S.cellWorld[a] = float(S.size) / P.worldToTex[a];
+        S.cellsPerWorld[a] = P.worldToTex[a] * (1.0f / float(S.size));
+        S.refCellWorld[a] = float(S.refSize) / P.worldToTex[a];
+        S.refCellsPerWorld[a] = P.worldToTex[a] * (1.0f / float(S.refSize));
+        S.refHi[a] = P.boxMin[a] + float(S.refGrid[a] * S.refSize) / P.worldToTex[a];
+    }
+}
+
 void fillSceneParams(Renderer& r, int w, int h, SceneParams& P, SupervoxelGrid& S) {
     std::memset(&P, 0, sizeof(P));
     const Mat4 vp = mul(projection(r, w, h), r.view);
@@ -655,6 +667,16 @@ void fillSceneParams(Renderer& r, int w, int h, SceneParams& P, SupervoxelGrid& 
     }
     P.extinctionMax = std::max(std::max(P.extinctionRGB[0], P.extinctionRGB[1]), P.extinctionRGB[2]);
     P.phaseG = float(r.phaseG);
+    {
+        // the device evaluated these per launch with fused multiply-adds; fmaf keeps the same roundings
+        const float g = P.phaseG;
+        P.phaseC[PHASE_ONE_MINUS_G2] = std::fmaf(-g, g, 1.0f);
+        P.phaseC[PHASE_ONE_PLUS_G2] = std::fmaf(g, g, 1.0f);
+        P.phaseC[PHASE_TWO_G] = 2.0f * g;
+        P.phaseC[PHASE_HALF_INV_G] = std::fabs(g) < 0.001f ? 0.0f : 0.5f / g;
+        P.phaseC[PHASE_ONE_MINUS_G] = 1.0f - g;
+        P.phaseC[PHASE_PDF_SCALE] = (0.25f / 3.14159265359f) * std::fmaf(-g, g, 1.0f);
+    }
     P.mode = r.mode;
     P.interpolation = r.interpolation;
     P.useLinearRGB = r.useLinearRGB;
@@ -691,6 +713,11 @@ void fillSceneParams(Renderer& r, int w, int h, SceneParams& P, SupervoxelGrid& 
     P.maxBounces = r.maxBounces;
     P.width = w;
     P.height = h;
+    P.widthF = float(w);
+    P.heightF = float(h);
+    P.divWidth = makeDiv31(uint32_t(std::max(w, 1)));
+    P.divPixels = makeDiv31(uint32_t(std::max(w, 1)) * uint32_t(std::max(h, 1)));
+    P.absorption = 1.0f - P.albedo;
     S.majorant = r.dMajorant.as<float>();
     S.rrt = r.dRrt.as<float>();
     S.decomp = r.dDecomp.as<float>();
@@ -700,6 +727,7 @@ void fillSceneParams(Renderer& r, int w, int h, SceneParams& P, SupervoxelGrid& 
         S.grid[a] = r.svGrid[a];
         S.refGrid[a] = r.refGrid[a];
     }
+    setWalkerConstants(P, S);
 }
 
 // Walker steps between refill ballots: the setting, else 24 for residual ratio and next-event tracking (their
@@ -875,7 +903,8 @@ int crvpt_render_frame(float* out, int channels, int height, int width, int fram
     // launches trace faster (16 M -> 256 M paths: +13 % on a 1024^2 x 256-spp sparse render). The records of one
     // launch take 16 bytes per path (4 GiB at 2^28), plus 48 bytes per path with feature maps (2^26 paths then).
     const uint32_t maxPaths = features ? (1u << 26) : (1u << 28);
-    const uint32_t framesPerLaunch = std::max(1u, std::min(uint32_t(std::max(traceFrames, 1)), maxPaths / npix));
+    const uint32_t framesPerLaunch = std::max(
+            1u, std::min({uint32_t(std::max(traceFrames, 1)), maxPaths / npix, maxFramesPerLaunch(width, height)}));
     HIP_TRY(r.dPaths.ensure(size_t(npix) * framesPerLaunch * sizeof(float4)));
     LaunchConfig cfg;
     cfg.numBlocks = uint32_t(launchBlocks(r));
@@ -2087,6 +2116,7 @@ int crvpt_batch_add_request(float* out, int channels, int height, int width, int
     q.P.numFrames = uint32_t(spp);
     q.P.numUnits = uint64_t(width) * uint64_t(height) * uint64_t(spp);
     q.S = vol.S;
+    setWalkerConstants(q.P, q.S);
     q.dense = vol.dense;
     q.nano = vol.nano;
     q.spp = uint32_t(spp);

@@ -12,6 +12,9 @@ namespace crvpt {
 
 #define CRVPT_DEV __device__ __forceinline__
 
+// n / d for n < 2^31 with the divisor's host-computed multiplier (Div31).
+CRVPT_DEV uint32_t udiv31(uint32_t n, Div31 d) { return uint32_t((uint64_t(n) * d.m) >> d.s); }
+
 constexpr float kPi = 3.14159265359f;
 constexpr float kTwoPi = 2.0f * 3.14159265359f;
 
@@ -135,37 +138,14 @@ CRVPT_DEV void orthonormalBasis(V3 d, V3& b, V3& t) {
     t = normalize(cross(d, b));
 }
 
-// Uniform direction on the sphere expressed in the basis of d (randomDirection).
-CRVPT_DEV V3 uniformSphereDirection(Rng& rng, V3 d) {
-    float r1 = rng.uniform();
-    float r2 = rng.uniform() * 2.0f - 1.0f;
-    float s = sqrtf(fmaxf(0.0f, 1.0f - r2 * r2));
-    float phi = kTwoPi * r1;
-    V3 t0, t1;
-    orthonormalBasis(d, t0, t1);
-    return t0 * (__cosf(phi) * s) + t1 * (__sinf(phi) * s) + d * r2;
-}
-
-CRVPT_DEV float hgPdf(float g, float cosTheta) {
-    float denom = (1.0f + g * g) - 2.0f * g * cosTheta;
-    return (0.25f / kPi) * (1.0f - g * g) * __builtin_amdgcn_rcpf(denom * sqrtf(denom));
-}
-
-// importanceSamplePhase: HG inversion (isotropic below |g| < 0.001).
-CRVPT_DEV V3 samplePhase(Rng& rng, float g, V3 d) {
-    if (fabsf(g) < 0.001f) return uniformSphereDirection(rng, -d);
-    float phi = rng.uniform() * kTwoPi;
-    float xi = rng.uniform();
-    float t = (1.0f - g * g) / (1.0f - g + 2.0f * g * xi);
-    float cosTheta = (0.5f / g) * ((1.0f + g * g) - t * t);
-    float sinTheta = sqrtf(fmaxf(0.0f, 1.0f - cosTheta * cosTheta));
-    V3 t0, t1;
-    orthonormalBasis(d, t0, t1);
-    return t0 * (sinTheta * __sinf(phi)) + t1 * (sinTheta * __cosf(phi)) + d * cosTheta;
+// Henyey-Greenstein pdf of cosTheta from the scene's phase constants (phaseC, set on the host from g).
+CRVPT_DEV float hgPdf(const SceneParams& P, float cosTheta) {
+    const float denom = P.phaseC[PHASE_ONE_PLUS_G2] - P.phaseC[PHASE_TWO_G] * cosTheta;
+    return P.phaseC[PHASE_PDF_SCALE] * __builtin_amdgcn_rcpf(denom * sqrtf(denom));
 }
 
 // evaluatePhase
-CRVPT_DEV float evalPhase(float g, V3 a, V3 b) { return hgPdf(g, dot(normalize(a), normalize(b))); }
+CRVPT_DEV float evalPhase(const SceneParams& P, V3 a, V3 b) { return hgPdf(P, dot(normalize(a), normalize(b))); }
 
 // ------------------------------------------------------------------------------------------------------------
 // Sky and lights (VptUtils.glsl:271-453)

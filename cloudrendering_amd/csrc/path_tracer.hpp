@@ -81,8 +81,12 @@ struct Walker {
 CRVPT_DEV float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
 
 template <int MODE>
-CRVPT_DEV int cellEdge(const SupervoxelGrid& S) {
-    return walksReferenceTable<MODE>() ? S.refSize : S.size;
+CRVPT_DEV float cellWorld(const SupervoxelGrid& S, int a) {
+    return walksReferenceTable<MODE>() ? S.refCellWorld[a] : S.cellWorld[a];
+}
+template <int MODE>
+CRVPT_DEV float cellsPerWorld(const SupervoxelGrid& S, int a) {
+    return walksReferenceTable<MODE>() ? S.refCellsPerWorld[a] : S.cellsPerWorld[a];
 }
 template <int MODE>
 CRVPT_DEV int cellGridDim(const SupervoxelGrid& S, int a) {
@@ -129,12 +133,7 @@ CRVPT_DEV bool walkerStart(Walker& W, const SceneParams& P, const SupervoxelGrid
     const float ay = fabsf(w.y) <= 0.000001f ? 0.000001f : w.y;
     const float az = fabsf(w.z) <= 0.000001f ? 0.000001f : w.z;
     const V3 inv = v3(rcpf(ax), rcpf(ay), rcpf(az));
-    V3 hi = v3(P.boxMax);
-    if (walksReferenceTable<MODE>()) {
-        hi = v3(P.boxMin[0] + float(S.refGrid[0] * S.refSize) * rcpf(P.worldToTex[0]),
-                P.boxMin[1] + float(S.refGrid[1] * S.refSize) * rcpf(P.worldToTex[1]),
-                P.boxMin[2] + float(S.refGrid[2] * S.refSize) * rcpf(P.worldToTex[2]));
-    }
+    const V3 hi = walksReferenceTable<MODE>() ? v3(S.refHi[0], S.refHi[1], S.refHi[2]) : v3(P.boxMax);
     const V3 c0 = (v3(P.boxMin) - x) * inv;
     const V3 c1 = ((gridBox ? hi : v3(P.boxMax)) - x) * inv;
     float tMin = fmaxf(fmaxf(fminf(c0.x, c1.x), fminf(c0.y, c1.y)), fminf(c0.z, c1.z));
@@ -159,7 +158,6 @@ CRVPT_DEV bool walkerStart(Walker& W, const SceneParams& P, const SupervoxelGrid
         W.nx = W.ny = W.nz = kInf;
         return true;
     }
-    const int edge = cellEdge<MODE>(S);
     // Cell of a point slightly inside the segment (robust against an entry point on a cell plane).
     const float tc = tMin + 1e-4f * (tMax - tMin);
     const V3 p = x + w * tc;
@@ -168,8 +166,8 @@ CRVPT_DEV bool walkerStart(Walker& W, const SceneParams& P, const SupervoxelGrid
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         const float wa = comp(w, a), pa = comp(p, a), ia = comp(inv, a);
-        const float cw = float(edge) * rcpf(P.worldToTex[a]);  // cell edge in world units
-        int i = int(floorf((pa - P.boxMin[a]) * (P.worldToTex[a] * (1.0f / float(edge)))));
+        const float cw = cellWorld<MODE>(S, a);
+        int i = int(floorf((pa - P.boxMin[a]) * cellsPerWorld<MODE>(S, a)));
         i = min(max(i, 0), cellGridDim<MODE>(S, a) - 1);
         idx[a] = i;
         // Components the slab test clamps to 1e-6 never reach a cell plane within the box (a segment moves less
@@ -198,7 +196,7 @@ enum : int { WALK_MOVED = 0, WALK_COLLIDE = 1, WALK_END = 2 };
 // is finite, i.e. |w_a| > 1e-6.
 template <int MODE>
 CRVPT_DEV float planeSpacing(const SceneParams& P, const SupervoxelGrid& S, float wa, int a) {
-    const float cw = float(cellEdge<MODE>(S)) * rcpf(P.worldToTex[a]);
+    const float cw = cellWorld<MODE>(S, a);
     const float ia = rcpf(wa);
     return wa > 0.0f ? cw * ia : -cw * ia;
 }
@@ -332,12 +330,13 @@ CRVPT_DEV V3 sphereDirection(float r1, float r2u, V3 d) {
     return t0 * (__cosf(phi) * s) + t1 * (__sinf(phi) * s) + d * r2;
 }
 
-// importanceSamplePhase from two uniforms (HG inversion; isotropic below |g| < 0.001).
-CRVPT_DEV V3 phaseDirection(float g, V3 d, float u0, float u1) {
-    if (fabsf(g) < 0.001f) return sphereDirection(u0, u1, -d);
+// importanceSamplePhase from two uniforms (HG inversion; isotropic below |g| < 0.001), with the scene's phase
+// constants.
+CRVPT_DEV V3 phaseDirection(const SceneParams& P, V3 d, float u0, float u1) {
+    if (fabsf(P.phaseG) < 0.001f) return sphereDirection(u0, u1, -d);
     const float phi = u0 * kTwoPi;
-    const float t = (1.0f - g * g) * rcpf(1.0f - g + 2.0f * g * u1);
-    const float cosTheta = (0.5f / g) * ((1.0f + g * g) - t * t);
+    const float t = P.phaseC[PHASE_ONE_MINUS_G2] * rcpf(P.phaseC[PHASE_TWO_G] * u1 + P.phaseC[PHASE_ONE_MINUS_G]);
+    const float cosTheta = P.phaseC[PHASE_HALF_INV_G] * (P.phaseC[PHASE_ONE_PLUS_G2] - t * t);
     const float sinTheta = sqrtf(fmaxf(0.0f, 1.0f - cosTheta * cosTheta));
     V3 t0, t1;
     orthonormalBasis(d, t0, t1);
@@ -347,9 +346,9 @@ CRVPT_DEV V3 phaseDirection(float g, V3 d, float u0, float u1) {
 // Camera ray of the path: pixel jitter from the camera block (Clouds.glsl:80-90).
 CRVPT_DEV void cameraRay(const SceneParams& P, const PathState& st, V3& x, V3& w) {
     const uint4 b = pathBlock(P, st, kCameraBlock);
-    const uint32_t px = st.pix % uint32_t(P.width), py = st.pix / uint32_t(P.width);
-    const float sx = 2.0f * (float(px) + u01(b.x)) / float(P.width) - 1.0f;
-    const float sy = 2.0f * (float(py) + u01(b.y)) / float(P.height) - 1.0f;
+    const uint32_t py = udiv31(st.pix, P.divWidth), px = st.pix - py * uint32_t(P.width);
+    const float sx = 2.0f * (float(px) + u01(b.x)) / P.widthF - 1.0f;
+    const float sy = 2.0f * (float(py) + u01(b.y)) / P.heightF - 1.0f;
     createCameraRay(P, sx, sy, x, w);
 }
 
@@ -378,9 +377,8 @@ CRVPT_DEV bool startPathAt(PathState& st, Walker& W, const SceneParams& P, const
 // Starts the path of launch unit st.unit = sample * (W*H) + pixel.
 template <int MODE>
 CRVPT_DEV bool startPath(PathState& st, Walker& W, const SceneParams& P, const SupervoxelGrid& S, float4* feat) {
-    const uint32_t npix = uint32_t(P.width) * uint32_t(P.height);
-    const uint32_t s = st.unit / npix;
-    st.pix = st.unit - s * npix;
+    const uint32_t s = udiv31(st.unit, P.divPixels);
+    st.pix = st.unit - s * (uint32_t(P.width) * uint32_t(P.height));
     st.frame = P.frameBase + s;
     return startPathAt<MODE>(st, W, P, S, feat);
 }
@@ -434,12 +432,12 @@ CRVPT_DEV bool scatterEvent(PathState& st, Walker& W, const SceneParams& P, cons
                             float pAbsOverReal, float rho, const Grid& grid, float4* feat) {
     const uint4 e = pathBlock(P, st, kEventSpace | st.events++);
     st.flags += 1u << kBounceShift;
-    const V3 nw = phaseDirection(P.phaseG, W.w, u01(e.x), u01(e.y));
+    const V3 nw = phaseDirection(P, W.w, u01(e.x), u01(e.y));
     if (feat && MODE != VPT_DECOMPOSITION_TRACKING) {
         // ratio tracking records neither depth nor density (RatioTracking.glsl:75-81)
         const bool ratio = MODE == VPT_RATIO_TRACKING;
         const float ext = (isNee<MODE>() || isSpectral<MODE>()) ? P.extinctionMax : P.extinction;
-        recordFirst<MODE>(st, P, grid, feat, p, nw, evalPhase(P.phaseG, W.w, nw), ratio ? 0.0f : rho * ext,
+        recordFirst<MODE>(st, P, grid, feat, p, nw, evalPhase(P, W.w, nw), ratio ? 0.0f : rho * ext,
                           ratio ? 0.0f : W.t);
     }
     if (MODE == VPT_RATIO_TRACKING) st.thr *= 1.0f - pAbsOverReal;
@@ -467,7 +465,7 @@ CRVPT_DEV bool scatterEvent(PathState& st, Walker& W, const SceneParams& P, cons
             const uint4 e2 = pathBlock(P, st, kEventSpace | st.events++);
             dir = sphereDirection(u01(e2.x), u01(e2.y), v3(1.0f, 0.0f, 0.0f));
         }
-        const float ph = evalPhase(P.phaseG, W.w, dir);
+        const float ph = evalPhase(P, W.w, dir);
         V3 L = toLight ? lightRadiance(P, p, li) * (probFactor * ph)
                        : skyRadiance(P, dir) * (probFactor * ph * (4.0f * kPi));
         if (isSpectral<MODE>()) L = L * vmin(st.weights, 100000.0f);
@@ -484,7 +482,7 @@ CRVPT_DEV bool scatterEvent(PathState& st, Walker& W, const SceneParams& P, cons
     }
     if (MODE == VPT_DECOMPOSITION_TRACKING) {
         // a real collision absorbs with probability 1 - albedo (DecompositionTracking.glsl:92-98)
-        if (u01(e.z) < 1.0f - P.albedo) {
+        if (u01(e.z) < P.absorption) {
             finishPath(st, v3(0.0f), true);
             return false;
         }
@@ -532,7 +530,7 @@ CRVPT_DEV bool rrtTraversalEnd(PathState& st, Walker& W, const SceneParams& P, c
     }
     st.flags += 1u << kBounceShift;
     const V3 x = W.o + W.w * st.resDist;
-    const V3 nw = phaseDirection(P.phaseG, W.w, u01(e.y), u01(e.z));
+    const V3 nw = phaseDirection(P, W.w, u01(e.y), u01(e.z));
     st.thr = st.resT;
     st.wSum = 0.0f;
     st.resT = 0.0f;
@@ -642,7 +640,7 @@ CRVPT_DEV bool stepPath(PathState& st, Walker& W, const SceneParams& P, const Su
         return true;
     }
     if (!(xi < pReal)) return true;  // null collision
-    const float pAbs = (1.0f - P.albedo) * pReal;
+    const float pAbs = P.absorption * pReal;
     const bool bounceLimit = P.maxBounces != 0u && (st.flags >> kBounceShift) >= P.maxBounces;
     if ((MODE != VPT_RATIO_TRACKING && xi < pAbs) || bounceLimit) {
         // absorption or a scattering event beyond the bounce limit: delta / ratio end with 0, next-event with its sum
@@ -673,7 +671,7 @@ CRVPT_DEV bool resolvePending(PathState& st, Walker& W, const SceneParams& P, co
     }
     const V3 p = W.o + W.w * W.t;
     // analog ratio tracking weights a real collision by its scattering probability, the albedo
-    return scatterEvent<MODE>(st, W, P, S, p, MODE == VPT_RATIO_TRACKING ? 1.0f - P.albedo : 0.0f, st.pendRho,
+    return scatterEvent<MODE>(st, W, P, S, p, MODE == VPT_RATIO_TRACKING ? P.absorption : 0.0f, st.pendRho,
                               grid, feat);
 }
 

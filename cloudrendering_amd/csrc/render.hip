@@ -18,6 +18,8 @@
 
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "path_tracer.hpp"
 
 namespace crvpt {
@@ -47,9 +49,10 @@ CRVPT_DEV bool claimToSlot(const SceneParams& P, uint32_t c, uint32_t& slot) {
     const uint32_t w = uint32_t(P.width), h = uint32_t(P.height);
     const uint32_t tx = (w + kClaimTile - 1u) / kClaimTile;
     const uint32_t perTile = tp * P.numFrames;
-    const uint32_t tile = c / perTile, rem = c - tile * perTile;
+    const uint32_t tile = udiv31(c, P.divTileClaims), rem = c - tile * perTile;
     const uint32_t s = rem / tp, q = rem - s * tp;
-    const uint32_t x = (tile % tx) * kClaimTile + q % kClaimTile, y = (tile / tx) * kClaimTile + q / kClaimTile;
+    const uint32_t ty = udiv31(tile, P.divTilesX);
+    const uint32_t x = (tile - ty * tx) * kClaimTile + q % kClaimTile, y = ty * kClaimTile + q / kClaimTile;
     if (x >= w || y >= h) return false;
     slot = s * (w * h) + y * w + x;
     return true;
@@ -108,7 +111,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(poolWaves<M
             const uint32_t want = uint32_t(__popcll(needMask));
             uint32_t base = 0;
             if (lane == leader) base = atomicAdd(&counter[part * kCounterStride], want);
-            base = __shfl(base, leader);
+            base = __builtin_amdgcn_readfirstlane(__shfl(base, leader));  // wave-uniform: the range state in SGPRs
             if (needs) {
                 const uint32_t c = base + uint32_t(__popcll(needMask & ((1ull << lane) - 1ull)));
                 if (c < count && claimToSlot(P, begin + c, st.unit)) {
@@ -826,9 +829,22 @@ static hipError_t launchRenderT(const SceneParams& P, const Grid& grid, const Su
     return hipGetLastError();
 }
 
-hipError_t launchRender(const SceneParams& P, const DenseGrid* dense, const NanoGrid* nano, const SupervoxelGrid& S,
+uint32_t maxFramesPerLaunch(int width, int height) {
+    // every claim index (screen tiles x tile pixels x frames, padding included) stays below 2^31 (udiv31)
+    const uint64_t tile = kClaimTile == 0u ? 1u : kClaimTile;
+    const uint64_t perFrame = ((uint64_t(width) + tile - 1u) / tile) * ((uint64_t(height) + tile - 1u) / tile) *
+                              tile * tile;
+    return uint32_t(std::min<uint64_t>(0x7FFFFFFFu / perFrame, 0xFFFFFFFFu));
+}
+
+hipError_t launchRender(const SceneParams& Pin, const DenseGrid* dense, const NanoGrid* nano, const SupervoxelGrid& S,
                         float4* paths, uint32_t* counter, const LaunchConfig& cfg, unsigned long long* stats,
                         float4* feat, hipStream_t stream) {
+    if (Pin.numFrames == 0u || Pin.numFrames > maxFramesPerLaunch(Pin.width, Pin.height)) return hipErrorInvalidValue;
+    SceneParams P = Pin;
+    const uint32_t tp = kClaimTile == 0u ? 1u : kClaimTile * kClaimTile;
+    P.divTileClaims = makeDiv31(tp * P.numFrames);
+    P.divTilesX = makeDiv31(kClaimTile == 0u ? 1u : (uint32_t(P.width) + kClaimTile - 1u) / kClaimTile);
     // variants: 64-bit offsets only for grids of 4 GiB and more; the trilinear code only for the trilinear filter
     const bool tri = P.interpolation == INTERP_TRILINEAR;
     if (nano && nano->wide) return launchRenderT(P, NanoAccess{*nano}, S, paths, counter, cfg, stats, feat, stream);

@@ -63,6 +63,8 @@ struct FbmParams {
 };
 
 // stats (nullable): 64-bit device counters [density taps, supervoxel majorant reads, paths], accumulated.
+// Largest numFrames of one launchRender call at this image size (claim indices stay below 2^31).
+uint32_t maxFramesPerLaunch(int width, int height);
 hipError_t launchRender(const SceneParams& P, const DenseGrid* dense, const NanoGrid* nano, const SupervoxelGrid& S,
                         float4* paths, uint32_t* counter, const LaunchConfig& cfg, unsigned long long* stats,
                         float4* feat, hipStream_t stream);

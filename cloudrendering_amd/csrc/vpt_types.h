@@ -36,6 +36,20 @@ enum TransmittanceEstimator : int32_t {
 
 constexpr int kMaxLights = 8;
 
+// Exact unsigned division of numerators below 2^31 by a divisor d fixed per launch: n / d = (n * m) >> s with
+// s = 31 + ceil(log2 d) and m = floor(2^s / d) + 1 < 2^32 (Granlund and Montgomery, "Division by invariant integers
+// using multiplication", PLDI 1994, theorem 4.2). Set on the host by makeDiv31.
+struct Div31 {
+    uint32_t m;
+    uint32_t s;
+};
+
+inline Div31 makeDiv31(uint32_t d) {
+    uint32_t l = 0;
+    while (l < 32u && (uint64_t(1) << l) < d) l++;
+    return Div31{uint32_t((uint64_t(1) << (31u + l)) / d + 1u), 31u + l};
+}
+
 // std140 light record of the reference (VptHeader.glsl:233-248), 64 bytes.
 struct Light {
     uint32_t lightType;   // 0 point, 1 spot, 2 directional
@@ -94,6 +108,25 @@ struct SupervoxelGrid {
     int32_t grid[3];        // majorant grid dimensions
     int32_t refSize;        // supervoxel edge of the reference tables
     int32_t refGrid[3];
+    // Walker constants of the two cell grids, computed on the host from the scene box (uniform values the kernels
+    // would otherwise derive at launch and hold in vector registers): cell edge in world units and cells per world
+    // unit per axis, and the upper corner of the reference grid (it overhangs the box when the grid size is not a
+    // multiple of the edge).
+    float cellWorld[3];
+    float cellsPerWorld[3];
+    float refCellWorld[3];
+    float refCellsPerWorld[3];
+    float refHi[3];
+};
+
+// Constants of the Henyey-Greenstein phase function derived from g once per launch (SceneParams::phaseC).
+enum PhaseConstant : int {
+    PHASE_ONE_MINUS_G2 = 0,  // 1 - g^2
+    PHASE_ONE_PLUS_G2 = 1,   // 1 + g^2
+    PHASE_TWO_G = 2,         // 2 g
+    PHASE_HALF_INV_G = 3,    // 0.5 / g (0 for |g| < 0.001, where sampling is isotropic)
+    PHASE_ONE_MINUS_G = 4,   // 1 - g
+    PHASE_PDF_SCALE = 5,     // (1 - g^2) / (4 pi)
 };
 
 // Per-launch scene constants (the build's counterpart of the reference's UniformData, VptHeader.glsl:45-112).
@@ -113,6 +146,7 @@ struct SceneParams {
     float albedoRGB[3];     // scatteringAlbedo (spectral trackers)
     float extinctionMax;    // maxComponent(extinction): majorant of the spectral trackers
     float phaseG;
+    float phaseC[6];        // Henyey-Greenstein constants of phaseG, indexed by PhaseConstant (set on the host)
     int32_t mode;
     int32_t interpolation;
     int32_t envKind;
@@ -133,6 +167,15 @@ struct SceneParams {
     uint32_t numFrames;     // samples per pixel in this launch
     uint32_t maxBounces;    // scattering events per path before it ends with zero radiance; 0 = unbounded
     uint64_t numUnits;      // width * height * numFrames
+    // Host-computed helpers of the per-path index arithmetic (uniform values the kernels would otherwise derive at
+    // launch in vector registers): divisions by the image width, the pixel count, the claims of one screen tile
+    // (tile pixels x numFrames) and the tiles per image row; the image size as floats; 1 - scatteringAlbedo.x.
+    Div31 divWidth;
+    Div31 divPixels;
+    Div31 divTileClaims;
+    Div31 divTilesX;
+    float widthF, heightF;
+    float absorption;
 };
 
 // One request of a batched render (the batched queue's device table; no reference counterpart). The scene
