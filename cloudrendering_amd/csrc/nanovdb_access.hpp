@@ -159,7 +159,7 @@ struct NanoAccessT {
     // multiply-adds per tap; the index coordinate differs from the step-by-step evaluation only in rounding.
     CRVPT_DEV float lookupWorldJ(const SceneParams& P, V3 pos, float jx, float jy, float jz, LeafCache& cache) const {
         V3 idx;
-        if (g.tapDiag) {
+        if (!WIDE || g.tapDiag) {  // the narrow accessors are only used for grids with a diagonal map
             idx = v3(pos.x * g.tapMat[0] + g.tapOff[0], pos.y * g.tapMat[4] + g.tapOff[1],
                      pos.z * g.tapMat[8] + g.tapOff[2]);
         } else {
@@ -206,7 +206,7 @@ struct NanoAccessT {
     }
 };
 using NanoAccess = NanoAccessT<true>;        // any grid (lookups, tables, transmittance, batched queue)
-using NanoAccessNarrow = NanoAccessT<false>;  // grids below 4 GiB (the single-frame path kernels)
+using NanoAccessNarrow = NanoAccessT<false>;  // grids below 4 GiB with axis-aligned voxels (single-frame path kernels)
 using NanoAccessNarrowPoint = NanoAccessT<false, false>;  // ... with a point filter
 
 }  // namespace crvpt

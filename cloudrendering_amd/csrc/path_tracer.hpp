@@ -37,8 +37,8 @@ constexpr float kInf = 3.0e38f;
 // dense 128^3 next-event workload): cell crossings per walker call (more crossings per step means fewer steps but
 // more lanes idle while the others cross), and the waves per SIMD the pool kernels are compiled for (128 registers
 // per lane at 4 waves, 96 at 5, 80 at 6). Next-event tracking carries the most per-lane state (the shadow ray's) and
-// runs best at 4 waves, delta and ratio tracking the least (6 waves); residual ratio and next-event tracking at 2
-// crossings.
+// runs best at 4 waves, spectral delta tracking at 5, the others at 6 (residual ratio tracking: 351 ms against 388 ms
+// at 5 waves per 2^28-path launch); residual ratio and next-event tracking at 2 crossings.
 template <int MODE>
 constexpr int maxCrossings() {
     return (MODE == VPT_RESIDUAL_RATIO_TRACKING || MODE == VPT_NEXT_EVENT_TRACKING ||
@@ -47,7 +47,7 @@ constexpr int maxCrossings() {
 template <int MODE>
 constexpr int poolWaves() {
     if (MODE == VPT_NEXT_EVENT_TRACKING || MODE == VPT_NEXT_EVENT_TRACKING_SPECTRAL) return 4;
-    return (MODE == VPT_DELTA_TRACKING || MODE == VPT_RATIO_TRACKING) ? 6 : 5;
+    return MODE == VPT_SPECTRAL_DELTA_TRACKING ? 5 : 6;
 }
 
 template <int MODE>
@@ -586,7 +586,8 @@ template <int MODE, class Grid>
 CRVPT_DEV bool stepPath(PathState& st, Walker& W, const SceneParams& P, const SupervoxelGrid& S, const Grid& grid,
                         typename Grid::Cache& gc, float4* feat) {
     const uint4 r = pathBlock(P, st, st.step++);
-    const int ev = walkAdvance<MODE>(W, P, S, -__logf(uOpen(r.x)));
+    // Exp(1) free-path budget -ln(u) = -log2(u) ln 2 from the hardware log2 (u in [2^-24, 1]: no denormal inputs)
+    const int ev = walkAdvance<MODE>(W, P, S, __builtin_amdgcn_logf(uOpen(r.x)) * -0.693147182f);
     if (ev == WALK_MOVED) return true;
     if (MODE == VPT_RESIDUAL_RATIO_TRACKING) {
         st.thr *= __expf(-W.ctrl);  // control transmittance of the walked stretch

@@ -845,9 +845,11 @@ hipError_t launchRender(const SceneParams& Pin, const DenseGrid* dense, const Na
     const uint32_t tp = kClaimTile == 0u ? 1u : kClaimTile * kClaimTile;
     P.divTileClaims = makeDiv31(tp * P.numFrames);
     P.divTilesX = makeDiv31(kClaimTile == 0u ? 1u : (uint32_t(P.width) + kClaimTile - 1u) / kClaimTile);
-    // variants: 64-bit offsets only for grids of 4 GiB and more; the trilinear code only for the trilinear filter
+    // variants: 64-bit offsets and the full index map only for grids of 4 GiB and more or with rotated voxels; the
+    // trilinear code only for the trilinear filter
     const bool tri = P.interpolation == INTERP_TRILINEAR;
-    if (nano && nano->wide) return launchRenderT(P, NanoAccess{*nano}, S, paths, counter, cfg, stats, feat, stream);
+    if (nano && (nano->wide || !nano->tapDiag))
+        return launchRenderT(P, NanoAccess{*nano}, S, paths, counter, cfg, stats, feat, stream);
     if (nano && tri) return launchRenderT(P, NanoAccessNarrow{*nano}, S, paths, counter, cfg, stats, feat, stream);
     if (nano) return launchRenderT(P, NanoAccessNarrowPoint{*nano}, S, paths, counter, cfg, stats, feat, stream);
     if (!tri) return launchRenderT(P, DenseAccessPoint{*dense}, S, paths, counter, cfg, stats, feat, stream);
