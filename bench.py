@@ -740,6 +740,27 @@ def copy_calibration(torch):
     print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}), flush=True)
 
 
+def kpn_inference(torch, calls: int, res: int = 1024):
+    """KPN inference alone (packing, six bf16 MFMA convolutions, softmax filter) on seeded inputs of one res x res
+    frame, `calls` times: the dispatches the MFMA counter passes profile."""
+    from cloudrendering_amd import kpn, vpt
+
+    vpt.initialize()
+    net = kpn.KpnDenoiser(seed=1)
+    g = torch.Generator(device="cuda").manual_seed(0)
+    f = torch.rand((1, kpn.NUM_INPUTS, res, res), device="cuda", generator=g)
+    x = torch.rand((1, 3, res, res), device="cuda", generator=g)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(calls):
+        y = net.denoise(f, x)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t) * 1e3 / calls
+    print(json.dumps({"kpn_inference_calls": calls, "res": res, "ms_per_call": round(ms, 4),
+                      "output_mean": float(y.mean())}), flush=True)
+    vpt.cleanup()
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -758,6 +779,8 @@ def main() -> int:
     ap.add_argument("--no-kpn", action="store_true",
                     help="c2_sparse512: skip the denoiser half of the metric (the C4 workload after the timed steps)")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--kpn-inference", type=int, default=0, metavar="CALLS",
+                    help="profiling helper: CALLS KPN inferences on seeded 1024^2 inputs, then exit")
     args = ap.parse_args()
     if args.steps <= 0:
         args.steps = {"c2_sparse512": 2, "c1_dense128": 12}.get(args.workload, 3)
@@ -770,6 +793,9 @@ def main() -> int:
 
     if args.copy_calibration:
         copy_calibration(torch)
+        return 0
+    if args.kpn_inference > 0:
+        kpn_inference(torch, args.kpn_inference)
         return 0
 
     from cloudrendering_amd import vpt
