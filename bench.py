@@ -761,6 +761,33 @@ def kpn_inference(torch, calls: int, res: int = 1024):
     vpt.cleanup()
 
 
+def denoise_frames(torch, frames: int, res: int = 1024):
+    """EAW (C4's 4 passes on 4-spp frames) and SVGF (1-spp frames) on the denoiser workload's cloud at res x res:
+    the dispatches the denoiser kernel traces profile. Prints the wall time per frame of each."""
+    import numpy as np
+
+    from cloudrendering_amd import vpt
+
+    vpt.initialize()
+    _c4_scene(vpt, 999)
+    _c4_camera(vpt, np.random.default_rng(999))
+    probe = torch.zeros((3, res, res), dtype=torch.float32, device="cuda")
+    out = {"res": res, "frames": frames}
+    for name, spp, prop in (("EAW", C4["input_spp"], str(C4["eaw_iterations"])), ("SVGF", 1, "5")):
+        vpt.set_denoiser(name)
+        vpt.set_denoiser_property("maxNumIterations", prop)
+        vpt.render_frame(probe, spp)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(frames):
+            vpt.render_frame(probe, spp)
+        torch.cuda.synchronize()
+        out[f"{name.lower()}_ms_per_frame"] = round((time.perf_counter() - t) * 1e3 / frames, 3)
+    vpt.set_denoiser("None")
+    print(json.dumps(out), flush=True)
+    vpt.cleanup()
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -781,6 +808,9 @@ def main() -> int:
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--kpn-inference", type=int, default=0, metavar="CALLS",
                     help="profiling helper: CALLS KPN inferences on seeded 1024^2 inputs, then exit")
+    ap.add_argument("--denoise-frames", type=int, default=0, metavar="FRAMES",
+                    help="profiling helper: FRAMES 4-spp EAW (4 passes) and FRAMES 1-spp SVGF renders of the "
+                         "denoiser workload's cloud at 1024^2, then exit")
     args = ap.parse_args()
     if args.steps <= 0:
         args.steps = {"c2_sparse512": 2, "c1_dense128": 12}.get(args.workload, 3)
@@ -796,6 +826,9 @@ def main() -> int:
         return 0
     if args.kpn_inference > 0:
         kpn_inference(torch, args.kpn_inference)
+        return 0
+    if args.denoise_frames > 0:
+        denoise_frames(torch, args.denoise_frames)
         return 0
 
     from cloudrendering_amd import vpt

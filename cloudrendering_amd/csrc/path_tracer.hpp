@@ -691,8 +691,10 @@ CRVPT_DEV void flushCounters(unsigned long long* stats, uint32_t taps, uint32_t 
 }
 
 #ifndef CRVPT_RESOLVE_LANES
-#define CRVPT_RESOLVE_LANES 16
+#define CRVPT_RESOLVE_LANES 24
 #endif
-constexpr int kResolveLanes = CRVPT_RESOLVE_LANES;  // pending lanes that trigger a resolve pass of the wave
+// Pending lanes that trigger a resolve pass of the wave (C2 sweep, 8 / 16 / 24 / 32 lanes: 1546 / 1652 / 1669 / 1650
+// Mpaths/s).
+constexpr int kResolveLanes = CRVPT_RESOLVE_LANES;
 
 }  // namespace crvpt

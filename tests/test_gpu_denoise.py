@@ -52,6 +52,26 @@ def test_eaw_matches_restatement(vpt, oracle, iterations):
     np.testing.assert_allclose(out, ref, rtol=2e-4, atol=2e-6)
 
 
+def test_eaw_odd_size_all_step_widths(vpt, oracle):
+    """A 44 x 37 frame through five passes (step widths 1 to 16): the passes stage each phase of the dilated lattice
+    in LDS tiles, so widths that do not divide the image and taps clamped at every edge are covered."""
+    print("case: EAW 44x37, 5 iterations", flush=True)
+    torch = _torch()
+    _scene(vpt, oracle)
+    vpt.set_denoiser("EAW")
+    vpt.set_denoiser_property("maxNumIterations", "5")
+    vpt.set_use_feature_maps(["First X", "Normal"])
+    t = torch.zeros((4, 37, 44), dtype=torch.float32, device="cuda")
+    out = _hwc(vpt.render_frame(t, 8))
+    res = _hwc(vpt.get_feature_map_from_string(t, "Result"))
+    pos = _hwc(vpt.get_feature_map_from_string(t, "First X"))
+    nrm = _hwc(vpt.get_feature_map_from_string(t, "Normal"))
+    vpt.set_denoiser("None")
+    vpt.set_use_feature_maps([])
+    ref = D.eaw(res, pos, nrm, iterations=5)
+    np.testing.assert_allclose(out, ref, rtol=2e-4, atol=2e-6)
+
+
 def test_svgf_matches_restatement_over_frames(vpt, oracle):
     print("case: SVGF 4 frames", flush=True)
     torch = _torch()
