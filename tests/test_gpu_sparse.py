@@ -133,7 +133,23 @@ def test_grid_beyond_4_gib(vpt):
     expect[inside] = d[q[inside, 2], q[inside, 1], q[inside, 0]]
     del d
     np.testing.assert_array_equal(got.view(np.uint32), expect.view(np.uint32))
+    # the single-frame path kernels' 64-bit variant (grids of 4 GiB and more) against the dense field of the same
+    # volume: delta tracking, stochastic filtering, a thin medium so paths stay short
+    print("case: 1100^3 render, sparse 64-bit path kernels vs dense", flush=True)
+    vpt.set_vpt_mode(0)
+    vpt.set_grid_interpolation_type("stochastic")
+    vpt.set_extinction_base([1.0, 1.0, 1.0])
+    vpt.set_extinction_scale(6.0)
+    vpt.set_scattering_albedo([0.9, 0.9, 0.9])
+    vpt.set_phase_g(0.5)
+    vpt.set_use_builtin_environment_map("Default")
+    vpt.set_use_lights(False)
+    vpt.set_seed_offset(0)
+    m_s, s_s, _ = gpu_image_stats(vpt, 256, 24)
     vpt.set_use_sparse_grid(False)
+    m_d, s_d, _ = gpu_image_stats(vpt, 256, 24)
+    assert_means_equal(m_s, s_s, m_d, s_d, "1100^3 sparse (64-bit offsets) vs dense")
+    assert m_d.min() > 0.01
 
 
 @pytest.mark.parametrize("interp", ["nearest", "stochastic"])
