@@ -728,6 +728,17 @@ def load_traffic(workload: str):
     return load_pmc(workload).get("traffic_bytes_per_launch")
 
 
+def valu_issue(workload: str):
+    """The dominant kernel's vector-instruction issue from the committed PMC summary: VALU instructions per SIMD per
+    cycle (SQ_INSTS_VALU over 1024 SIMDs and the per-XCD GRBM_GUI_ACTIVE cycles) and the fraction of lanes active per
+    issued instruction (SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU)), or None without one."""
+    c = load_pmc(workload).get("counters_per_dispatch", {})
+    if not all(k in c for k in ("SQ_INSTS_VALU", "GRBM_GUI_ACTIVE", "SQ_THREAD_CYCLES_VALU", "SQ_ACTIVE_INST_VALU")):
+        return None
+    return {"insts_per_simd_cycle": round(c["SQ_INSTS_VALU"] / 1024 / (c["GRBM_GUI_ACTIVE"] / 8), 4),
+            "lane_utilisation": round(c["SQ_THREAD_CYCLES_VALU"] / (64 * c["SQ_ACTIVE_INST_VALU"]), 4)}
+
+
 def copy_calibration(torch):
     """A plain streaming copy kernel of 256 MiB read + 256 MiB written (an elementwise multiply by 1, so it runs as
     a kernel dispatch the PMC counters see): the FETCH_SIZE / WRITE_SIZE scale factor for the PMC summaries."""
@@ -945,6 +956,7 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": load_traffic(args.workload),
                      "valu_insts_per_launch": load_pmc(args.workload).get("valu_insts_per_launch"),
+                     "valu_issue": valu_issue(args.workload),
                      "kernel": kernel, "launches_per_step": launches,
                      "launch_ms": round(dom_ms / launches, 4), "algorithmic_bytes_per_launch": int(b_alg / launches),
                      "density_taps": int(taps), "supervoxel_reads": int(svr), "tree_descents": int(desc),
