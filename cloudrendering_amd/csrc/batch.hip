@@ -58,7 +58,7 @@ CRVPT_DEV uint32_t unitPartBegin(uint32_t totalUnits, uint32_t part) {
 }
 
 template <int MODE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(poolWaves<MODE>()))) void k_batchPool(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(batchPoolWaves<MODE>()))) void k_batchPool(
         const BatchRequest* __restrict__ reqs, const uint64_t* __restrict__ unitBegins, int nreq, uint32_t totalUnits,
         uint32_t samplesPerUnit, const uint32_t* __restrict__ frameOffset, float4* __restrict__ records,
         uint32_t* __restrict__ counter, int stepsPerRefill) {

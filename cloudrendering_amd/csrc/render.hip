@@ -834,7 +834,7 @@ uint32_t maxFramesPerLaunch(int width, int height) {
     const uint64_t tile = kClaimTile == 0u ? 1u : kClaimTile;
     const uint64_t perFrame = ((uint64_t(width) + tile - 1u) / tile) * ((uint64_t(height) + tile - 1u) / tile) *
                               tile * tile;
-    return uint32_t(std::min<uint64_t>(0x7FFFFFFFu / perFrame, 0xFFFFFFFFu));
+    return uint32_t(std::min<uint64_t>(0x7FFFFFFFu / std::max<uint64_t>(perFrame, 1u), 0xFFFFFFFFu));
 }
 
 hipError_t launchRender(const SceneParams& Pin, const DenseGrid* dense, const NanoGrid* nano, const SupervoxelGrid& S,

@@ -45,6 +45,7 @@ struct Div31 {
 };
 
 inline Div31 makeDiv31(uint32_t d) {
+    if (d == 0u) d = 1u;  // callers pass sizes of at least 1; a zero divisor is treated as 1
     uint32_t l = 0;
     while (l < 32u && (uint64_t(1) << l) < d) l++;
     return Div31{uint32_t((uint64_t(1) << (31u + l)) / d + 1u), 31u + l};
