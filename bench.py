@@ -220,7 +220,7 @@ def run_c3(args, vpt, torch):
 
 
 C4 = {"crops": 4096, "crop": 128, "clouds": 16, "input_spp": 4, "mixed_input_spp": [4, 16, 64, 256],
-      "eval_input_spp": [4, 16, 64, 128, 256], "target_spp": 1024, "batch": 16, "train_steps": 6000, "lr": 1e-3,
+      "eval_input_spp": [4, 16, 32, 48, 64, 128, 256], "target_spp": 1024, "batch": 16, "train_steps": 6000, "lr": 1e-3,
       "eval_res": 1024, "ref_spp": 16384, "plain_spp": [4, 16, 64, 256, 1024, 4096], "max_bounces": 128,
       "eaw_iterations": 4, "svgf_max_frames": 256}
 
