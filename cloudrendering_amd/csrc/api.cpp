@@ -183,7 +183,7 @@ struct Renderer {
 
     // ---- device state ----
     hipStream_t stream = nullptr;
-    DevBuf dRaw, dNorm, dNvdb, dMajorant, dRrt, dDecomp, dEnv, dAcc, dPaths, dCounter, dStage, dRays;
+    DevBuf dRaw, dNorm, dNvdb, dMajorant, dRrt, dDecomp, dEnv, dAcc, dPaths, dCounter, dStage, dRays, dCam;
     DevBuf dSvMin, dSvMax, dSvAvg, dDen0, dDen1, dStats, dFeat, dFeatAcc, dSvgf, dKpn, dBlockLeaf, dBlockTile, dBlockBad;
     // flat 8^3-block index of the NanoVDB grid (NanoGrid::blockLeaf), valid for volume version blockTableVersion
     bool blockTableOk = false;
@@ -839,7 +839,7 @@ int crvpt_cleanup(void) {
         for (DevBuf* b : {&r->dRaw, &r->dNorm, &r->dNvdb, &r->dMajorant, &r->dRrt, &r->dDecomp, &r->dEnv, &r->dAcc,
                           &r->dPaths, &r->dCounter, &r->dStage, &r->dRays, &r->dSvMin, &r->dSvMax, &r->dSvAvg,
                           &r->dDen0, &r->dDen1, &r->dStats, &r->dFeat, &r->dFeatAcc, &r->dSvgf, &r->dKpn,
-                          &r->dBlockLeaf, &r->dBlockTile, &r->dBlockBad})
+                          &r->dBlockLeaf, &r->dBlockTile, &r->dBlockBad, &r->dCam})
             b->release();
         (void)hipEventDestroy(r->ev0);
         (void)hipEventDestroy(r->ev1);
@@ -878,6 +878,10 @@ int crvpt_render_frame(float* out, int channels, int height, int width, int fram
     SceneParams P;
     SupervoxelGrid S;
     fillSceneParams(r, width, height, P, S);
+    // the path kernels' camera rays read the matrix from device memory (SceneParams::camMatrix)
+    HIP_TRY(r.dCam.ensure(sizeof(P.invViewProj)));
+    HIP_TRY(launchStoreMatrix(r.dCam.as<float>(), P.invViewProj, s));
+    P.camMatrix = r.dCam.as<float>();
     DenseGrid dg;
     NanoGrid ng;
     bool sp;

@@ -133,7 +133,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(batchPoolWa
                 st.frame = R.P.frameBase + frameOff + sample;
                 sample++;
                 left--;
-                if (!startPathAt<MODE>(st, W, R.P, R.S, nullptr)) finishSample();
+                if (!startPathAt<MODE>(st, W, R.P, R.S, nullptr, R.P.invViewProj)) finishSample();
             }
             if (__ballot(!(st.flags & PF_ALIVE) && (left > 0u || !exhausted)) == 0ull) break;
         }

@@ -108,10 +108,11 @@ CRVPT_DEV void mulMat4(const float* m, float x, float y, float z, float w, float
 }
 
 // createCameraRay: unproject the near (z = 0) and far (z = 1) NDC points through inverse(P V).
-CRVPT_DEV void createCameraRay(const SceneParams& P, float sx, float sy, V3& x, V3& w) {
+// M: the column-major inverse view-projection (SceneParams::invViewProj or a copy of it).
+CRVPT_DEV void createCameraRay(const float* M, float sx, float sy, V3& x, V3& w) {
     float px, py, pz, pw, tx, ty, tz, tw;
-    mulMat4(P.invViewProj, sx, sy, 0.0f, 1.0f, px, py, pz, pw);
-    mulMat4(P.invViewProj, sx, sy, 1.0f, 1.0f, tx, ty, tz, tw);
+    mulMat4(M, sx, sy, 0.0f, 1.0f, px, py, pz, pw);
+    mulMat4(M, sx, sy, 1.0f, 1.0f, tx, ty, tz, tw);
     x = v3(px / pw, py / pw, pz / pw);
     w = normalize(v3(tx / tw, ty / tw, tz / tw) - x);
 }

@@ -355,17 +355,19 @@ CRVPT_DEV V3 phaseDirection(const SceneParams& P, V3 d, float u0, float u1) {
 }
 
 // Camera ray of the path: pixel jitter from the camera block (Clouds.glsl:80-90).
-CRVPT_DEV void cameraRay(const SceneParams& P, const PathState& st, V3& x, V3& w) {
+CRVPT_DEV void cameraRay(const SceneParams& P, const float* cam, const PathState& st, V3& x, V3& w) {
     const uint4 b = pathBlock(P, st, kCameraBlock);
     const uint32_t py = udiv31(st.pix, P.divWidth), px = st.pix - py * uint32_t(P.width);
     const float sx = 2.0f * (float(px) + u01(b.x)) / P.widthF - 1.0f;
     const float sy = 2.0f * (float(py) + u01(b.y)) / P.heightF - 1.0f;
-    createCameraRay(P, sx, sy, x, w);
+    createCameraRay(cam, sx, sy, x, w);
 }
 
-// Starts the camera path of (st.pix, st.frame). False when the camera ray misses the box (the path is finished).
+// Starts the camera path of (st.pix, st.frame) through the inverse view-projection `cam`. False when the camera ray
+// misses the box (the path is finished).
 template <int MODE>
-CRVPT_DEV bool startPathAt(PathState& st, Walker& W, const SceneParams& P, const SupervoxelGrid& S, float4* feat) {
+CRVPT_DEV bool startPathAt(PathState& st, Walker& W, const SceneParams& P, const SupervoxelGrid& S, float4* feat,
+                           const float* cam) {
     st.step = 0;
     st.events = 0;
     st.flags = PF_ALIVE;
@@ -377,12 +379,20 @@ CRVPT_DEV bool startPathAt(PathState& st, Walker& W, const SceneParams& P, const
     st.weights = v3(1.0f);
     if (feat) feat[3 * size_t(st.unit)] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);  // no first event yet
     V3 x, w;
-    cameraRay(P, st, x, w);
+    cameraRay(P, cam, st, x, w);
     if (!walkerStart<MODE>(W, P, S, x, w, kInf)) {
         finishPath(st, skyRadiance(P, w), false);
         return false;
     }
     return true;
+}
+
+// The camera matrix the single-frame path kernels read: the device copy, which keeps the matrix out of the scalar
+// registers (C2 sweep: delta, ratio and decomposition tracking 7-9 % faster), except for next-event tracking, which
+// ran 3.5 % slower with it and reads the launch constants.
+template <int MODE>
+CRVPT_DEV const float* poolCamera(const SceneParams& P) {
+    return isNee<MODE>() ? P.invViewProj : P.camMatrix;
 }
 
 // Starts the path of launch unit st.unit = sample * (W*H) + pixel.
@@ -391,7 +401,7 @@ CRVPT_DEV bool startPath(PathState& st, Walker& W, const SceneParams& P, const S
     const uint32_t s = udiv31(st.unit, P.divPixels);
     st.pix = st.unit - s * (uint32_t(P.width) * uint32_t(P.height));
     st.frame = P.frameBase + s;
-    return startPathAt<MODE>(st, W, P, S, feat);
+    return startPathAt<MODE>(st, W, P, S, feat, poolCamera<MODE>(P));
 }
 
 // getCloudFiniteDifference (VptUtils.glsl:726-748) at p for the normal feature map: central differences of the

@@ -320,7 +320,7 @@ __global__ void k_accumulateFeatures(SceneParams P, const float4* __restrict__ p
         st.pix = p;
         st.frame = P.frameBase + f;
         V3 xo, w;
-        cameraRay(P, st, xo, w);
+        cameraRay(P, P.invViewProj, st, xo, w);
         auto mix4 = [&](float4 o, float4 v) {
             return make_float4(o.x * b + v.x * a, o.y * b + v.y * a, o.z * b + v.z * a, o.w * b + v.w * a);
         };
@@ -826,6 +826,21 @@ static hipError_t launchRenderT(const SceneParams& P, const Grid& grid, const Su
             launchPool<VPT_DELTA_TRACKING>(gridDim, block, stream, P, grid, S, paths, counter, st, stats, feat);
             break;
     }
+    return hipGetLastError();
+}
+
+struct Mat16 {
+    float m[16];
+};
+
+__global__ void k_storeMatrix(float* __restrict__ dst, Mat16 m) {
+    if (threadIdx.x < 16) dst[threadIdx.x] = m.m[threadIdx.x];
+}
+
+hipError_t launchStoreMatrix(float* dst, const float* m16, hipStream_t stream) {
+    Mat16 m;
+    for (int i = 0; i < 16; i++) m.m[i] = m16[i];
+    hipLaunchKernelGGL(k_storeMatrix, dim3(1), dim3(64), 0, stream, dst, m);
     return hipGetLastError();
 }
 

@@ -177,6 +177,9 @@ struct SceneParams {
     Div31 divTilesX;
     float widthF, heightF;
     float absorption;
+    // Device copy of invViewProj (16 floats) for the single-frame path kernels' camera rays: read from memory at
+    // path start, so the matrix does not occupy scalar registers across the persistent loop. Set by render_frame.
+    const float* camMatrix;
 };
 
 // One request of a batched render (the batched queue's device table; no reference counterpart). The scene
