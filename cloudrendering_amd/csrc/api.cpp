@@ -183,7 +183,7 @@ struct Renderer {
 
     // ---- device state ----
     hipStream_t stream = nullptr;
-    DevBuf dRaw, dNorm, dNvdb, dMajorant, dRrt, dDecomp, dEnv, dAcc, dPaths, dCounter, dStage, dRays, dCam;
+    DevBuf dRaw, dNorm, dNvdb, dMajorant, dRrt, dDecomp, dEnv, dAcc, dPaths, dCounter, dStage, dRays, dParams;
     DevBuf dSvMin, dSvMax, dSvAvg, dDen0, dDen1, dStats, dFeat, dFeatAcc, dSvgf, dKpn, dBlockLeaf, dBlockTile, dBlockBad;
     // flat 8^3-block index of the NanoVDB grid (NanoGrid::blockLeaf), valid for volume version blockTableVersion
     bool blockTableOk = false;
@@ -839,7 +839,7 @@ int crvpt_cleanup(void) {
         for (DevBuf* b : {&r->dRaw, &r->dNorm, &r->dNvdb, &r->dMajorant, &r->dRrt, &r->dDecomp, &r->dEnv, &r->dAcc,
                           &r->dPaths, &r->dCounter, &r->dStage, &r->dRays, &r->dSvMin, &r->dSvMax, &r->dSvAvg,
                           &r->dDen0, &r->dDen1, &r->dStats, &r->dFeat, &r->dFeatAcc, &r->dSvgf, &r->dKpn,
-                          &r->dBlockLeaf, &r->dBlockTile, &r->dBlockBad, &r->dCam})
+                          &r->dBlockLeaf, &r->dBlockTile, &r->dBlockBad, &r->dParams})
             b->release();
         (void)hipEventDestroy(r->ev0);
         (void)hipEventDestroy(r->ev1);
@@ -878,10 +878,10 @@ int crvpt_render_frame(float* out, int channels, int height, int width, int fram
     SceneParams P;
     SupervoxelGrid S;
     fillSceneParams(r, width, height, P, S);
-    // the path kernels' camera rays read the matrix from device memory (SceneParams::camMatrix)
-    HIP_TRY(r.dCam.ensure(sizeof(P.invViewProj)));
-    HIP_TRY(launchStoreMatrix(r.dCam.as<float>(), P.invViewProj, s));
-    P.camMatrix = r.dCam.as<float>();
+    // the path kernels read their rarely used constants (camera, sky) from a device copy (SceneParams::mem)
+    HIP_TRY(r.dParams.ensure(sizeof(SceneParams)));
+    P.mem = r.dParams.as<SceneParams>();
+    HIP_TRY(launchStoreParams(r.dParams.as<SceneParams>(), P, s));
     DenseGrid dg;
     NanoGrid ng;
     bool sp;
@@ -2032,6 +2032,8 @@ int uploadBatchTable(Batch& b, uint32_t samplesPerUnit) {
         return fail(CRVPT_ERR_INVALID_ARGUMENT,
                     "batch: more than 2^32 units; raise the samples per unit or split the batch");
     HIP_TRY(b.dReqs.ensure(n * sizeof(BatchRequest)));
+    // each request's constants point at their own record in the device table (SceneParams::mem)
+    for (size_t i = 0; i < n; i++) b.reqs[i].P.mem = &b.dReqs.as<BatchRequest>()[i].P;
     HIP_TRY(b.dUnitBegins.ensure(n * sizeof(uint64_t)));
     HIP_TRY(b.dPixBegins.ensure(n * sizeof(uint64_t)));
     HIP_TRY(b.dRecords.ensure(std::max<uint64_t>(units, 1) * sizeof(float4)));

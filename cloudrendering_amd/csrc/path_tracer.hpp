@@ -354,6 +354,19 @@ CRVPT_DEV V3 phaseDirection(const SceneParams& P, V3 d, float u0, float u1) {
     return t0 * (sinTheta * __sinf(phi)) + t1 * (sinTheta * __cosf(phi)) + d * cosTheta;
 }
 
+// Where the path kernels read the camera matrix and the sky: the device copy of the constants (P.mem) keeps them out
+// of the scalar registers. Measured on the 512^3 sparse sweep: the camera from memory makes delta, ratio and
+// decomposition tracking 7-9 % faster and next-event tracking 3.5 % slower; the sky from memory makes residual ratio
+// tracking 8 % faster and delta and ratio tracking 6-7 % slower.
+template <int MODE>
+CRVPT_DEV const float* poolCamera(const SceneParams& P) {
+    return isNee<MODE>() ? P.invViewProj : P.mem->invViewProj;
+}
+template <int MODE>
+CRVPT_DEV const SceneParams& skyParams(const SceneParams& P) {
+    return (MODE == VPT_RESIDUAL_RATIO_TRACKING || MODE == VPT_DECOMPOSITION_TRACKING) ? *P.mem : P;
+}
+
 // Camera ray of the path: pixel jitter from the camera block (Clouds.glsl:80-90).
 CRVPT_DEV void cameraRay(const SceneParams& P, const float* cam, const PathState& st, V3& x, V3& w) {
     const uint4 b = pathBlock(P, st, kCameraBlock);
@@ -381,18 +394,10 @@ CRVPT_DEV bool startPathAt(PathState& st, Walker& W, const SceneParams& P, const
     V3 x, w;
     cameraRay(P, cam, st, x, w);
     if (!walkerStart<MODE>(W, P, S, x, w, kInf)) {
-        finishPath(st, skyRadiance(P, w), false);
+        finishPath(st, skyRadiance(skyParams<MODE>(P), w), false);
         return false;
     }
     return true;
-}
-
-// The camera matrix the single-frame path kernels read: the device copy, which keeps the matrix out of the scalar
-// registers (C2 sweep: delta, ratio and decomposition tracking 7-9 % faster), except for next-event tracking, which
-// ran 3.5 % slower with it and reads the launch constants.
-template <int MODE>
-CRVPT_DEV const float* poolCamera(const SceneParams& P) {
-    return isNee<MODE>() ? P.invViewProj : P.camMatrix;
 }
 
 // Starts the path of launch unit st.unit = sample * (W*H) + pixel.
@@ -488,7 +493,7 @@ CRVPT_DEV bool scatterEvent(PathState& st, Walker& W, const SceneParams& P, cons
         }
         const float ph = evalPhase(P, W.w, dir);
         V3 L = toLight ? lightRadiance(P, p, li) * (probFactor * ph)
-                       : skyRadiance(P, dir) * (probFactor * ph * (4.0f * kPi));
+                       : skyRadiance(skyParams<MODE>(P), dir) * (probFactor * ph * (4.0f * kPi));
         if (isSpectral<MODE>()) L = L * vmin(st.weights, 100000.0f);
         st.mainX = p;
         st.mainW = nw;
@@ -510,8 +515,8 @@ CRVPT_DEV bool scatterEvent(PathState& st, Walker& W, const SceneParams& P, cons
     }
     if (!walkerStart<MODE>(W, P, S, p, nw, kInf, false, MODE == VPT_DECOMPOSITION_TRACKING)) {
         const V3 c = isNee<MODE>() ? st.color
-                                   : skyRadiance(P, nw) * (isSpectral<MODE>() ? vmin(st.weights, 100000.0f)
-                                                                              : v3(st.thr));
+                                   : skyRadiance(skyParams<MODE>(P), nw) *
+                                         (isSpectral<MODE>() ? vmin(st.weights, 100000.0f) : v3(st.thr));
         finishPath(st, c, true);
         return false;
     }
@@ -531,8 +536,8 @@ CRVPT_DEV bool endEvent(PathState& st, Walker& W, const SceneParams& P, const Su
     const bool hv = (st.flags & PF_HAS_VALUE) != 0;
     V3 c;
     const V3 wts = isSpectral<MODE>() ? vmin(st.weights, 100000.0f) : v3(st.thr);
-    if (isNee<MODE>()) c = hv ? st.color : st.color + skyRadiance(P, W.w) * wts;
-    else c = skyRadiance(P, W.w) * wts;
+    if (isNee<MODE>()) c = hv ? st.color : st.color + skyRadiance(skyParams<MODE>(P), W.w) * wts;
+    else c = skyRadiance(skyParams<MODE>(P), W.w) * wts;
     finishPath(st, c, hv);
     return false;
 }
@@ -543,7 +548,7 @@ CRVPT_DEV bool endEvent(PathState& st, Walker& W, const SceneParams& P, const Su
 CRVPT_DEV bool rrtTraversalEnd(PathState& st, Walker& W, const SceneParams& P, const SupervoxelGrid& S) {
     constexpr int MODE = VPT_RESIDUAL_RATIO_TRACKING;
     const uint4 e = pathBlock(P, st, kEventSpace | st.events++);
-    st.color += skyRadiance(P, W.w) * st.thr;
+    st.color += skyRadiance(skyParams<MODE>(P), W.w) * st.thr;
     const uint32_t iteration = st.flags >> kBounceShift;
     if (u01(e.x) > st.wSum || iteration >= 10u) {
         finishPath(st, st.color, true);
@@ -557,7 +562,7 @@ CRVPT_DEV bool rrtTraversalEnd(PathState& st, Walker& W, const SceneParams& P, c
     st.resT = 0.0f;
     st.resDist = 0.0f;
     if (!walkerStart<MODE>(W, P, S, x, nw, kInf)) {
-        st.color += skyRadiance(P, nw) * st.thr;
+        st.color += skyRadiance(skyParams<MODE>(P), nw) * st.thr;
         finishPath(st, st.color, true);
         return false;
     }

@@ -829,18 +829,16 @@ static hipError_t launchRenderT(const SceneParams& P, const Grid& grid, const Su
     return hipGetLastError();
 }
 
-struct Mat16 {
-    float m[16];
-};
-
-__global__ void k_storeMatrix(float* __restrict__ dst, Mat16 m) {
-    if (threadIdx.x < 16) dst[threadIdx.x] = m.m[threadIdx.x];
+__global__ void k_storeParams(SceneParams* __restrict__ dst, SceneParams P) {
+    // word by word from the kernel argument (the struct is a whole number of 4-byte words)
+    static_assert(sizeof(SceneParams) % 4 == 0, "SceneParams is copied in 4-byte words");
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(&P);
+    uint32_t* out = reinterpret_cast<uint32_t*>(dst);
+    for (uint32_t i = threadIdx.x; i < sizeof(SceneParams) / 4; i += blockDim.x) out[i] = src[i];
 }
 
-hipError_t launchStoreMatrix(float* dst, const float* m16, hipStream_t stream) {
-    Mat16 m;
-    for (int i = 0; i < 16; i++) m.m[i] = m16[i];
-    hipLaunchKernelGGL(k_storeMatrix, dim3(1), dim3(64), 0, stream, dst, m);
+hipError_t launchStoreParams(SceneParams* dst, const SceneParams& P, hipStream_t stream) {
+    hipLaunchKernelGGL(k_storeParams, dim3(1), dim3(256), 0, stream, dst, P);
     return hipGetLastError();
 }
 

@@ -63,8 +63,8 @@ struct FbmParams {
 };
 
 // stats (nullable): 64-bit device counters [density taps, supervoxel majorant reads, paths], accumulated.
-// Stores a 4x4 matrix (16 floats) to device memory from a kernel argument (stream-ordered, no host staging).
-hipError_t launchStoreMatrix(float* dst, const float* m16, hipStream_t stream);
+// Stores the launch constants to device memory from a kernel argument (stream-ordered, no host staging).
+hipError_t launchStoreParams(SceneParams* dst, const SceneParams& P, hipStream_t stream);
 // Largest numFrames of one launchRender call at this image size (claim indices stay below 2^31).
 uint32_t maxFramesPerLaunch(int width, int height);
 hipError_t launchRender(const SceneParams& P, const DenseGrid* dense, const NanoGrid* nano, const SupervoxelGrid& S,

@@ -177,9 +177,11 @@ struct SceneParams {
     Div31 divTilesX;
     float widthF, heightF;
     float absorption;
-    // Device copy of invViewProj (16 floats) for the single-frame path kernels' camera rays: read from memory at
-    // path start, so the matrix does not occupy scalar registers across the persistent loop. Set by render_frame.
-    const float* camMatrix;
+    // Device copy of these constants, for branches the path kernels take rarely (camera rays at path start, the sky
+    // at segment ends): read from memory there, those constants do not occupy scalar registers across the persistent
+    // loop. render_frame stores the copy before its launches (its per-launch fields are those of the first launch
+    // and are not read through it); the batched queue points it at the request's own record in the request table.
+    const SceneParams* mem;
 };
 
 // One request of a batched render (the batched queue's device table; no reference counterpart). The scene
