@@ -878,10 +878,10 @@ int crvpt_render_frame(float* out, int channels, int height, int width, int fram
     SceneParams P;
     SupervoxelGrid S;
     fillSceneParams(r, width, height, P, S);
-    // the path kernels read their rarely used constants (camera, sky) from a device copy (SceneParams::mem)
+    // the path kernels read their rarely used constants (camera, sky, claim arithmetic) from a device copy
+    // (SceneParams::mem), which launchRender stores before each launch
     HIP_TRY(r.dParams.ensure(sizeof(SceneParams)));
     P.mem = r.dParams.as<SceneParams>();
-    HIP_TRY(launchStoreParams(r.dParams.as<SceneParams>(), P, s));
     DenseGrid dg;
     NanoGrid ng;
     bool sp;

@@ -114,7 +114,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(poolWaves<M
             base = __builtin_amdgcn_readfirstlane(__shfl(base, leader));  // wave-uniform: the range state in SGPRs
             if (needs) {
                 const uint32_t c = base + uint32_t(__popcll(needMask & ((1ull << lane) - 1ull)));
-                if (c < count && claimToSlot(P, begin + c, st.unit)) {
+                if (c < count && claimToSlot(launchParams<MODE>(P), begin + c, st.unit)) {
                     paths++;
                     if (!startPath<MODE>(st, W, P, S, feat)) writePath(out, st);
                 }
@@ -853,11 +853,14 @@ uint32_t maxFramesPerLaunch(int width, int height) {
 hipError_t launchRender(const SceneParams& Pin, const DenseGrid* dense, const NanoGrid* nano, const SupervoxelGrid& S,
                         float4* paths, uint32_t* counter, const LaunchConfig& cfg, unsigned long long* stats,
                         float4* feat, hipStream_t stream) {
-    if (Pin.numFrames == 0u || Pin.numFrames > maxFramesPerLaunch(Pin.width, Pin.height)) return hipErrorInvalidValue;
+    if (Pin.numFrames == 0u || Pin.numFrames > maxFramesPerLaunch(Pin.width, Pin.height) || !Pin.mem)
+        return hipErrorInvalidValue;
     SceneParams P = Pin;
     const uint32_t tp = kClaimTile == 0u ? 1u : kClaimTile * kClaimTile;
     P.divTileClaims = makeDiv31(tp * P.numFrames);
     P.divTilesX = makeDiv31(kClaimTile == 0u ? 1u : (uint32_t(P.width) + kClaimTile - 1u) / kClaimTile);
+    // this launch's constants, with its frame range and claim divisors, in the device copy the kernels read
+    if (const hipError_t e = launchStoreParams(const_cast<SceneParams*>(Pin.mem), P, stream)) return e;
     // variants: 64-bit offsets and the full index map only for grids of 4 GiB and more or with rotated voxels; the
     // trilinear code only for the trilinear filter
     const bool tri = P.interpolation == INTERP_TRILINEAR;

@@ -177,10 +177,10 @@ struct SceneParams {
     Div31 divTilesX;
     float widthF, heightF;
     float absorption;
-    // Device copy of these constants, for branches the path kernels take rarely (camera rays at path start, the sky
-    // at segment ends): read from memory there, those constants do not occupy scalar registers across the persistent
-    // loop. render_frame stores the copy before its launches (its per-launch fields are those of the first launch
-    // and are not read through it); the batched queue points it at the request's own record in the request table.
+    // Device copy of these constants, for branches the path kernels take rarely (claims and camera rays at path
+    // start, the sky at segment ends): read from memory there, those constants do not occupy scalar registers across
+    // the persistent loop. The single-frame launcher stores the copy before each launch; the batched queue points it
+    // at the request's own record in the request table.
     const SceneParams* mem;
 };
 
