@@ -38,10 +38,11 @@ constexpr float kInf = 3.0e38f;
 // more lanes idle while the others cross), and the waves per SIMD the pool kernels are compiled for (128 registers
 // per lane at 4 waves, 96 at 5, 80 at 6). Next-event tracking carries the most per-lane state (the shadow ray's):
 // spectral next-event tracking runs best at 4 waves, next-event and spectral delta tracking at 5 (dense 128^3
-// next-event workload: 51.5 ms against 54.7 ms at 4 waves), the others at 6 (residual ratio tracking: 351 ms against
-// 388 ms at 5 waves per 2^28-path launch); residual ratio and next-event tracking at 2 crossings. The batched
-// queue's kernel also holds its request's constants per lane and keeps next-event tracking at 4 waves, residual
-// ratio and decomposition tracking at 5 (batchPoolWaves).
+// next-event workload: 51.5 ms against 54.7 ms at 4 waves), residual ratio and decomposition tracking at 6 (residual
+// ratio: 351 ms against 388 ms at 5 waves and 410 ms at 7 per 2^28-path launch), delta and ratio tracking at 7 (66.4
+// and 124.0 ms against 67.4 and 127.5 ms at 6); residual ratio and next-event tracking at 2 crossings, the others at
+// 3. The batched queue's kernel also holds its request's constants per lane and keeps next-event tracking at 4 waves,
+// residual ratio and decomposition tracking at 5, delta and ratio tracking at 6 (batchPoolWaves).
 template <int MODE>
 constexpr int maxCrossings() {
     return (MODE == VPT_RESIDUAL_RATIO_TRACKING || MODE == VPT_NEXT_EVENT_TRACKING ||
@@ -51,6 +52,7 @@ template <int MODE>
 constexpr int poolWaves() {
     if (MODE == VPT_NEXT_EVENT_TRACKING_SPECTRAL) return 4;
     if (MODE == VPT_NEXT_EVENT_TRACKING) return 5;
+    if (MODE == VPT_DELTA_TRACKING || MODE == VPT_RATIO_TRACKING) return 7;
     return MODE == VPT_SPECTRAL_DELTA_TRACKING ? 5 : 6;
 }
 
@@ -58,6 +60,7 @@ template <int MODE>
 constexpr int batchPoolWaves() {
     if (MODE == VPT_NEXT_EVENT_TRACKING) return 4;
     if (MODE == VPT_RESIDUAL_RATIO_TRACKING || MODE == VPT_DECOMPOSITION_TRACKING) return 5;
+    if (MODE == VPT_DELTA_TRACKING || MODE == VPT_RATIO_TRACKING) return 6;
     return poolWaves<MODE>();
 }
 
