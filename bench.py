@@ -219,7 +219,7 @@ def run_c3(args, vpt, torch):
     print(json.dumps(line), flush=True)
 
 
-C4 = {"crops": 4096, "crop": 128, "clouds": 16, "input_spp": 4, "mixed_input_spp": [4, 16, 64, 256],
+C4 = {"crops": 4096, "crop": 128, "clouds": 16, "input_spp": 4, "mixed_input_spp": [4, 16, 32, 48, 64, 256],
       "eval_input_spp": [4, 16, 32, 48, 64, 128, 256], "target_spp": 1024, "batch": 16, "train_steps": 6000, "lr": 1e-3,
       "eval_res": 1024, "ref_spp": 16384, "plain_spp": [4, 16, 64, 256, 1024, 4096], "max_bounces": 128,
       "eaw_iterations": 4, "svgf_max_frames": 256}
@@ -266,8 +266,8 @@ def _train(torch, net, feats, noisy, target, steps, gen, log):
 def run_c4(args, vpt, torch, emit=True):
     """Kernel-prediction denoiser at the C4 scale: 4096 seeded 128^2 crops of 16 fBm clouds, each with a 1024-spp
     target (all targets in one batched launch) and two inputs with the tracer's feature maps: 4 spp (network A, the
-    4-spp -> 1024-spp training of the configuration) and 4, 16, 64 or 256 spp (network B, the same crops and
-    targets). Evaluation on an unseen cloud and camera at 1024^2, judged against a 16k-spp render:
+    4-spp -> 1024-spp training of the configuration) and 4, 16, 32, 48, 64 or 256 spp in turn (network B, the same
+    crops and targets). Evaluation on an unseen cloud and camera at 1024^2, judged against a 16k-spp render:
       * equal time: network A on a 4-spp render against EAW (4 a-trous iterations on the same 4-spp render), SVGF (1-spp
         frames of the static view accumulated by its temporal filter for the same wall time) and plain tracking at
         the same wall time (interpolated on the plain curve);
