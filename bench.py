@@ -279,6 +279,9 @@ def run_c4(args, vpt, torch, emit=True):
 
     crops = args.kpn_crops or C4["crops"]
     cs = C4["crop"]
+    # the renderer's default launch configuration (majorant supervoxels, workgroups, refill interval), whatever a
+    # workload run before this one in the same process set
+    vpt.set_launch_config(0, 0, 0)
     featsA = torch.zeros((crops, 15, cs, cs), dtype=torch.float32, device="cuda")
     noisyA = torch.zeros((crops, 3, cs, cs), dtype=torch.float32, device="cuda")
     featsB = torch.zeros((crops, 15, cs, cs), dtype=torch.float32, device="cuda")
