@@ -548,6 +548,8 @@ int prepareScene(Renderer& r) {
         const int* ad = activeDims(r);
         for (int a = 0; a < 3; a++) r.svGrid[a] = (ad[a] + r.svSize - 1) / r.svSize;
         const size_t n = size_t(r.svGrid[0]) * r.svGrid[1] * r.svGrid[2];
+        // the path kernels index cells with 32-bit arithmetic
+        if (n >= (size_t(1) << 32)) return fail(CRVPT_ERR_INVALID_ARGUMENT, "majorant grid of 2^32 cells or more");
         HIP_TRY(r.dMajorant.ensure(n * sizeof(float)));
         HIP_TRY(launchMajorant(sp ? nullptr : &dg, sp ? &ng : nullptr, ad, r.svSize, r.svGrid,
                                r.dMajorant.as<float>(), r.stream));
@@ -567,6 +569,8 @@ int prepareScene(Renderer& r) {
         r.refSvSize = sv;
         for (int a = 0; a < 3; a++) r.refGrid[a] = (ad[a] + sv - 1) / sv;
         const size_t n = size_t(r.refGrid[0]) * r.refGrid[1] * r.refGrid[2];
+        // the path kernels index cells with 32-bit arithmetic
+        if (n >= (size_t(1) << 32)) return fail(CRVPT_ERR_INVALID_ARGUMENT, "supervoxel grid of 2^32 cells or more");
         HIP_TRY(r.dSvMin.ensure(n * sizeof(float)));
         HIP_TRY(r.dSvMax.ensure(n * sizeof(float)));
         HIP_TRY(r.dSvAvg.ensure(n * sizeof(float)));

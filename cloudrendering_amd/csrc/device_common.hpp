@@ -260,7 +260,8 @@ struct NoCache {
 };
 
 // TRI: the trilinear filter is compiled in; the path kernels run TRI = false for the point filters (nearest,
-// stochastic), where the eight-tap code would only hold registers.
+// stochastic), where the eight-tap code would only hold registers. The TRI = false variant also indexes voxels with
+// 32-bit arithmetic: the launcher uses it only for grids below 2^32 voxels.
 template <bool TRI>
 struct DenseAccessT {
     DenseGrid g;
@@ -269,6 +270,7 @@ struct DenseAccessT {
         if ((unsigned)i >= (unsigned)g.dim[0] || (unsigned)j >= (unsigned)g.dim[1] ||
             (unsigned)k >= (unsigned)g.dim[2])
             return g.borderValue;
+        if (!TRI) return g.values[(uint32_t(k) * uint32_t(g.dim[1]) + uint32_t(j)) * uint32_t(g.dim[0]) + uint32_t(i)];
         return g.values[(size_t(k) * g.dim[1] + j) * g.dim[0] + i];
     }
     CRVPT_DEV float voxelAt(int i, int j, int k) const { return voxel(i, j, k); }

@@ -861,14 +861,17 @@ hipError_t launchRender(const SceneParams& Pin, const DenseGrid* dense, const Na
     P.divTilesX = makeDiv31(kClaimTile == 0u ? 1u : (uint32_t(P.width) + kClaimTile - 1u) / kClaimTile);
     // this launch's constants, with its frame range and claim divisors, in the device copy the kernels read
     if (const hipError_t e = launchStoreParams(const_cast<SceneParams*>(Pin.mem), P, stream)) return e;
-    // variants: 64-bit offsets and the full index map only for grids of 4 GiB and more or with rotated voxels; the
-    // trilinear code only for the trilinear filter
+    // variants: 64-bit offsets and the full index map only for NanoVDB grids of 4 GiB and more or with rotated
+    // voxels, 64-bit voxel indices only for dense grids of 2^32 voxels and more; the trilinear code only for the
+    // trilinear filter
     const bool tri = P.interpolation == INTERP_TRILINEAR;
     if (nano && (nano->wide || !nano->tapDiag))
         return launchRenderT(P, NanoAccess{*nano}, S, paths, counter, cfg, stats, feat, stream);
     if (nano && tri) return launchRenderT(P, NanoAccessNarrow{*nano}, S, paths, counter, cfg, stats, feat, stream);
     if (nano) return launchRenderT(P, NanoAccessNarrowPoint{*nano}, S, paths, counter, cfg, stats, feat, stream);
-    if (!tri) return launchRenderT(P, DenseAccessPoint{*dense}, S, paths, counter, cfg, stats, feat, stream);
+    const bool small =
+            uint64_t(dense->dim[0]) * uint64_t(dense->dim[1]) * uint64_t(dense->dim[2]) < (uint64_t(1) << 32);
+    if (!tri && small) return launchRenderT(P, DenseAccessPoint{*dense}, S, paths, counter, cfg, stats, feat, stream);
     return launchRenderT(P, DenseAccess{*dense}, S, paths, counter, cfg, stats, feat, stream);
 }
 
