@@ -69,11 +69,15 @@ __global__ __launch_bounds__(256) void k_conv3x3(const __bf16* __restrict__ in, 
         B[t][0] = wsw[(t * 2 + 0) * 64 + lane];
         B[t][1] = wsw[(t * 2 + 1) * 64 + lane];
     }
-    float bco[2] = {0.0f, 0.0f};
-    if (EPI != EPI_MASK_BF16) {
-        bco[0] = bias[m];
-        bco[1] = bias[16 + m];
-    }
+    // The MFMA takes the weights as its A operand and the pixels as B, so the 16x16 result is channels x pixels:
+    // lane (m, kb) holds output channels nt * 16 + kb * 4 + 0..3 of pixel m of the tile row, four consecutive
+    // channels that leave in one 8-byte (bf16) or 16-byte (f32) store.
+    float bco[2][4] = {};
+    if (EPI != EPI_MASK_BF16)
+#pragma unroll
+        for (int nt = 0; nt < 2; nt++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) bco[nt][i] = bias[nt * 16 + kb * 4 + i];
     const int tilesX = (W + kTile - 1) / kTile, tilesY = (H + kTile - 1) / kTile;
     const int numTiles = N * tilesX * tilesY;
     for (int ti = blockIdx.x; ti < numTiles; ti += gridDim.x) {
@@ -95,31 +99,38 @@ __global__ __launch_bounds__(256) void k_conv3x3(const __bf16* __restrict__ in, 
                 const int row = wave * 4 + mt;
                 const bf16x8 A =
                     *reinterpret_cast<const bf16x8*>(tile + ((row + hy) * kHalo + (m + hx)) * kPixStride + kb * 8);
-                acc[mt][0] = mfma16(A, B[t][0], acc[mt][0]);
-                acc[mt][1] = mfma16(A, B[t][1], acc[mt][1]);
+                acc[mt][0] = mfma16(B[t][0], A, acc[mt][0]);
+                acc[mt][1] = mfma16(B[t][1], A, acc[mt][1]);
             }
         }
+        const int x = x0 + m;
 #pragma unroll
         for (int mt = 0; mt < 4; mt++) {
             const int y = y0 + wave * 4 + mt;
-            if (y >= H) continue;
+            if (y >= H || x >= W) continue;
+            const size_t p = (size_t(n) * H + y) * W + x;
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int x = x0 + kb * 4 + i;
-                if (x >= W) continue;
-                const size_t p = (size_t(n) * H + y) * W + x;
-#pragma unroll
-                for (int nt = 0; nt < 2; nt++) {
-                    const int co = nt * 16 + m;
-                    const float v = acc[mt][nt][i];
+            for (int nt = 0; nt < 2; nt++) {
+                const size_t o = p * kC + nt * 16 + kb * 4;
+                const f32x4 v = acc[mt][nt];
+                if (EPI == EPI_LINEAR_F32) {
+                    *reinterpret_cast<float4*>(reinterpret_cast<float*>(out) + o) =
+                        make_float4(v[0] + bco[nt][0], v[1] + bco[nt][1], v[2] + bco[nt][2], v[3] + bco[nt][3]);
+                } else {
+                    __bf16 r[4];
                     if (EPI == EPI_RELU_BF16) {
-                        reinterpret_cast<__bf16*>(out)[p * kC + co] = (__bf16)fmaxf(v + bco[nt], 0.0f);
-                    } else if (EPI == EPI_LINEAR_F32) {
-                        reinterpret_cast<float*>(out)[p * kC + co] = v + bco[nt];
+#pragma unroll
+                        for (int i = 0; i < 4; i++) r[i] = (__bf16)fmaxf(v[i] + bco[nt][i], 0.0f);
                     } else {
-                        const bool on = (float)mask[p * kC + co] > 0.0f;
-                        reinterpret_cast<__bf16*>(out)[p * kC + co] = (__bf16)(on ? v : 0.0f);
+                        const uint2 mw = *reinterpret_cast<const uint2*>(mask + o);
+                        __bf16 mv[4];
+                        __builtin_memcpy(mv, &mw, sizeof(mw));
+#pragma unroll
+                        for (int i = 0; i < 4; i++) r[i] = (__bf16)((float)mv[i] > 0.0f ? v[i] : 0.0f);
                     }
+                    uint2 w;
+                    __builtin_memcpy(&w, r, sizeof(w));
+                    *reinterpret_cast<uint2*>(reinterpret_cast<__bf16*>(out) + o) = w;
                 }
             }
         }
